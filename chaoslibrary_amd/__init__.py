@@ -1,0 +1,188 @@
+"""chaoslibrary_amd -- MI355X (gfx950) ChaosLibrary hot path over a C-ABI.
+
+This module binds ``libchaos.so`` (hand-written HIP kernels + C-ABI, built by
+``__graft_entry__.build()`` / ``make -C chaoslibrary_amd/csrc``) with ctypes
+and mirrors the reference's C++ host API names:
+
+    bifurcation1D, bifurcation2D   <- Bifurcation/include/bifurcationHOST.h:16-50
+    LLE2D                          <- LLE/include/LLEHost.h:25-41
+    basinsOfAttraction_2           <- Basins/include/basinsHOST.h:15-30
+
+plus value-returning variants (``bif1d``, ``bif2d``, ``lle2d``, ``basins2d``,
+``integrate``).  There is no CPU fallback: every call runs the HIP kernels,
+and importing fails loudly when the library is missing.  Errors raise
+``ChaosError`` (the reference prints and exits, debri/cudaMacros.cu:3-10).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = [
+    "ChaosError", "Options", "Stats", "SYSTEMS", "SEM_REFERENCE", "SEM_CONFIG", "lib", "LIB_PATH",
+    "system_dims", "integrate", "lle2d", "LLE2D", "bif1d", "bifurcation1D", "bif2d",
+    "bifurcation2D", "basins2d", "basinsOfAttraction_2", "device_count",
+]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libchaos.so")
+
+SYSTEMS = {"chameleon": 0, "rossler_cd": 1, "basins": 2, "basins_2": 3, "damir": 4,
+           "rossler_rk4": 5, "lorenz_rk4": 6}
+SEM_REFERENCE, SEM_CONFIG = 0, 1
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+
+
+class ChaosError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Stats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("analysis_ms", C.c_double), ("total_ms", C.c_double),
+                ("rk4_steps", C.c_double), ("peaks", C.c_longlong), ("launches", C.c_int),
+                ("reserved", C.c_int)]
+
+
+class Options(C.Structure):
+    _fields_ = [("system", C.c_int), ("semantics", C.c_int), ("seed", C.c_uint64),
+                ("device", C.c_int), ("device_buffers", C.c_int), ("stream", C.c_void_p),
+                ("min_pts", C.c_int), ("block", C.c_int), ("stats", C.POINTER(Stats))]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"chaoslibrary_amd: {LIB_PATH} is missing; build it with "
+                          "`python -c 'import __graft_entry__ as g; g.build()'`")
+    return C.CDLL(LIB_PATH)
+
+
+lib = _load()
+
+_SIGS = {
+    "chaos_options_init": (None, [C.POINTER(Options)]),
+    "chaos_api_version": (C.c_int, []),
+    "chaos_last_error": (C.c_char_p, []),
+    "chaos_system_dims": (C.c_int, [C.c_int, _I, _I]),
+    "chaos_set_default_options": (C.c_int, [C.POINTER(Options)]),
+    "chaos_device_count": (C.c_int, [_I]),
+    "chaos_device_malloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t, C.c_int]),
+    "chaos_device_free": (C.c_int, [C.c_void_p]),
+    "chaos_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "chaos_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "chaos_device_synchronize": (C.c_int, [C.c_int]),
+    "chaos_integrate": (C.c_int, [C.POINTER(Options), C.c_int, _D, _D, C.c_longlong, C.c_double]),
+    "chaos_lle2d": (C.c_int, [C.POINTER(Options), C.c_double, C.c_double, C.c_int, C.c_double,
+                              C.c_double, _D, C.c_int, _D, _I, C.c_int, C.c_double, C.c_double, _D,
+                              C.c_int, C.c_void_p]),
+    "chaos_LLE2D": (C.c_int, [C.POINTER(Options), C.c_double, C.c_double, C.c_int, C.c_double,
+                              C.c_double, _D, C.c_int, _D, _I, C.c_int, C.c_double, C.c_double, _D,
+                              C.c_int, C.c_char_p]),
+}
+
+
+def _bind():
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+_bind()
+
+
+def _check(rc):
+    if rc != 0:
+        raise ChaosError(rc, lib.chaos_last_error().decode(errors="replace"))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_I)
+
+
+def make_options(system="chameleon", semantics=SEM_REFERENCE, seed=12345, device=0, block=0,
+                 min_pts=20, stats=None, device_buffers=False):
+    o = Options()
+    lib.chaos_options_init(C.byref(o))
+    o.system = SYSTEMS[system] if isinstance(system, str) else int(system)
+    o.semantics = int(semantics)
+    o.seed = int(seed)
+    o.device = int(device)
+    o.block = int(block)
+    o.min_pts = int(min_pts)
+    o.device_buffers = 1 if device_buffers else 0
+    if stats is not None:
+        o.stats = C.pointer(stats)
+    return o
+
+
+def system_dims(system):
+    nx, na = C.c_int(), C.c_int()
+    sid = SYSTEMS[system] if isinstance(system, str) else int(system)
+    _check(lib.chaos_system_dims(sid, C.byref(nx), C.byref(na)))
+    return nx.value, na.value
+
+
+def device_count():
+    c = C.c_int()
+    _check(lib.chaos_device_count(C.byref(c)))
+    return c.value
+
+
+def integrate(system, params, states, nsteps, h, **kw):
+    """Advance states[n][NX] by nsteps of the system map; returns the new states."""
+    o = make_options(system, **kw)
+    params = _f64(params)
+    states = _f64(states).copy()
+    n = states.shape[0]
+    _check(lib.chaos_integrate(C.byref(o), n, _dp(params), _dp(states), int(nsteps), float(h)))
+    return states
+
+
+def lle2d(tMax, NT, nPts, h, eps, initialConditions, ranges, indicesOfMutVars, transientTime,
+          values, writableVar=0, maxValue=0.0, system="chameleon", semantics=SEM_REFERENCE,
+          out_device_ptr=None, **kw):
+    """LLE map; returns out[j, i] = LLE(A_i, B_j) (row = second swept parameter)."""
+    o = make_options(system, semantics, device_buffers=out_device_ptr is not None, **kw)
+    ic, rg, ix, vals = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
+    if out_device_ptr is None:
+        out = np.empty((nPts, nPts), dtype=np.float64)
+        ptr = out.ctypes.data
+    else:
+        out, ptr = None, out_device_ptr
+    _check(lib.chaos_lle2d(C.byref(o), tMax, NT, nPts, h, eps, _dp(ic), ic.size, _dp(rg), _ip(ix),
+                           writableVar, maxValue, transientTime, _dp(vals), vals.size, C.c_void_p(ptr)))
+    return out
+
+
+def LLE2D(tMax, NT, nPts, h, eps, initialConditions, amountOfInitialConditions, ranges,
+          indicesOfMutVars, writableVar, maxValue, transientTime, values, amountOfValues,
+          OUT_FILE_PATH, options=None):
+    """LLE::LLE2D (LLE/include/LLEHost.h:25-41): writes the reference CSV."""
+    ic, rg, ix, vals = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
+    o = options if options is not None else None
+    _check(lib.chaos_LLE2D(C.byref(o) if o is not None else None, tMax, NT, nPts, h, eps, _dp(ic),
+                           amountOfInitialConditions, _dp(rg), _ip(ix), writableVar, maxValue,
+                           transientTime, _dp(vals), amountOfValues, str(OUT_FILE_PATH).encode()))
+
+
+def bif1d(*a, **k):  # bound once the bifurcation module is built
+    raise ChaosError(-7, "bifurcation entry points not bound")
+
+
+bifurcation1D = bif2d = bifurcation2D = basins2d = basinsOfAttraction_2 = bif1d

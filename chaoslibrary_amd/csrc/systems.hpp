@@ -1,0 +1,299 @@
+// systems.hpp -- per-point step maps for gfx950, one lane per trajectory.
+//
+// Each system is a compile-time struct with NX (state size), NA (parameter
+// count) and a step() that advances X in place, the runtime counterpart of
+// the reference's compile-time plug-in (systems/systems.cuh:9, one
+// USE_*_MODEL per build).  Kernels are instantiated per system and chosen by
+// chaos_system id at launch.  State and parameters live in VGPRs/SGPRs.
+//
+// Floating point: the library is built with -ffp-contract=off, so every
+// product and sum rounds separately unless written as fma(); the fma()
+// placements are the reference's __fma_rn calls (systems/systems.cuh:26-58)
+// and, for RK4, the canonical order documented in RK4 below.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace chaos {
+
+enum SystemId : int {
+    SYS_CHAMELEON = 0,
+    SYS_ROSSLER_CD = 1,
+    SYS_BASINS = 2,
+    SYS_BASINS_2 = 3,
+    SYS_DAMIR = 4,
+    SYS_ROSSLER_RK4 = 5,
+    SYS_LORENZ_RK4 = 6,
+    SYS_COUNT = 7
+};
+
+// Step constants derived once per launch from h (identical IEEE operations
+// on every lane and in the host oracle).
+struct StepConst {
+    double h;    // step
+    double hh;   // 0.5 * h
+    double h6;   // h / 6.0
+};
+
+__host__ __device__ inline StepConst make_step_const(double h)
+{
+    StepConst c;
+    c.h = h;
+    c.hh = 0.5 * h;
+    c.h6 = h / 6.0;
+    return c;
+}
+
+template <int S>
+struct System;
+
+// Waves per SIMD requested from the register allocator for a system's
+// integration kernels: the RK4 systems fit the 64-VGPR budget of 8 waves
+// without spilling; the transcendental maps use the compiler's choice.
+template <int S>
+struct Occupancy {
+    static constexpr int waves = 0;
+};
+template <>
+struct Occupancy<5> {
+    static constexpr int waves = 8;
+};
+template <>
+struct Occupancy<6> {
+    static constexpr int waves = 8;
+};
+
+// systems/systems.cuh:11-39 -- Chameleon, two-stage composition with cos().
+template <>
+struct System<SYS_CHAMELEON> {
+    static constexpr int NX = 3, NA = 7;
+    __device__ __forceinline__ static void step(double* X, const double* a, const StepConst& c)
+    {
+        const double h1 = a[0] * c.h;
+        const double h2 = (1 - a[0]) * c.h;
+        X[0] = fma(h1, -a[6] * X[1], X[0]);
+        X[1] = fma(h1, a[6] * X[0] + a[1] * X[2], X[1]);
+        const double cos_term = cos(a[5] * X[1]);
+        X[2] = fma(h1, a[2] - a[3] * X[2] + a[4] * cos_term, X[2]);
+        X[2] = fma(h2, (a[2] + a[4] * cos_term), X[2]) / (1 + a[3] * h2);
+        X[1] = fma(h2, (a[6] * X[0] + a[1] * X[2]), X[1]);
+        X[0] = fma(h2, -a[6] * X[1], X[0]);
+    }
+};
+
+// systems/systems.cuh:41-63 -- Rossler, semi-implicit CD scheme
+// (a[0] symmetry parameter, a[1]=a, a[2]=b, a[3]=c).
+template <>
+struct System<SYS_ROSSLER_CD> {
+    static constexpr int NX = 3, NA = 4;
+    __device__ __forceinline__ static void step(double* x, const double* a, const StepConst& c)
+    {
+        const double h1 = 0.5 * c.h + a[0];
+        const double h2 = 0.5 * c.h - a[0];
+        x[0] = fma(h1, -x[1] - x[2], x[0]);
+        x[1] = fma(h1, x[0] + a[1] * x[1], x[1]);
+        x[2] = fma(h1, a[2] + x[2] * (x[0] - a[3]), x[2]);
+        double temp = fma(-h2, x[0] - a[3], 1.0);
+        x[2] = fma(h2, a[2], x[2]) / temp;
+        temp = fma(-h2, a[1], 1.0);
+        x[1] = fma(h2, x[0], x[1]) / temp;
+        x[0] = fma(h2, -x[1] - x[2], x[0]);
+    }
+};
+
+// systems/systems.cuh:67-83 -- the reference narrows h2 to float.
+template <>
+struct System<SYS_BASINS> {
+    static constexpr int NX = 3, NA = 5;
+    __device__ __forceinline__ static void step(double* X, const double* a, const StepConst& c)
+    {
+        const double h = c.h;
+        const double h2 = (double)(float)(h * (1 - a[0]));
+        X[0] = X[0] + h * (sin(X[1]) - a[1] * X[0]);
+        X[1] = X[1] + h * (sin(X[2]) - a[1] * X[1]);
+        X[2] = X[2] + h * (sin(X[0]) - a[1] * X[2]);
+        X[2] = (X[2] + h2 * sin(X[0])) / (1 + h2 * a[1]);
+        X[1] = (X[1] + h2 * sin(X[2])) / (1 + h2 * a[1]);
+        X[0] = (X[0] + h2 * sin(X[1])) / (1 + h2 * a[1]);
+    }
+};
+
+// systems/systems.cuh:85-101 -- h1, h2 and the intermediate z are floats.
+template <>
+struct System<SYS_BASINS_2> {
+    static constexpr int NX = 2, NA = 2;
+    __device__ __forceinline__ static void step(double* X, const double* a, const StepConst& c)
+    {
+        const double h1 = (double)(float)(c.h * a[0]);
+        const double h2 = (double)(float)(c.h * (1 - a[0]));
+        X[0] = X[0] + h1 * (-X[1]);
+        X[1] = X[1] + h1 * (a[1] * X[0] + sin(X[1]));
+        const double z = (double)(float)X[1];
+        X[1] = z + h2 * (a[1] * X[0] + sin(X[1]));
+        X[0] = X[0] + h2 * (-X[1]);
+    }
+};
+
+// systems/systems.cuh:103-178 -- tunnel-diode / memristor circuit, explicit
+// Euler, time carried in X[2].
+template <>
+struct System<SYS_DAMIR> {
+    static constexpr int NX = 3, NA = 2;
+    __device__ __forceinline__ static void step(double* X, const double* a, const StepConst& c)
+    {
+        const double h = c.h;
+        const double Vc = X[0], XSV = X[1], t = X[2];
+        const double U1 = 0.0, U2 = -0.04, C1 = 22e-9;
+        const double Is = 1.15E-7, Vt = 1.0 / 15.0, Vp = 0.09, Ip = 2.1e-5;
+        const double Iv = -3e-6, D = 26.0, E = 0.14;
+        const double Vd = Vc + U2;
+        const double Vm = Vc + U1;
+        const double idiode = Is * (exp(Vd / Vt) - exp(-Vd / Vt));
+        const double itunnel = Ip / Vp * Vd * exp(-(Vd - Vp) / Vp);
+        const double iex = Iv * (atan(D * (Vd - E)) + atan(D * (Vd + E)));
+        const double Id = idiode + itunnel + iex;
+        const double Ron = 1434.0, Roff = 1e6, Von1 = 0.28, Voff1 = 0.14;
+        const double Von2 = -0.12, Voff2 = -0.006, TAU = 1e-7, T = 2.0;
+        const double boltz = 1.380649e-23, echarge = 1.602176634e-19;
+        const double g = XSV / Ron + (1.0 - XSV) / Roff;
+        const double arg1 = -1.0 / (T * boltz / echarge) * (Vm - Von1) * (Vm - Von2);
+        const double arg2 = -1.0 / (T * boltz / echarge) * (Vm - Voff2) * (Vm - Voff1);
+        const double f1 = 1.0 / (1.0 + exp(arg1));
+        const double f2 = 1.0 / (1.0 + exp(arg2));
+        const double Ix = (1.0 / TAU) * (f1 * (1.0 - XSV) - (1.0 - f2) * XSV);
+        const double Imem = Vm * g;
+        const double Iin = a[0] * sin(a[1] * t);
+        const double dVc = (Iin - Imem - Id) / C1;
+        X[0] = Vc + h * dVc;
+        X[1] = XSV + h * Ix;
+        X[2] = t + h;
+    }
+};
+
+// Classic RK4 in one canonical operation order (shared with the oracle):
+//   k1 = f(X); Y = fma(h/2,k1,X); k2 = f(Y); Y = fma(h/2,k2,X); k3 = f(Y);
+//   Y = fma(h,k3,X); k4 = f(Y);
+//   s = fma(2,k2,k1); s = fma(2,k3,s); s = s + k4; X = fma(h/6,s,X).
+// The weighted sum is accumulated as the stages complete, so only X, Y, k
+// and s are live: 12 doubles per trajectory.
+template <class F>
+__device__ __forceinline__ void rk4_step(double* X, const double* a, const StepConst& c)
+{
+    double k[3], Y[3], s[3];
+    F::rhs(X, a, k);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        s[i] = k[i];
+        Y[i] = fma(c.hh, k[i], X[i]);
+    }
+    F::rhs(Y, a, k);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        s[i] = fma(2.0, k[i], s[i]);
+        Y[i] = fma(c.hh, k[i], X[i]);
+    }
+    F::rhs(Y, a, k);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        s[i] = fma(2.0, k[i], s[i]);
+        Y[i] = fma(c.h, k[i], X[i]);
+    }
+    F::rhs(Y, a, k);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        s[i] = s[i] + k[i];
+        X[i] = fma(c.h6, s[i], X[i]);
+    }
+}
+
+// Rossler x'=-y-z, y'=x+a y, z'=b+z(x-c); a[0]=a, a[1]=b, a[2]=c.
+// 4 VALU ops per evaluation (the negation folds into its consumers).
+template <>
+struct System<SYS_ROSSLER_RK4> {
+    static constexpr int NX = 3, NA = 3;
+    __device__ __forceinline__ static void rhs(const double* x, const double* a, double* f)
+    {
+        f[0] = -(x[1] + x[2]);
+        f[1] = fma(a[0], x[1], x[0]);
+        f[2] = fma(x[2], x[0] - a[2], a[1]);
+    }
+    __device__ __forceinline__ static void step(double* X, const double* a, const StepConst& c)
+    {
+        rk4_step<System>(X, a, c);
+    }
+};
+
+// Lorenz x'=s(y-x), y'=x(r-z)-y, z'=xy-b z; a[0]=sigma, a[1]=rho, a[2]=beta.
+template <>
+struct System<SYS_LORENZ_RK4> {
+    static constexpr int NX = 3, NA = 3;
+    __device__ __forceinline__ static void rhs(const double* x, const double* a, double* f)
+    {
+        f[0] = a[0] * (x[1] - x[0]);
+        f[1] = fma(x[0], a[1] - x[2], -x[1]);
+        f[2] = fma(x[0], x[1], -(a[2] * x[2]));
+    }
+    __device__ __forceinline__ static void step(double* X, const double* a, const StepConst& c)
+    {
+        rk4_step<System>(X, a, c);
+    }
+};
+
+// Grid value start + k*((end-start)/(n-1)), product and sum rounded
+// separately (Bifurcation/src/bifurcationCUDA.cu:400-419,
+// LLE/src/LLECUDA.cu:172-203; the host CSV writer uses the same form).
+__host__ __device__ inline double grid_value(int k, int n, double start, double end)
+{
+    if (n == 1) return start;
+    const double scale = (end - start) / (n - 1);
+    return start + k * scale;
+}
+
+// Parameter vector with up to two swept entries, written without dynamic
+// register indexing (idxA/idxB are launch-uniform).
+template <int NA>
+__device__ __forceinline__ void build_params(double* a, const double* values, int idxA, double va,
+                                             int idxB, double vb)
+{
+#pragma unroll
+    for (int k = 0; k < NA; ++k) a[k] = (k == idxA) ? va : ((k == idxB) ? vb : values[k]);
+}
+
+// SplitMix64 (Steele, Lea & Flood 2014) -- seeded per point for the
+// Benettin perturbation direction.
+__host__ __device__ inline unsigned long long splitmix64(unsigned long long& state)
+{
+    unsigned long long z = (state += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+// Uniform direction on the unit sphere by rejection from the cube [-1,1)^NX
+// (exact 53-bit uniforms, 1e-6 < r2 <= 1, at most 64 tries), then division
+// by sqrt(r2).  Replaces curand XORWOW at LLE/src/LLECUDA.cu:88-101.
+template <int NX>
+__device__ inline void unit_vector(unsigned long long seed, long long point, double* v)
+{
+    unsigned long long st = seed ^ ((unsigned long long)(point + 1) * 0xD1B54A32D192ED03ULL);
+    for (int tr = 0; tr < 64; ++tr) {
+        double r2 = 0;
+#pragma unroll
+        for (int i = 0; i < NX; ++i) {
+            const unsigned long long z = splitmix64(st);
+            v[i] = (double)((long long)(z >> 10) - (1LL << 53)) * 0x1p-53;
+        }
+#pragma unroll
+        for (int i = 0; i < NX; ++i) r2 = r2 + v[i] * v[i];
+        if (r2 > 1e-6 && r2 <= 1.0) {
+            const double norm = sqrt(r2);
+#pragma unroll
+            for (int i = 0; i < NX; ++i) v[i] = v[i] / norm;
+            return;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NX; ++i) v[i] = i == 0 ? 1.0 : 0.0;
+}
+
+}  // namespace chaos

@@ -1,0 +1,206 @@
+/*
+ * chaoslibrary.h -- C-ABI of the MI355X (gfx950) ChaosLibrary hot path.
+ *
+ * The reference exposes a C++ host API only (no FFI).  Each file-writing
+ * entry point below replaces one of its C++ functions with the same argument
+ * list, in the same order, plus a leading `const chaos_options*` (NULL =
+ * defaults) and a `const char*` path instead of std::string:
+ *
+ *   chaos_bifurcation1D        <- Bifurcation::bifurcation1D
+ *                                 (Bifurcation/include/bifurcationHOST.h:16-30)
+ *   chaos_bifurcation2D        <- Bifurcation::bifurcation2D
+ *                                 (Bifurcation/include/bifurcationHOST.h:35-50)
+ *   chaos_LLE2D                <- LLE::LLE2D (LLE/include/LLEHost.h:25-41)
+ *   chaos_basinsOfAttraction_2 <- Basins::basinsOfAttraction_2
+ *                                 (Basins/include/basinsHOST.h:15-30)
+ *
+ * The compute entry points (chaos_bif1d, chaos_bif2d, chaos_lle2d,
+ * chaos_basins2d, chaos_integrate) return the same values into caller
+ * buffers instead of files; they are what the file writers call.
+ *
+ * Conventions: every function returns 0 (CHAOS_OK) or a negative chaos_status;
+ * chaos_last_error() gives a thread-local message for the last failure.
+ * The reference instead prints and exit()s (debri/cudaMacros.cu:3-10).
+ * Inputs are borrowed for the duration of the call; all entry points are
+ * reentrant (no module-global device symbols) and never reset the device.
+ * Plain pointers and sizes only; hip types appear as void*.
+ */
+#ifndef CHAOSLIBRARY_H
+#define CHAOSLIBRARY_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CHAOS_API_VERSION 1
+
+typedef enum chaos_status {
+    CHAOS_OK            = 0,
+    CHAOS_E_INVALID     = -1,  /* bad argument                                  */
+    CHAOS_E_HIP         = -2,  /* HIP runtime error                             */
+    CHAOS_E_NOMEM       = -3,  /* host or device allocation failed             */
+    CHAOS_E_IO          = -4,  /* output file could not be opened / written    */
+    CHAOS_E_CAPACITY    = -5,  /* caller buffer too small (size reported back) */
+    CHAOS_E_NODEVICE    = -6,  /* no HIP device                                */
+    CHAOS_E_UNSUPPORTED = -7,  /* combination not implemented                  */
+    CHAOS_E_BUSY        = -8,  /* server queue full / not admitted             */
+    CHAOS_E_NOTFOUND    = -9   /* unknown job id                               */
+} chaos_status;
+
+/* Systems: the five reference systems of systems/systems.cuh (one chosen per
+ * reference build by #define USE_*_MODEL, systems/systems.cuh:9) and the two
+ * RK4 systems of the benchmark configurations, selected at run time. */
+typedef enum chaos_system {
+    CHAOS_SYS_CHAMELEON   = 0, /* NX 3, NA 7  systems/systems.cuh:11-39   */
+    CHAOS_SYS_ROSSLER_CD  = 1, /* NX 3, NA 4  systems/systems.cuh:41-63   */
+    CHAOS_SYS_BASINS      = 2, /* NX 3, NA 5  systems/systems.cuh:67-83   */
+    CHAOS_SYS_BASINS_2    = 3, /* NX 2, NA 2  systems/systems.cuh:85-101  */
+    CHAOS_SYS_DAMIR       = 4, /* NX 3, NA 2  systems/systems.cuh:103-178 */
+    CHAOS_SYS_ROSSLER_RK4 = 5, /* NX 3, NA 3  (a, b, c)                   */
+    CHAOS_SYS_LORENZ_RK4  = 6, /* NX 3, NA 3  (sigma, rho, beta)          */
+    CHAOS_SYS_COUNT       = 7
+} chaos_system;
+
+/* REFERENCE replays the reference's semantics (int(T/h) horizons, FP32
+ * divergence tests, quirky per-point dbscan, single-linkage basins);
+ * CONFIG uses the benchmark-config semantics (FP64 tests, llround horizons,
+ * 1-D DBSCAN peak counts, z-scored 3-D DBSCAN basins). */
+typedef enum chaos_semantics {
+    CHAOS_SEM_REFERENCE = 0,
+    CHAOS_SEM_CONFIG    = 1
+} chaos_semantics;
+
+typedef struct chaos_stats {
+    double kernel_ms;       /* device time of the integration kernel(s)      */
+    double analysis_ms;     /* device time of peak/cluster stages           */
+    double total_ms;        /* host wall time of the call                   */
+    double rk4_steps;       /* trajectory steps integrated (all points)     */
+    long long peaks;        /* peaks stored (bifurcation)                   */
+    int launches;           /* kernel launches issued                       */
+    int reserved;
+} chaos_stats;
+
+typedef struct chaos_options {
+    int system;             /* chaos_system; default CHAOS_SYS_CHAMELEON     */
+    int semantics;          /* chaos_semantics; default REFERENCE            */
+    uint64_t seed;          /* LLE perturbation seed; default 12345          */
+    int device;             /* HIP device ordinal; default 0                 */
+    int device_buffers;     /* compute outputs are device pointers           */
+    void* stream;           /* hipStream_t to use; NULL = library stream     */
+    int min_pts;            /* CONFIG basins DBSCAN minPts; default 20       */
+    int block;              /* threads per block (0 = tuned default)         */
+    chaos_stats* stats;     /* optional timing/work report                   */
+} chaos_options;
+
+/* Fill opt with the process defaults (CHAMELEON, REFERENCE, seed 12345,
+ * device 0, minPts 20).  chaos_set_default_options replaces the defaults used
+ * when an entry point receives opt == NULL (the C++ API always does); it is
+ * the run-time counterpart of the reference's compile-time system choice
+ * (#define USE_*_MODEL, systems/systems.cuh:9). */
+void        chaos_options_init(chaos_options* opt);
+int         chaos_set_default_options(const chaos_options* opt);
+int         chaos_api_version(void);
+const char* chaos_last_error(void);
+int         chaos_system_dims(int system, int* nx, int* na);
+int         chaos_device_count(int* count);
+
+/* Device memory helpers (plain pointers) for callers without a framework. */
+int chaos_device_malloc(void** ptr, size_t bytes, int device);
+int chaos_device_free(void* ptr);
+int chaos_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int chaos_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int chaos_device_synchronize(int device);
+
+/* Advance n trajectories nsteps each: states[n][NX] in/out, params[n][NA].
+ * The per-step map is the selected system's calcDiscreteModel
+ * (systems/systems.cuh) or RK4 step. */
+int chaos_integrate(const chaos_options* opt, int n, const double* params,
+                    double* states, long long nsteps, double h);
+
+/* ---------------- LLE (LLE/src/LLECUDA.cu:205-428) ---------------------- */
+/* out[j*nPts + i] = LLE(A_i, B_j), A = linspace(ranges[0], ranges[1], nPts)
+ * on values[indicesOfMutVars[0]], B likewise on ranges[2..3] /
+ * indicesOfMutVars[1].  writableVar and maxValue are ignored, as in the
+ * reference. */
+int chaos_lle2d(const chaos_options* opt, double tMax, double NT, int nPts,
+                double h, double eps, const double* initialConditions,
+                int amountOfInitialConditions, const double* ranges,
+                const int* indicesOfMutVars, int writableVar, double maxValue,
+                double transientTime, const double* values, int amountOfValues,
+                double* out);
+int chaos_LLE2D(const chaos_options* opt, double tMax, double NT, int nPts,
+                double h, double eps, const double* initialConditions,
+                int amountOfInitialConditions, const double* ranges,
+                const int* indicesOfMutVars, int writableVar, double maxValue,
+                double transientTime, const double* values, int amountOfValues,
+                const char* OUT_FILE_PATH);
+
+/* ---------------- Bifurcation (Bifurcation/src/bifurcationCUDA.cu) -------- */
+/* 1-D: counts[p] = kept peaks of point p (>= 0), or -1 for a fixed point
+ * (first[p] then holds its first recorded sample).  peaks/intervals receive
+ * the kept peaks of all points concatenated in point order; capacity is
+ * their length.  *total receives the number needed; CHAOS_E_CAPACITY if it
+ * exceeds capacity (counts/first/total are still valid).  Host pointers. */
+int chaos_bif1d(const chaos_options* opt, double tMax, int nPts, double h,
+                int amountOfInitialConditions, const double* initialConditions,
+                const double* ranges, const int* indicesOfMutVars,
+                int writableVar, double maxValue, double transientTime,
+                const double* values, int amountOfValues, int preScaller,
+                int* counts, double* first, double* peaks, double* intervals,
+                long long capacity, long long* total);
+int chaos_bifurcation1D(const chaos_options* opt, double tMax, int nPts,
+                        double h, int amountOfInitialConditions,
+                        const double* initialConditions, const double* ranges,
+                        const int* indicesOfMutVars, int writableVar,
+                        double maxValue, double transientTime,
+                        const double* values, int amountOfValues,
+                        int preScaller, const char* OUT_FILE_PATH);
+
+/* 2-D: out[j*nPts + i] (i over ranges[0..1], j over ranges[2..3]) is -1 for
+ * a fixed point, 0 for a divergent or peak-less point, else the cluster
+ * count (REFERENCE: the reference's per-point dbscan on intervals; CONFIG:
+ * number of 1-D DBSCAN(eps, minPts=1) clusters of the peak values). */
+int chaos_bif2d(const chaos_options* opt, double tMax, int nPts, double h,
+                int amountOfInitialConditions, const double* initialConditions,
+                const double* ranges, const int* indicesOfMutVars,
+                int writableVar, double maxValue, double transientTime,
+                const double* values, int amountOfValues, int preScaller,
+                double eps, int* out);
+int chaos_bifurcation2D(const chaos_options* opt, double tMax, int nPts,
+                        double h, int amountOfInitialConditions,
+                        const double* initialConditions, const double* ranges,
+                        const int* indicesOfMutVars, int writableVar,
+                        double maxValue, double transientTime,
+                        const double* values, int amountOfValues,
+                        int preScaller, double eps, const char* OUT_FILE_PATH);
+
+/* ---------------- Basins (Basins/src/basinsCUDA.cu:442-813) ------------- */
+/* labels[p], p = j*nPts + i over the IC grid (X[indicesOfMutVars[0]] over
+ * ranges[0..1], X[indicesOfMutVars[1]] over ranges[2..3]); flags[p] in
+ * {1,0,-1}; features f[k*nPts*nPts + p] (REFERENCE k<2: average peak,
+ * average interval; CONFIG k<3: z-scored mean x, mean of the last state
+ * component, peak std).  features/flags may be NULL. */
+int chaos_basins2d(const chaos_options* opt, double tMax, int nPts, double h,
+                   int amountOfInitialConditions,
+                   const double* initialConditions, const double* ranges,
+                   const int* indicesOfMutVars, int writableVar,
+                   double maxValue, double transientTime, const double* values,
+                   int amountOfValues, int preScaller, double eps, int* labels,
+                   double* features, int* flags);
+int chaos_basinsOfAttraction_2(const chaos_options* opt, double tMax, int nPts,
+                               double h, int amountOfInitialConditions,
+                               const double* initialConditions,
+                               const double* ranges,
+                               const int* indicesOfMutVars, int writableVar,
+                               double maxValue, double transientTime,
+                               const double* values, int amountOfValues,
+                               int preScaller, double eps,
+                               const char* OUT_FILE_PATH);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CHAOSLIBRARY_H */

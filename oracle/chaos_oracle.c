@@ -1,0 +1,947 @@
+/*
+ * chaos_oracle.c -- CPU restatement of the ChaosLibrary per-point hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP product.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load this library.
+ *
+ * Compiled with -ffp-contract=off: every fused multiply-add below is an
+ * explicit fma(), every other product and sum rounds separately, exactly as
+ * the reference's per-point code compiled for the host with contraction off.
+ * Citations are reference paths (file:line).
+ */
+#include "chaos_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_MAX_NX 12
+
+/* ------------------------------------------------------------------------ */
+/* Systems (L1 plug-in)                                                      */
+/* ------------------------------------------------------------------------ */
+
+int orc_dims(int sys, int* nx, int* na)
+{
+    static const int NX[ORC_SYS_COUNT] = {3, 3, 3, 2, 3, 3, 3};
+    static const int NA[ORC_SYS_COUNT] = {7, 4, 5, 2, 2, 3, 3};
+    if (sys < 0 || sys >= ORC_SYS_COUNT) return -1;
+    if (nx) *nx = NX[sys];
+    if (na) *na = NA[sys];
+    return 0;
+}
+
+/* systems/systems.cuh:11-39 (USE_CHAMELEON_MODEL); __fma_rn -> fma. */
+static void step_chameleon(double* X, const double* a, double h)
+{
+    double h1 = a[0] * h;
+    double h2 = (1 - a[0]) * h;
+    X[0] = fma(h1, -a[6] * X[1], X[0]);
+    X[1] = fma(h1, a[6] * X[0] + a[1] * X[2], X[1]);
+    double cos_term = cos(a[5] * X[1]);
+    X[2] = fma(h1, a[2] - a[3] * X[2] + a[4] * cos_term, X[2]);
+    X[2] = fma(h2, (a[2] + a[4] * cos_term), X[2]) / (1 + a[3] * h2);
+    X[1] = fma(h2, (a[6] * X[0] + a[1] * X[2]), X[1]);
+    X[0] = fma(h2, -a[6] * X[1], X[0]);
+}
+
+/* systems/systems.cuh:41-63 (USE_ROSSLER_MODEL, semi-implicit CD scheme). */
+static void step_rossler_cd(double* x, const double* a, double h)
+{
+    double h1 = 0.5 * h + a[0];
+    double h2 = 0.5 * h - a[0];
+    x[0] = fma(h1, -x[1] - x[2], x[0]);
+    x[1] = fma(h1, x[0] + a[1] * x[1], x[1]);
+    x[2] = fma(h1, a[2] + x[2] * (x[0] - a[3]), x[2]);
+    double temp = fma(-h2, x[0] - a[3], 1.0);
+    x[2] = fma(h2, a[2], x[2]) / temp;
+    temp = fma(-h2, a[1], 1.0);
+    x[1] = fma(h2, x[0], x[1]) / temp;
+    x[0] = fma(h2, -x[1] - x[2], x[0]);
+}
+
+/* systems/systems.cuh:67-83 (USE_SYSTEM_FOR_BASINS); h2 is a float there. */
+static void step_basins(double* X, const double* a, double h)
+{
+    float h2 = (float)(h * (1 - a[0]));
+    X[0] = X[0] + h * (sin(X[1]) - a[1] * X[0]);
+    X[1] = X[1] + h * (sin(X[2]) - a[1] * X[1]);
+    X[2] = X[2] + h * (sin(X[0]) - a[1] * X[2]);
+    X[2] = (X[2] + (double)h2 * sin(X[0])) / (1 + (double)h2 * a[1]);
+    X[1] = (X[1] + (double)h2 * sin(X[2])) / (1 + (double)h2 * a[1]);
+    X[0] = (X[0] + (double)h2 * sin(X[1])) / (1 + (double)h2 * a[1]);
+}
+
+/* systems/systems.cuh:85-101 (USE_SYSTEM_FOR_BASINS_2); h1, h2, z are floats. */
+static void step_basins2(double* X, const double* a, double h)
+{
+    float h1 = (float)(h * a[0]);
+    float h2 = (float)(h * (1 - a[0]));
+    X[0] = X[0] + (double)h1 * (-X[1]);
+    X[1] = X[1] + (double)h1 * (a[1] * X[0] + sin(X[1]));
+    float z = (float)X[1];
+    X[1] = (double)z + (double)h2 * (a[1] * X[0] + sin(X[1]));
+    X[0] = X[0] + (double)h2 * (-X[1]);
+}
+
+/* systems/systems.cuh:103-178 (USE_DAMIR_SYSTEM, explicit Euler, time in X[2]). */
+static void step_damir(double* X, const double* a, double h)
+{
+    double Vc = X[0], XSV = X[1], t = X[2];
+    double U1 = 0.0, U2 = -0.04, C1 = 22e-9;
+    double Is = 1.15E-7, Vt = 1.0 / 15.0, Vp = 0.09, Ip = 2.1e-5;
+    double Iv = -3e-6, D = 26.0, E = 0.14;
+    double Vd = Vc + U2;
+    double Vm = Vc + U1;
+    double idiode = Is * (exp(Vd / Vt) - exp(-Vd / Vt));
+    double itunnel = Ip / Vp * Vd * exp(-(Vd - Vp) / Vp);
+    double iex = Iv * (atan(D * (Vd - E)) + atan(D * (Vd + E)));
+    double Id = idiode + itunnel + iex;
+    double Ron = 1434.0, Roff = 1e6, Von1 = 0.28, Voff1 = 0.14;
+    double Von2 = -0.12, Voff2 = -0.006, TAU = 1e-7, T = 2.0;
+    double boltz = 1.380649e-23, echarge = 1.602176634e-19;
+    double g = XSV / Ron + (1.0 - XSV) / Roff;
+    double arg1 = -1.0 / (T * boltz / echarge) * (Vm - Von1) * (Vm - Von2);
+    double arg2 = -1.0 / (T * boltz / echarge) * (Vm - Voff2) * (Vm - Voff1);
+    double exp1 = exp(arg1);
+    double exp2 = exp(arg2);
+    double f1 = 1.0 / (1.0 + exp1);
+    double f2 = 1.0 / (1.0 + exp2);
+    double Ix = (1.0 / TAU) * (f1 * (1.0 - XSV) - (1.0 - f2) * XSV);
+    double Imem = Vm * g;
+    double Iin = a[0] * sin(a[1] * t);
+    double dVc = (Iin - Imem - Id) / C1;
+    double dXSV = Ix;
+    X[0] = Vc + h * dVc;
+    X[1] = XSV + h * dXSV;
+    X[2] = t + h;
+}
+
+/*
+ * Classic RK4 (new systems; no reference counterpart, systems/systems.cuh
+ * keeps RK4 only as commented templates, debri/cudaLibrary.cu:2927-2975).
+ * Canonical operation order shared with the HIP kernels:
+ *   k1 = f(X); Y = fma(h/2,k1,X); k2 = f(Y); Y = fma(h/2,k2,X); k3 = f(Y);
+ *   Y = fma(h,k3,X); k4 = f(Y);
+ *   s = fma(2,k2,k1); s = fma(2,k3,s); s = s + k4; X = fma(h/6,s,X)
+ * with h/2 = 0.5*h and h/6 = h/6.0 (IEEE division).
+ * Pinned by: closed-form linear growth factor (generic RK4 in the test,
+ * not this code), order-4 error decay against scipy DOP853, Lorenz origin
+ * fixed point and (x,y,z)->(-x,-y,z) symmetry (bit-exact).
+ */
+static inline void f_rossler(const double* x, const double* a, double* f)
+{
+    f[0] = -(x[1] + x[2]);
+    f[1] = fma(a[0], x[1], x[0]);
+    f[2] = fma(x[2], x[0] - a[2], a[1]);
+}
+
+static inline void f_lorenz(const double* x, const double* a, double* f)
+{
+    f[0] = a[0] * (x[1] - x[0]);
+    f[1] = fma(x[0], a[1] - x[2], -x[1]);
+    f[2] = fma(x[0], x[1], -(a[2] * x[2]));
+}
+
+static void step_rk4(void (*f)(const double*, const double*, double*),
+                     double* X, const double* a, double h)
+{
+    double hh = 0.5 * h, h6 = h / 6.0;
+    double k[3], Y[3], s[3];
+    int i;
+    f(X, a, k);
+    for (i = 0; i < 3; ++i) { s[i] = k[i]; Y[i] = fma(hh, k[i], X[i]); }
+    f(Y, a, k);
+    for (i = 0; i < 3; ++i) { s[i] = fma(2.0, k[i], s[i]); Y[i] = fma(hh, k[i], X[i]); }
+    f(Y, a, k);
+    for (i = 0; i < 3; ++i) { s[i] = fma(2.0, k[i], s[i]); Y[i] = fma(h, k[i], X[i]); }
+    f(Y, a, k);
+    for (i = 0; i < 3; ++i) { s[i] = s[i] + k[i]; X[i] = fma(h6, s[i], X[i]); }
+}
+
+void orc_step(int sys, double* X, const double* a, double h)
+{
+    switch (sys) {
+    case ORC_SYS_CHAMELEON:   step_chameleon(X, a, h); break;
+    case ORC_SYS_ROSSLER_CD:  step_rossler_cd(X, a, h); break;
+    case ORC_SYS_BASINS:      step_basins(X, a, h); break;
+    case ORC_SYS_BASINS_2:    step_basins2(X, a, h); break;
+    case ORC_SYS_DAMIR:       step_damir(X, a, h); break;
+    case ORC_SYS_ROSSLER_RK4: step_rk4(f_rossler, X, a, h); break;
+    case ORC_SYS_LORENZ_RK4:  step_rk4(f_lorenz, X, a, h); break;
+    default: break;
+    }
+}
+
+void orc_steps(int sys, double* X, const double* a, double h, long long n)
+{
+    for (long long i = 0; i < n; ++i) orc_step(sys, X, a, h);
+}
+
+void orc_integrate(int sys, int n, const double* params, double* states,
+                   long long nsteps, double h)
+{
+    int nx, na;
+    if (orc_dims(sys, &nx, &na)) return;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int p = 0; p < n; ++p)
+        orc_steps(sys, states + (size_t)p * nx, params + (size_t)p * na, h, nsteps);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Grid values                                                               */
+/* ------------------------------------------------------------------------ */
+
+/* Bifurcation/src/bifurcationCUDA.cu:400-419 and LLE/src/LLECUDA.cu:172-203:
+ * start + k * ((end - start) / (n - 1)), product and sum rounded separately. */
+double orc_grid_value(int k, int n, double start, double end)
+{
+    if (n == 1) return start;
+    double scale = (end - start) / (n - 1);
+    return start + k * scale;
+}
+
+/* Basins/src/basinsCUDA.cu:185-189 (pow form, same value as above). */
+double orc_grid_value_pow(int idx, int n, double start, double end, int dim)
+{
+    int k = ((int)((int)idx / pow((double)n, (double)dim))) % n;
+    return start + (k * ((double)(end - start) / (double)(n - 1)));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Integration loop with divergence / fixed-point flags                     */
+/* ------------------------------------------------------------------------ */
+
+static int diverged(double v, double maxValue, int check)
+{
+    if (check == ORC_CHK_BIF_REF) {
+        /* Bifurcation/src/bifurcationCUDA.cu:445-453 */
+        if (v != v || v == v + 1.0) return 1;
+        if (maxValue != 0 && (double)fabsf((float)v) > maxValue) return 1;
+    } else if (check == ORC_CHK_BASINS_REF) {
+        /* Basins/src/basinsCUDA.cu:350-361 */
+        if (isnan(v) || isinf(v)) return 1;
+        if (maxValue != 0 && (double)fabsf((float)v) > maxValue) return 1;
+    } else {
+        if (!isfinite(v)) return 1;
+        if (maxValue != 0 && fabs(v) > maxValue) return 1;
+    }
+    return 0;
+}
+
+static int fixed_point(double t, int check)
+{
+    if (check == ORC_CHK_BIF_REF)           /* bifurcationCUDA.cu:462 */
+        return (double)sqrtf(fabsf((float)t)) < 1e-9;
+    if (check == ORC_CHK_BASINS_REF)        /* basinsCUDA.cu:372 */
+        return sqrt(fabs(t)) < 1e-9;
+    return sqrt(t) < 1e-9;
+}
+
+/*
+ * Bifurcation/src/bifurcationCUDA.cu:421-467 (preScaller == 1) and
+ * Basins/src/basinsCUDA.cu:318-380 (preScaller sub-steps per sample).
+ * Stores X[writableVar] before each sample's steps; returns 0 (diverged),
+ * -1 (last sample moved less than 1e-9) or 1.  amount <= 0 returns 1 (the
+ * reference reads an uninitialised xPrev there).
+ * Pinned by: hand-built divergence and equilibrium cases in tests.
+ */
+int orc_loop(int sys, double* X, const double* a, double h, int amount,
+             int preScaller, int writableVar, double maxValue, int check,
+             double* data)
+{
+    int nx, na, i, j;
+    double xPrev[ORC_MAX_NX];
+    if (orc_dims(sys, &nx, &na)) return 0;
+    if (amount <= 0) return 1;
+    if (preScaller < 1) preScaller = 1;
+    for (i = 0; i < amount; ++i) {
+        for (j = 0; j < nx; ++j) xPrev[j] = X[j];
+        if (data) data[i] = X[writableVar];
+        for (j = 0; j < preScaller; ++j) orc_step(sys, X, a, h);
+        if (diverged(X[writableVar], maxValue, check)) return 0;
+    }
+    double t = 0;
+    for (j = 0; j < nx; ++j) t += ((X[j] - xPrev[j]) * (X[j] - xPrev[j]));
+    return fixed_point(t, check) ? -1 : 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Peak finder                                                               */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Bifurcation/src/bifurcationCUDA.cu:469-514 (= Basins/src/basinsCUDA.cu:382-440),
+ * with startDataIndex = 0 and out-of-place outputs (the reference's in-place
+ * writes never overtake its reads).  Plateau-aware maxima with 1e-13
+ * thresholds; the first peak is dropped and intervals are (t[k+1]-t[k])*h.
+ * Pinned by: sampled sines, plateaus, staircases and the rejected-plateau
+ * skip case, worked by hand in tests/test_oracle_peaks.py.
+ */
+int orc_peak_finder(const double* data, int n, double* outPeaks,
+                    double* timeOfPeaks, double h)
+{
+    int amountOfPeaks = 0;
+    for (int i = 1; i < n - 1; ++i) {
+        if (data[i] - data[i - 1] > 1e-13 && data[i] >= data[i + 1]) {
+            for (int j = i; j < n - 1; ++j) {
+                if (data[j] < data[j + 1]) {
+                    i = j + 1;
+                    break;
+                }
+                if (data[j] - data[j + 1] > 1e-13) {
+                    if (outPeaks) outPeaks[amountOfPeaks] = data[j];
+                    if (timeOfPeaks)
+                        timeOfPeaks[amountOfPeaks] = trunc(((double)j + (double)i) / (double)2);
+                    ++amountOfPeaks;
+                    i = j + 1;
+                    break;
+                }
+            }
+        }
+    }
+    if (amountOfPeaks > 1) {
+        for (int k = 0; k < amountOfPeaks - 1; k++) {
+            if (outPeaks) outPeaks[k] = outPeaks[k + 1];
+            if (timeOfPeaks)
+                timeOfPeaks[k] = (double)(timeOfPeaks[k + 1] - timeOfPeaks[k]) * h;
+        }
+        amountOfPeaks = amountOfPeaks - 1;
+    } else {
+        amountOfPeaks = 0;
+    }
+    return amountOfPeaks;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Per-point clustering                                                      */
+/* ------------------------------------------------------------------------ */
+
+/* Bifurcation/src/bifurcationCUDA.cu:575-583 */
+static double distance_ref(double x1, double y1, double x2, double y2)
+{
+    if (x1 == x2 && y1 == y2) return 0;
+    double dx = x2 - x1;
+    double dy = y2 - y1;
+    return (double)hypotf((float)dx, (float)dy);
+}
+
+/*
+ * Bifurcation/src/bifurcationCUDA.cu:516-572, restated line by line with its
+ * quirks: peak values are zeroed (clustering on intervals only), k never
+ * reaches the last point, the neighbour "stack" reads slot P+NumNeibor-1
+ * while pushes write slot P+k, and the result is cluster-1.
+ * helpful must hold sizeOfHelpful >= 2*P-1 doubles.
+ * Pinned by: hand-traced cases in tests/test_oracle_cluster.py.
+ */
+int orc_dbscan_ref(double* data, const double* intervals, double* helpful,
+                   int amountOfPeaks, int sizeOfHelpful, double eps)
+{
+    if (amountOfPeaks <= 0) return 0;
+    if (amountOfPeaks == 1) return 1;
+    int cluster = 0;
+    int NumNeibor = 0;
+    for (int i = 0; i < sizeOfHelpful; ++i) helpful[i] = 0;
+    for (int i = 0; i < amountOfPeaks; i++) data[i] = 0;
+    for (int i = 0; i < amountOfPeaks; i++)
+        if (NumNeibor >= 1) {
+            i = (int)helpful[amountOfPeaks + NumNeibor - 1];
+            helpful[amountOfPeaks + NumNeibor - 1] = 0;
+            NumNeibor = NumNeibor - 1;
+            for (int k = 0; k < amountOfPeaks - 1; k++) {
+                if (i != k && helpful[k] == 0) {
+                    if (distance_ref(data[i], intervals[i], data[k], intervals[k]) < eps) {
+                        helpful[k] = cluster;
+                        helpful[amountOfPeaks + k] = k;
+                        ++NumNeibor;
+                    }
+                }
+            }
+        } else if (helpful[i] == 0) {
+            NumNeibor = 0;
+            ++cluster;
+            helpful[i] = cluster;
+            for (int k = 0; k < amountOfPeaks - 1; k++) {
+                if (i != k && helpful[k] == 0) {
+                    if (distance_ref(data[i], intervals[i], data[k], intervals[k]) < eps) {
+                        helpful[k] = cluster;
+                        helpful[amountOfPeaks + k] = k;
+                        ++NumNeibor;
+                    }
+                }
+            }
+        }
+    return cluster - 1;
+}
+
+static int cmp_double(const void* a, const void* b)
+{
+    double x = *(const double*)a, y = *(const double*)b;
+    return (x > y) - (x < y);
+}
+
+/*
+ * 1-D DBSCAN with minPts = 1 (every point is core): the clusters are the
+ * connected components of |u - v| <= eps, i.e. 1 + #{sorted gaps > eps}.
+ * Pinned by: sklearn.cluster.DBSCAN(eps, min_samples=1) fixtures.
+ */
+int orc_dbscan1d_count(const double* values, int n, double eps)
+{
+    if (n <= 0) return 0;
+    double* v = (double*)malloc(sizeof(double) * (size_t)n);
+    if (!v) return -1;
+    memcpy(v, values, sizeof(double) * (size_t)n);
+    qsort(v, (size_t)n, sizeof(double), cmp_double);
+    int c = 1;
+    for (int k = 1; k < n; ++k)
+        if (v[k] - v[k - 1] > eps) ++c;
+    free(v);
+    return c;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Bifurcation drivers                                                       */
+/* ------------------------------------------------------------------------ */
+
+static int horizon(double T, double h, int sem)
+{
+    /* Reference: int(T/h) truncation (bifurcationCUDA.cu:60-62). */
+    return sem == ORC_SEM_REFERENCE ? (int)(T / h) : (int)llround(T / h);
+}
+
+static int record_len(double tMax, double h, int preScaller, int sem)
+{
+    return sem == ORC_SEM_REFERENCE ? (int)(tMax / h / preScaller)
+                                    : (int)llround(tMax / h / preScaller);
+}
+
+/* One bifurcation point: Bifurcation/src/bifurcationCUDA.cu:258-346.
+ * Returns the flag; data[B] receives the recorded samples. */
+static int bif_point(int sys, int sem, const double* ic, int nx,
+                     const double* a, double h, int S, int B, int writableVar,
+                     double maxValue, double* data)
+{
+    double X[ORC_MAX_NX];
+    int chk = sem == ORC_SEM_REFERENCE ? ORC_CHK_BIF_REF : ORC_CHK_CONFIG;
+    for (int i = 0; i < nx; ++i) X[i] = ic[i];
+    int flag = orc_loop(sys, X, a, h, S, 1, writableVar, maxValue, chk, NULL);
+    if (flag != 1) return 0;     /* bifurcationCUDA.cu:334-342 */
+    return orc_loop(sys, X, a, h, B, 1, writableVar, maxValue, chk, data);
+}
+
+/* Bifurcation/src/bifurcationCUDA.cu:587-783 (values, not the CSV). */
+long long orc_bif1d(int sys, int sem, double tMax, int nPts, double h,
+                    const double* ic, int nx, const double* ranges,
+                    const int* idx, int writableVar, double maxValue,
+                    double transientTime, const double* values, int na,
+                    int preScaller, int* counts, double* first,
+                    double* peaks, double* intervals, long long capacity)
+{
+    int snx, sna;
+    if (orc_dims(sys, &snx, &sna) || nx != snx || na != sna || nPts < 1) return -2;
+    int B = record_len(tMax, h, preScaller, sem);
+    int S = horizon(transientTime, h, sem);
+    if (B < 1) return -2;
+    double* data = (double*)malloc(sizeof(double) * (size_t)B * 3);
+    if (!data) return -3;
+    double* pk = data + B;
+    double* iv = data + 2 * (size_t)B;
+    double a[ORC_MAX_NX];
+    long long total = 0;
+    for (int p = 0; p < nPts; ++p) {
+        for (int i = 0; i < na; ++i) a[i] = values[i];
+        a[idx[0]] = orc_grid_value(p % nPts, nPts, ranges[0], ranges[1]);
+        int flag = bif_point(sys, sem, ic, nx, a, h, S, B, writableVar, maxValue, data);
+        first[p] = 0;
+        if (flag == -1) {
+            counts[p] = -1;
+            first[p] = data[0];
+        } else if (flag == 0) {
+            counts[p] = 0;
+        } else {
+            int P = orc_peak_finder(data, B, pk, iv, h * (double)preScaller);
+            counts[p] = P;
+            for (int k = 0; k < P; ++k) {
+                if (total < capacity) {
+                    peaks[total] = pk[k];
+                    intervals[total] = iv[k];
+                }
+                ++total;
+            }
+        }
+    }
+    free(data);
+    return total > capacity ? -1 : total;
+}
+
+/* Bifurcation/src/bifurcationCUDA.cu:43-256 (values, not the CSV).
+ * REFERENCE: quirky dbscan on (zeroed peaks, intervals); CONFIG: 1-D DBSCAN
+ * cluster count of the kept peak values. */
+int orc_bif2d(int sys, int sem, double tMax, int nPts, double h,
+              const double* ic, int nx, const double* ranges, const int* idx,
+              int writableVar, double maxValue, double transientTime,
+              const double* values, int na, int preScaller, double eps,
+              int* out, int row_begin, int row_end)
+{
+    int snx, sna;
+    if (orc_dims(sys, &snx, &sna) || nx != snx || na != sna || nPts < 1) return -2;
+    int B = record_len(tMax, h, preScaller, sem);
+    int S = horizon(transientTime, h, sem);
+    if (B < 1) return -2;
+    int err = 0;
+#pragma omp parallel
+    {
+        double* data = (double*)malloc(sizeof(double) * (size_t)B * 4);
+        if (!data) {
+#pragma omp atomic write
+            err = -3;
+        }
+#pragma omp for schedule(dynamic, 4)
+        for (long long p = (long long)row_begin * nPts; p < (long long)row_end * nPts; ++p) {
+            if (!data) continue;
+            double* pk = data + B;
+            double* iv = data + 2 * (size_t)B;
+            double* helpful = data + 3 * (size_t)B;
+            double a[ORC_MAX_NX];
+            for (int i = 0; i < na; ++i) a[i] = values[i];
+            a[idx[0]] = orc_grid_value((int)(p % nPts), nPts, ranges[0], ranges[1]);
+            a[idx[1]] = orc_grid_value((int)(p / nPts), nPts, ranges[2], ranges[3]);
+            int flag = bif_point(sys, sem, ic, nx, a, h, S, B, writableVar, maxValue, data);
+            int r;
+            if (flag == -1) r = -1;
+            else if (flag == 0) r = 0;
+            else {
+                int P = orc_peak_finder(data, B, pk, iv, h * (double)preScaller);
+                if (sem == ORC_SEM_REFERENCE)
+                    r = orc_dbscan_ref(pk, iv, helpful, P, B, eps);
+                else
+                    r = orc_dbscan1d_count(pk, P, eps);
+            }
+            out[p - (long long)row_begin * nPts] = r;
+        }
+        free(data);
+    }
+    return err;
+}
+
+/* ------------------------------------------------------------------------ */
+/* LLE (Benettin)                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* SplitMix64 (Steele, Lea & Flood, OOPSLA 2014), the public-domain mixer. */
+uint64_t orc_splitmix64(uint64_t* state)
+{
+    uint64_t z = (*state += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+/*
+ * Seeded unit vector for the Benettin perturbation (replaces curand XORWOW at
+ * LLE/src/LLECUDA.cu:88-101, whose output cannot be reproduced here).
+ * Stream state = seed ^ (point+1)*0xD1B54A32D192ED03; draws map to exact
+ * uniforms on [-1,1) as (int64)(z>>10 - 2^53) * 2^-53; rejection to the unit
+ * ball (1e-6 < r2 <= 1, at most 64 tries) then division by sqrt(r2).
+ * Pinned by: |v| = 1 within 2 ulp and uniform-direction moments in tests.
+ */
+void orc_unit_vector(uint64_t seed, long long point, int nx, double* v)
+{
+    uint64_t st = seed ^ ((uint64_t)(point + 1) * 0xD1B54A32D192ED03ULL);
+    for (int tr = 0; tr < 64; ++tr) {
+        double r2 = 0;
+        for (int i = 0; i < nx; ++i) {
+            uint64_t z = orc_splitmix64(&st);
+            v[i] = (double)((int64_t)(z >> 10) - ((int64_t)1 << 53)) * 0x1p-53;
+        }
+        for (int i = 0; i < nx; ++i) r2 = r2 + v[i] * v[i];
+        if (r2 > 1e-6 && r2 <= 1.0) {
+            double norm = sqrt(r2);
+            for (int i = 0; i < nx; ++i) v[i] = v[i] / norm;
+            return;
+        }
+    }
+    for (int i = 0; i < nx; ++i) v[i] = i == 0 ? 1.0 : 0.0;
+}
+
+/*
+ * One LLE point: LLE/src/LLECUDA.cu:56-112 (transient + perturbation) and
+ * :114-166 (Benettin windows).  d = |X - Xp| / eps; sum log(d);
+ * Xp = X - (X - Xp) / d; result sum / tMax.
+ * REFERENCE windows: int(tMax/h)/int(NT/h) + 2 (LLECUDA.cu:227,147);
+ * CONFIG windows: llround(tMax/h)/llround(NT/h).  log is the FP64 log.
+ * Pinned by: linear diagonal system (LLE = ln R(h lambda)/h), stable
+ * equilibrium, periodic orbit ~0 and published Lorenz/Rossler values.
+ */
+double orc_lle_point(int sys, int sem, double tMax, double NT, double h,
+                     double eps, const double* ic, int nx, const double* a,
+                     double transientTime, uint64_t seed, long long point)
+{
+    double X[ORC_MAX_NX], Xp[ORC_MAX_NX], v[ORC_MAX_NX];
+    int S = horizon(transientTime, h, sem);
+    int NTp = horizon(NT, h, sem);
+    int all = horizon(tMax, h, sem);
+    if (NTp <= 0) return NAN;
+    int W = sem == ORC_SEM_REFERENCE ? all / NTp + 2 : all / NTp;
+    for (int i = 0; i < nx; ++i) X[i] = ic[i];
+    orc_steps(sys, X, a, h, S);
+    orc_unit_vector(seed, point, nx, v);
+    for (int i = 0; i < nx; ++i) {
+        double d = eps * v[i];
+        Xp[i] = X[i] + d;
+    }
+    double sum = 0.0;
+    for (int w = 0; w < W; ++w) {
+        orc_steps(sys, X, a, h, NTp);
+        orc_steps(sys, Xp, a, h, NTp);
+        double d2 = 0.0;
+        for (int l = 0; l < nx; ++l) {
+            double diff = X[l] - Xp[l];
+            d2 = d2 + diff * diff;
+        }
+        double dist = sqrt(d2) / eps;
+        sum = sum + log(dist);
+        for (int l = 0; l < nx; ++l) Xp[l] = X[l] - ((X[l] - Xp[l]) / dist);
+    }
+    return sum / tMax;
+}
+
+/* LLE/src/LLECUDA.cu:205-428 (values, not the CSV): out[(j-row_begin)*n + i]
+ * = LLE(A_i, B_j) with A = linspace(ranges[0..1]), B = linspace(ranges[2..3]);
+ * the perturbation seed index is j*n + i (the reference's idx_a + idx_b*n). */
+int orc_lle2d(int sys, int sem, double tMax, double NT, int nPts, double h,
+              double eps, const double* ic, int nx, const double* ranges,
+              const int* idx, double transientTime, const double* values,
+              int na, uint64_t seed, double* out, int row_begin, int row_end)
+{
+    return orc_lle2d_omp(sys, sem, tMax, NT, nPts, h, eps, ic, nx, ranges, idx,
+                         transientTime, values, na, seed, out, row_begin,
+                         row_end, 1, 0) < 0 ? -2 : 0;
+}
+
+int orc_lle2d_omp(int sys, int sem, double tMax, double NT, int nPts, double h,
+                  double eps, const double* ic, int nx, const double* ranges,
+                  const int* idx, double transientTime, const double* values,
+                  int na, uint64_t seed, double* out, int row_begin,
+                  int row_end, int row_stride, int threads)
+{
+    int snx, sna;
+    if (orc_dims(sys, &snx, &sna) || nx != snx || na != sna || nPts < 1) return -2;
+    if (row_stride < 1) row_stride = 1;
+    int nrows = (row_end - row_begin + row_stride - 1) / row_stride;
+    int used = 1;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+#pragma omp parallel
+    {
+#ifdef _OPENMP
+#pragma omp single
+        used = omp_get_num_threads();
+#endif
+#pragma omp for schedule(dynamic, 64)
+        for (long long q = 0; q < (long long)nrows * nPts; ++q) {
+            int r = (int)(q / nPts), i = (int)(q % nPts);
+            int j = row_begin + r * row_stride;
+            double a[ORC_MAX_NX];
+            for (int k = 0; k < na; ++k) a[k] = values[k];
+            a[idx[0]] = orc_grid_value(i, nPts, ranges[0], ranges[1]);
+            a[idx[1]] = orc_grid_value(j, nPts, ranges[2], ranges[3]);
+            out[q] = orc_lle_point(sys, sem, tMax, NT, h, eps, ic, nx, a,
+                                   transientTime, seed, (long long)j * nPts + i);
+        }
+    }
+    return used;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Basins                                                                    */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Per-IC features for Basins/src/basinsCUDA.cu:233-314 (transient without
+ * checks, then B samples with preScaller sub-steps) and :818-864 (REFERENCE:
+ * avg peak / avg interval with sentinels 999 / (last sample, -1) / 1000).
+ * CONFIG: f0 = mean X[0], f1 = mean X[nx-1] over the B recorded samples
+ * (sequential sums), f2 = population std of the kept writableVar peaks
+ * (Welford: n+=1; d=p-m; m+=d/n; M2+=d*(p-m); std = sqrt(M2/n), 0 if n<2);
+ * divergent points get (0,0,0) and flag 0.
+ */
+int orc_basins_features(int sys, int sem, double tMax, int nPts, double h,
+                        const double* ic, int nx, const double* ranges,
+                        const int* idx, int writableVar, double maxValue,
+                        double transientTime, const double* values, int na,
+                        int preScaller, double* f0, double* f1, double* f2,
+                        int* flags, int point_begin, int point_end)
+{
+    int snx, sna;
+    if (orc_dims(sys, &snx, &sna) || nx != snx || na != sna || nPts < 2) return -2;
+    if (preScaller < 1) preScaller = 1;
+    int B = record_len(tMax, h, preScaller, sem);
+    int S = horizon(transientTime, h, sem);
+    if (B < 1) return -2;
+    int err = 0;
+#pragma omp parallel
+    {
+        double* data = (double*)malloc(sizeof(double) * (size_t)B * 3 * (sem == ORC_SEM_REFERENCE ? 1 : 2));
+        if (!data) {
+#pragma omp atomic write
+            err = -3;
+        }
+#pragma omp for schedule(dynamic, 16)
+        for (int p = point_begin; p < point_end; ++p) {
+            if (!data) continue;
+            double* pk = data + B;
+            double* iv = data + 2 * (size_t)B;
+            double X[ORC_MAX_NX];
+            const double* a = values;
+            int q = p - point_begin;
+            for (int i = 0; i < nx; ++i) X[i] = ic[i];
+            X[idx[0]] = orc_grid_value_pow(p, nPts, ranges[0], ranges[1], 0);
+            X[idx[1]] = orc_grid_value_pow(p, nPts, ranges[2], ranges[3], 1);
+            orc_steps(sys, X, a, h, S);
+            if (sem == ORC_SEM_REFERENCE) {
+                int flag = orc_loop(sys, X, a, h, B, preScaller, writableVar,
+                                    maxValue, ORC_CHK_BASINS_REF, data);
+                flags[q] = flag;
+                if (flag == 0) { f0[q] = 999; f1[q] = 999; }
+                else if (flag == -1) { f0[q] = data[B - 1]; f1[q] = -1.0; }
+                else {
+                    int P = orc_peak_finder(data, B, pk, iv, h * preScaller);
+                    if (P <= 0) { f0[q] = 1000; f1[q] = 1000; }
+                    else {
+                        double s0 = 0, s1 = 0;
+                        for (int k = 0; k < P; ++k) { s0 += pk[k]; s1 += iv[k]; }
+                        f0[q] = s0 / P;
+                        f1[q] = s1 / P;
+                    }
+                }
+                if (f2) f2[q] = 0;
+            } else {
+                /* record all state components we need: data = X[wv], plus
+                 * the two mean channels accumulated alongside. */
+                double* dx0 = data + 3 * (size_t)B;
+                double* dxn = data + 4 * (size_t)B;
+                double Y[ORC_MAX_NX];
+                double xPrev[ORC_MAX_NX];
+                int flag = 1;
+                for (int i = 0; i < nx; ++i) Y[i] = X[i];
+                for (int s = 0; s < B && flag; ++s) {
+                    for (int j = 0; j < nx; ++j) xPrev[j] = Y[j];
+                    data[s] = Y[writableVar];
+                    dx0[s] = Y[0];
+                    dxn[s] = Y[nx - 1];
+                    for (int j = 0; j < preScaller; ++j) orc_step(sys, Y, a, h);
+                    if (diverged(Y[writableVar], maxValue, ORC_CHK_CONFIG)) flag = 0;
+                }
+                if (flag) {
+                    double t = 0;
+                    for (int j = 0; j < nx; ++j) t += ((Y[j] - xPrev[j]) * (Y[j] - xPrev[j]));
+                    flag = fixed_point(t, ORC_CHK_CONFIG) ? -1 : 1;
+                }
+                flags[q] = flag;
+                if (flag == 0) { f0[q] = 0; f1[q] = 0; f2[q] = 0; }
+                else {
+                    double s0 = 0, s1 = 0;
+                    for (int s = 0; s < B; ++s) { s0 = s0 + dx0[s]; s1 = s1 + dxn[s]; }
+                    f0[q] = s0 / B;
+                    f1[q] = s1 / B;
+                    int P = orc_peak_finder(data, B, pk, iv, h * preScaller);
+                    double m = 0, M2 = 0;
+                    for (int k = 0; k < P; ++k) {
+                        double d = pk[k] - m;
+                        m = m + d / (k + 1);
+                        M2 = M2 + d * (pk[k] - m);
+                    }
+                    f2[q] = P >= 2 ? sqrt(M2 / P) : 0.0;
+                }
+            }
+        }
+        free(data);
+    }
+    return err;
+}
+
+/* Union-find helpers (path halving, union by smaller index as root). */
+static int uf_find(int* par, int x)
+{
+    while (par[x] != x) { par[x] = par[par[x]]; x = par[x]; }
+    return x;
+}
+static void uf_union(int* par, int a, int b)
+{
+    a = uf_find(par, a); b = uf_find(par, b);
+    if (a == b) return;
+    if (a < b) par[b] = a; else par[a] = b;
+}
+
+/*
+ * Basins/src/basinsCUDA.cu:35-179,866-923 without its races: components of
+ * the graph sqrt(dx*dx + dy*dy) <= eps over non-divergent points (flag != 0;
+ * the reference never expands through a divergent point, :913-916).  A
+ * component holding a fixed point (flag -1) gets a negative label, the rest
+ * positive (fixed-point seeds are searched first, :86-126).  Labels are
+ * numbered by smallest member index within each sign (the reference's seed
+ * choice is racy, so its numbering is pinned only up to permutation).
+ * O(n^2): for n up to ~2e4.  Pinned by: hand-built point sets in tests.
+ */
+int orc_basins_cluster_ref(const double* f0, const double* f1,
+                           const int* flags, int n, double eps, int* labels)
+{
+    int* par = (int*)malloc(sizeof(int) * (size_t)n * 3);
+    if (!par) return -3;
+    int* neg = par + n;
+    int* num = par + 2 * (size_t)n;
+    for (int i = 0; i < n; ++i) { par[i] = i; neg[i] = 0; num[i] = 0; }
+    for (int i = 0; i < n; ++i) {
+        if (flags[i] == 0) continue;
+        for (int j = i + 1; j < n; ++j) {
+            if (flags[j] == 0) continue;
+            double dx = f0[i] - f0[j];
+            double dy = f1[i] - f1[j];
+            if (sqrt(dx * dx + dy * dy) <= eps) uf_union(par, i, j);
+        }
+    }
+    for (int i = 0; i < n; ++i)
+        if (flags[i] == -1) neg[uf_find(par, i)] = 1;
+    int npos = 0, nneg = 0;
+    for (int i = 0; i < n; ++i) {
+        if (flags[i] == 0) { labels[i] = 0; continue; }
+        int r = uf_find(par, i);
+        if (neg[r]) {
+            /* number negatives by smallest fixed-point index */
+            if (flags[i] == -1 && num[r] == 0) num[r] = -(++nneg);
+        } else if (num[r] == 0) {
+            num[r] = ++npos;
+        }
+    }
+    for (int i = 0; i < n; ++i)
+        labels[i] = flags[i] == 0 ? 0 : num[uf_find(par, i)];
+    free(par);
+    return 0;
+}
+
+/*
+ * Deterministic fixed-order sum shared with the HIP reduction: blocks of 1024
+ * consecutive entries (masked-out entries contribute +0), each reduced by the
+ * pairwise tree s[t] += s[t+stride] for stride = 512, 256, ..., 1; block sums
+ * then added sequentially in block order.
+ */
+double orc_det_sum(const double* x, const int* mask, long long n)
+{
+    double s[1024];
+    double tot = 0.0;
+    for (long long b = 0; b * 1024 < n; ++b) {
+        for (int t = 0; t < 1024; ++t) {
+            long long i = b * 1024 + t;
+            s[t] = (i < n && (!mask || mask[i] != 0)) ? x[i] : 0.0;
+        }
+        for (int stride = 512; stride > 0; stride >>= 1)
+            for (int t = 0; t < stride; ++t) s[t] = s[t] + s[t + stride];
+        tot = tot + s[0];
+    }
+    return tot;
+}
+
+/* Z-score the three CONFIG features over points with flag != 0, in place:
+ * mean = S/N, std = sqrt(sum((x-mean)^2)/N) (two passes, det_sum order);
+ * z = (x - mean) / std, or 0 when std == 0.  Excluded points are left. */
+int orc_zscore(double* f0, double* f1, double* f2, const int* flags, long long n,
+               double* moments)
+{
+    double* f[3] = {f0, f1, f2};
+    long long N = 0;
+    for (long long i = 0; i < n; ++i) N += flags[i] != 0;
+    if (N == 0) return 0;
+    double* tmp = (double*)malloc(sizeof(double) * (size_t)n);
+    if (!tmp) return -3;
+    for (int k = 0; k < 3; ++k) {
+        double mean = orc_det_sum(f[k], flags, n) / (double)N;
+        for (long long i = 0; i < n; ++i) {
+            double d = f[k][i] - mean;
+            tmp[i] = d * d;
+        }
+        double sd = sqrt(orc_det_sum(tmp, flags, n) / (double)N);
+        if (moments) { moments[2 * k] = mean; moments[2 * k + 1] = sd; }
+        for (long long i = 0; i < n; ++i) {
+            if (flags[i] == 0) continue;
+            f[k][i] = sd > 0 ? (f[k][i] - mean) / sd : 0.0;
+        }
+    }
+    free(tmp);
+    return 0;
+}
+
+/*
+ * Brute-force DBSCAN (O(n^2)) with the build's deterministic rules: a point is
+ * core when #{q : sqrt(sum_k (p_k - q_k)^2) <= eps} >= minPts (self counted,
+ * squares summed in coordinate order); core points within eps are joined; a
+ * non-core point within eps of a core point takes the cluster of the
+ * smallest-index such core point; noise and excluded points get 0; clusters
+ * are numbered 1..K by their smallest core index.  pts is [n][dim].
+ * Pinned by: sklearn.cluster.DBSCAN core-sample partition equality.
+ */
+int orc_dbscan_bruteforce(const double* pts, int n, int dim, double eps,
+                          int minPts, const int* include, int* labels)
+{
+    int* par = (int*)malloc(sizeof(int) * (size_t)n * 3);
+    if (!par) return -3;
+    int* core = par + n;
+    int* num = par + 2 * (size_t)n;
+#define ORC_D(i, j, out)                                               \
+    do {                                                               \
+        double d2_ = 0.0;                                              \
+        for (int k_ = 0; k_ < dim; ++k_) {                             \
+            double df_ = pts[(size_t)(i) * dim + k_] - pts[(size_t)(j) * dim + k_]; \
+            d2_ = d2_ + df_ * df_;                                     \
+        }                                                              \
+        (out) = sqrt(d2_) <= eps;                                      \
+    } while (0)
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; ++i) {
+        core[i] = 0;
+        if (include && !include[i]) continue;
+        int c = 0;
+        for (int j = 0; j < n && c < minPts; ++j) {
+            if (include && !include[j]) continue;
+            int w;
+            ORC_D(i, j, w);
+            c += w;
+        }
+        core[i] = c >= minPts;
+    }
+    for (int i = 0; i < n; ++i) { par[i] = i; num[i] = 0; }
+    for (int i = 0; i < n; ++i) {
+        if (!core[i]) continue;
+        for (int j = i + 1; j < n; ++j) {
+            if (!core[j]) continue;
+            int w;
+            ORC_D(i, j, w);
+            if (w) uf_union(par, i, j);
+        }
+    }
+    int K = 0;
+    for (int i = 0; i < n; ++i)
+        if (core[i]) {
+            int r = uf_find(par, i);
+            if (num[r] == 0) num[r] = ++K;
+        }
+    /* core labels first (sequential: uf_find compresses paths) */
+    for (int i = 0; i < n; ++i) labels[i] = core[i] ? num[uf_find(par, i)] : 0;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; ++i) {
+        if (core[i] || (include && !include[i])) continue;
+        for (int j = 0; j < n; ++j) {
+            if (!core[j]) continue;
+            int w;
+            ORC_D(i, j, w);
+            if (w) { labels[i] = labels[j]; break; }
+        }
+    }
+#undef ORC_D
+    free(par);
+    return K;
+}

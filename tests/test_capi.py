@@ -1,0 +1,59 @@
+"""CPU checks of the C-ABI library: it loads, exports every symbol that
+include/*.h declares, and rejects bad arguments before touching a device."""
+import ctypes as C
+import glob
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names.update(re.findall(r"\b(chaos_[A-Za-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_header_declares_entry_points():
+    names = declared_symbols()
+    for must in ("chaos_lle2d", "chaos_LLE2D", "chaos_bif1d", "chaos_bifurcation1D", "chaos_bif2d",
+                 "chaos_bifurcation2D", "chaos_basins2d", "chaos_basinsOfAttraction_2",
+                 "chaos_integrate", "chaos_last_error"):
+        assert must in names
+
+
+def test_every_declared_symbol_is_exported(chaos):
+    missing = [n for n in declared_symbols() if not hasattr(chaos.lib, n)]
+    assert not missing, missing
+
+
+def test_version_and_dims(chaos):
+    assert chaos.lib.chaos_api_version() == 1
+    assert chaos.system_dims("rossler_rk4") == (3, 3)
+    assert chaos.system_dims("chameleon") == (3, 7)
+    assert chaos.system_dims("basins_2") == (2, 2)
+    with pytest.raises(chaos.ChaosError) as e:
+        chaos.system_dims(42)
+    assert e.value.code == -1
+
+
+def test_invalid_arguments_rejected_without_device(chaos):
+    with pytest.raises(chaos.ChaosError) as e:
+        chaos.lle2d(10.0, 0.5, 0, 0.01, 1e-8, [0.1] * 3, [0, 1, 0, 1], [0, 2], 1.0, [0.2, 0.2, 5.7],
+                    system="rossler_rk4")
+    assert e.value.code == -1 and "nPts" in str(e.value)
+    with pytest.raises(chaos.ChaosError):
+        chaos.lle2d(10.0, 0.5, 4, 0.01, 1e-8, [0.1] * 3, [0, 1, 0, 1], [0, 5], 1.0, [0.2, 0.2, 5.7],
+                    system="rossler_rk4")
+    with pytest.raises(chaos.ChaosError):  # wrong parameter count for the system
+        chaos.lle2d(10.0, 0.5, 4, 0.01, 1e-8, [0.1] * 3, [0, 1, 0, 1], [0, 2], 1.0, [0.2, 0.2],
+                    system="rossler_rk4")
+
+
+def test_device_count_does_not_crash(chaos):
+    assert chaos.device_count() >= 0
