@@ -181,8 +181,127 @@ def LLE2D(tMax, NT, nPts, h, eps, initialConditions, amountOfInitialConditions, 
                            transientTime, _dp(vals), amountOfValues, str(OUT_FILE_PATH).encode()))
 
 
-def bif1d(*a, **k):  # bound once the bifurcation module is built
-    raise ChaosError(-7, "bifurcation entry points not bound")
+_SIGS2 = {
+    "chaos_bif1d": (C.c_int, [C.POINTER(Options), C.c_double, C.c_int, C.c_double, C.c_int, _D, _D, _I,
+                              C.c_int, C.c_double, C.c_double, _D, C.c_int, C.c_int, _I, _D, _D, _D,
+                              C.c_longlong, C.POINTER(C.c_longlong)]),
+    "chaos_bifurcation1D": (C.c_int, [C.POINTER(Options), C.c_double, C.c_int, C.c_double, C.c_int, _D, _D,
+                                      _I, C.c_int, C.c_double, C.c_double, _D, C.c_int, C.c_int, C.c_char_p]),
+    "chaos_bif2d": (C.c_int, [C.POINTER(Options), C.c_double, C.c_int, C.c_double, C.c_int, _D, _D, _I,
+                              C.c_int, C.c_double, C.c_double, _D, C.c_int, C.c_int, C.c_double, C.c_void_p]),
+    "chaos_bifurcation2D": (C.c_int, [C.POINTER(Options), C.c_double, C.c_int, C.c_double, C.c_int, _D, _D,
+                                      _I, C.c_int, C.c_double, C.c_double, _D, C.c_int, C.c_int, C.c_double,
+                                      C.c_char_p]),
+    "chaos_basins2d": (C.c_int, [C.POINTER(Options), C.c_double, C.c_int, C.c_double, C.c_int, _D, _D, _I,
+                                 C.c_int, C.c_double, C.c_double, _D, C.c_int, C.c_int, C.c_double,
+                                 C.c_void_p, C.c_void_p, C.c_void_p]),
+    "chaos_peak_finder": (C.c_int, [C.POINTER(Options), _D, C.c_int, C.c_int, C.c_double, _I, _D, _D]),
+    "chaos_basinsOfAttraction_2": (C.c_int, [C.POINTER(Options), C.c_double, C.c_int, C.c_double, C.c_int,
+                                             _D, _D, _I, C.c_int, C.c_double, C.c_double, _D, C.c_int,
+                                             C.c_int, C.c_double, C.c_char_p]),
+}
+for _n, (_r, _a) in _SIGS2.items():
+    _fn = getattr(lib, _n)
+    _fn.restype = _r
+    _fn.argtypes = _a
 
 
-bifurcation1D = bif2d = bifurcation2D = basins2d = basinsOfAttraction_2 = bif1d
+def _opt_ref(options):
+    return C.byref(options) if options is not None else None
+
+
+def peak_finder(data, h, **kw):
+    """Reference peakFinder on each row of data (GPU, one lane per row):
+    returns a list of (peaks, intervals) per row."""
+    o = make_options(**kw)
+    d = _f64(np.atleast_2d(data))
+    rows, n = d.shape
+    counts = np.zeros(rows, dtype=np.int32)
+    pk = np.zeros((rows, max(n, 1)))
+    iv = np.zeros((rows, max(n, 1)))
+    _check(lib.chaos_peak_finder(C.byref(o), _dp(d), rows, n, h, _ip(counts), _dp(pk), _dp(iv)))
+    return [(pk[r, :counts[r]].copy(), iv[r, :counts[r]].copy()) for r in range(rows)]
+
+
+def bif1d(tMax, nPts, h, initialConditions, ranges, indicesOfMutVars, writableVar, maxValue,
+          transientTime, values, preScaller=1, system="chameleon", semantics=SEM_REFERENCE, **kw):
+    """Values of Bifurcation::bifurcation1D: (counts, first, peaks, intervals).
+
+    counts[p] = kept peaks of point p, or -1 for a fixed point (first[p] is
+    then its first recorded sample); peaks/intervals are concatenated in
+    point order."""
+    o = make_options(system, semantics, **kw)
+    ic, rg, ix, vals = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
+    counts = np.zeros(nPts, dtype=np.int32)
+    first = np.zeros(nPts)
+    total = C.c_longlong(0)
+    args = (C.byref(o), tMax, nPts, h, ic.size, _dp(ic), _dp(rg), _ip(ix), writableVar, maxValue,
+            transientTime, _dp(vals), vals.size, preScaller, _ip(counts), _dp(first))
+    rc = lib.chaos_bif1d(*args, None, None, 0, C.byref(total))
+    if rc not in (0, -5):
+        _check(rc)
+    cap = max(1, total.value)
+    pk, iv = np.zeros(cap), np.zeros(cap)
+    _check(lib.chaos_bif1d(*args, _dp(pk), _dp(iv), cap, C.byref(total)))
+    return counts, first, pk[:total.value], iv[:total.value]
+
+
+def bifurcation1D(tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges, indicesOfMutVars,
+                  writableVar, maxValue, transientTime, values, amountOfValues, preScaller, OUT_FILE_PATH,
+                  options=None):
+    """Bifurcation::bifurcation1D (Bifurcation/include/bifurcationHOST.h:16-30)."""
+    ic, rg, ix, vals = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
+    _check(lib.chaos_bifurcation1D(_opt_ref(options), tMax, nPts, h, amountOfInitialConditions, _dp(ic), _dp(rg),
+                                   _ip(ix), writableVar, maxValue, transientTime, _dp(vals), amountOfValues,
+                                   preScaller, str(OUT_FILE_PATH).encode()))
+
+
+def bif2d(tMax, nPts, h, initialConditions, ranges, indicesOfMutVars, writableVar, maxValue, transientTime,
+          values, preScaller=1, eps=1e-2, system="chameleon", semantics=SEM_REFERENCE, out_device_ptr=None, **kw):
+    """Values of Bifurcation::bifurcation2D: out[j, i] (row = second swept parameter)."""
+    o = make_options(system, semantics, device_buffers=out_device_ptr is not None, **kw)
+    ic, rg, ix, vals = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
+    if out_device_ptr is None:
+        out = np.zeros((nPts, nPts), dtype=np.int32)
+        ptr = out.ctypes.data
+    else:
+        out, ptr = None, out_device_ptr
+    _check(lib.chaos_bif2d(C.byref(o), tMax, nPts, h, ic.size, _dp(ic), _dp(rg), _ip(ix), writableVar, maxValue,
+                           transientTime, _dp(vals), vals.size, preScaller, eps, C.c_void_p(ptr)))
+    return out
+
+
+def bifurcation2D(tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges, indicesOfMutVars,
+                  writableVar, maxValue, transientTime, values, amountOfValues, preScaller, eps, OUT_FILE_PATH,
+                  options=None):
+    """Bifurcation::bifurcation2D (Bifurcation/include/bifurcationHOST.h:35-50)."""
+    ic, rg, ix, vals = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
+    _check(lib.chaos_bifurcation2D(_opt_ref(options), tMax, nPts, h, amountOfInitialConditions, _dp(ic), _dp(rg),
+                                   _ip(ix), writableVar, maxValue, transientTime, _dp(vals), amountOfValues,
+                                   preScaller, eps, str(OUT_FILE_PATH).encode()))
+
+
+def basins2d(tMax, nPts, h, initialConditions, ranges, indicesOfMutVars, writableVar, maxValue, transientTime,
+             values, preScaller=1, eps=0.5, system="chameleon", semantics=SEM_REFERENCE, **kw):
+    """Values of Basins::basinsOfAttraction_2: (labels[j,i], features[k,j,i], flags[j,i])."""
+    o = make_options(system, semantics, **kw)
+    ic, rg, ix, vals = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
+    labels = np.zeros((nPts, nPts), dtype=np.int32)
+    flags = np.zeros((nPts, nPts), dtype=np.int32)
+    nf = 2 if int(semantics) == SEM_REFERENCE else 3
+    feats = np.zeros((nf, nPts, nPts))
+    _check(lib.chaos_basins2d(C.byref(o), tMax, nPts, h, ic.size, _dp(ic), _dp(rg), _ip(ix), writableVar,
+                              maxValue, transientTime, _dp(vals), vals.size, preScaller, eps,
+                              C.c_void_p(labels.ctypes.data), C.c_void_p(feats.ctypes.data),
+                              C.c_void_p(flags.ctypes.data)))
+    return labels, feats, flags
+
+
+def basinsOfAttraction_2(tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges, indicesOfMutVars,
+                         writableVar, maxValue, transientTime, values, amountOfValues, preScaller, eps,
+                         OUT_FILE_PATH, options=None):
+    """Basins::basinsOfAttraction_2 (Basins/include/basinsHOST.h:15-30)."""
+    ic, rg, ix, vals = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
+    _check(lib.chaos_basinsOfAttraction_2(_opt_ref(options), tMax, nPts, h, amountOfInitialConditions, _dp(ic),
+                                          _dp(rg), _ip(ix), writableVar, maxValue, transientTime, _dp(vals),
+                                          amountOfValues, preScaller, eps, str(OUT_FILE_PATH).encode()))

@@ -1,0 +1,167 @@
+// basins.hip -- basins of attraction over a 2-D grid of initial conditions
+// (Basins/src/basinsCUDA.cu:442-813).
+//
+// sweep_kernel<KIND_BASINS_*> produces per-IC features and flags in one
+// launch; the labels come from dbscan_device:
+//   REFERENCE: (mean peak, mean interval) with the reference's sentinels,
+//              single linkage at <= eps over non-divergent points, negative
+//              labels for components that hold a fixed point;
+//   CONFIG:    (mean X[0], mean X[NX-1], peak std) z-scored over
+//              non-divergent points, DBSCAN(eps, minPts) in 3-D.
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "csv.hpp"
+#include "dbscan.hpp"
+#include "dispatch.hpp"
+#include "sweep.hpp"
+
+using namespace chaos;
+
+extern "C" int chaos_basins2d(const chaos_options* opt, double tMax, int nPts, double h, int amountOfInitialConditions,
+                              const double* initialConditions, const double* ranges, const int* indicesOfMutVars,
+                              int writableVar, double maxValue, double transientTime, const double* values,
+                              int amountOfValues, int preScaller, double eps, int* labels, double* features,
+                              int* flags)
+{
+    return guarded("chaos_basins2d", [&] {
+        chaos_options o = resolve_options(opt);
+        const int nx = amountOfInitialConditions, na = amountOfValues;
+        check_system(o.system, nx, na);
+        CHAOS_REQUIRE(labels, "labels is NULL");
+        CHAOS_REQUIRE(nPts >= 2, "nPts must be >= 2 (got %d)", nPts);
+        CHAOS_REQUIRE(h > 0 && std::isfinite(h), "h must be positive");
+        CHAOS_REQUIRE(preScaller >= 1, "preScaller must be >= 1");
+        CHAOS_REQUIRE(initialConditions && ranges && indicesOfMutVars && values, "NULL input array");
+        CHAOS_REQUIRE(indicesOfMutVars[0] >= 0 && indicesOfMutVars[0] < nx && indicesOfMutVars[1] >= 0 &&
+                          indicesOfMutVars[1] < nx,
+                      "indicesOfMutVars index the state vector here (basinsCUDA.cu:259) and must be in [0,%d)", nx);
+        CHAOS_REQUIRE(writableVar >= 0 && writableVar < nx, "writableVar out of range");
+        CHAOS_REQUIRE(eps > 0, "eps must be positive");
+        const long long n = (long long)nPts * nPts;
+        CHAOS_REQUIRE(n <= (1LL << 31) - 64, "grid too large");
+        const bool ref = o.semantics == CHAOS_SEM_REFERENCE;
+
+        SweepArgs A{};
+        for (int k = 0; k < na; ++k) A.values[k] = values[k];
+        for (int k = 0; k < nx; ++k) A.ic[k] = initialConditions[k];
+        for (int k = 0; k < 4; ++k) A.r[k] = ranges[k];
+        A.idx0 = indicesOfMutVars[0];
+        A.idx1 = indicesOfMutVars[1];
+        A.nPts = nPts;
+        A.dims = 2;
+        A.n_points = n;
+        A.S = horizon_steps(transientTime, h, o.semantics);
+        A.B = record_samples(tMax, h, preScaller, o.semantics);
+        CHAOS_REQUIRE(A.B >= 1, "tMax/h/preScaller must give at least one recorded sample");
+        A.pre = preScaller;
+        A.check = ref ? CHK_BASINS_REF : CHK_CONFIG;
+        A.maxValue = maxValue;
+        A.L = divergence_fast_bound(A.check, maxValue);
+        A.hp = h * preScaller;  // basinsCUDA.cu:684
+        A.sc = make_step_const(h);
+
+        CallCtx ctx(o);
+        DevBuf<int> d_flags(n), d_labels;
+        DevBuf<double> d_f(3 * (size_t)n);
+        A.flags = d_flags.get();
+        A.f0 = d_f.get();
+        A.f1 = d_f.get() + n;
+        A.f2 = d_f.get() + 2 * n;
+        A.pool.nchunks = 0;
+        int* out_labels = labels;
+        if (!o.device_buffers) {
+            d_labels.alloc(n);
+            out_labels = d_labels.get();
+        }
+        ctx.begin();
+        launch_sweep(ctx, o.system, writableVar, ref ? KIND_BASINS_REF : KIND_BASINS_CFG, A);
+        ctx.add_kernel_ms(ctx.end());
+        ctx.add_steps((double)n * ((double)A.S + (double)A.B * A.pre));
+
+        ctx.begin();
+        if (ref) {
+            const double* pts[2] = {A.f0, A.f1};
+            DbscanParams prm{2, eps, 1, true};
+            dbscan_device(ctx, pts, A.flags, n, prm, out_labels);
+        } else {
+            // z-score a copy; the caller gets the raw features
+            DevBuf<double> z(3 * (size_t)n);
+            CHAOS_HIP(hipMemcpyAsync(z.get(), d_f.get(), z.bytes(), hipMemcpyDeviceToDevice, ctx.stream()));
+            double* f[3] = {z.get(), z.get() + n, z.get() + 2 * n};
+            zscore_device(ctx, f, 3, A.flags, n, nullptr);
+            const double* pts[3] = {f[0], f[1], f[2]};
+            DbscanParams prm{3, eps, o.min_pts, false};
+            dbscan_device(ctx, pts, A.flags, n, prm, out_labels);
+        }
+        ctx.add_analysis_ms(ctx.end());
+        const hipMemcpyKind k = o.device_buffers ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+        if (!o.device_buffers)
+            CHAOS_HIP(hipMemcpyAsync(labels, out_labels, n * sizeof(int), k, ctx.stream()));
+        if (features)
+            CHAOS_HIP(hipMemcpyAsync(features, d_f.get(), (ref ? 2 : 3) * n * sizeof(double), k, ctx.stream()));
+        if (flags) CHAOS_HIP(hipMemcpyAsync(flags, d_flags.get(), n * sizeof(int), k, ctx.stream()));
+        ctx.sync();
+        ctx.finish();
+    });
+}
+
+extern "C" int chaos_basinsOfAttraction_2(const chaos_options* opt, double tMax, int nPts, double h,
+                                          int amountOfInitialConditions, const double* initialConditions,
+                                          const double* ranges, const int* indicesOfMutVars, int writableVar,
+                                          double maxValue, double transientTime, const double* values,
+                                          int amountOfValues, int preScaller, double eps,
+                                          const char* OUT_FILE_PATH)
+{
+    return guarded("chaos_basinsOfAttraction_2", [&] {
+        CHAOS_REQUIRE(OUT_FILE_PATH, "OUT_FILE_PATH is NULL");
+        CHAOS_REQUIRE(nPts >= 2, "nPts must be >= 2");
+        chaos_options o = resolve_options(opt);
+        o.device_buffers = 0;
+        const bool ref = o.semantics == CHAOS_SEM_REFERENCE;
+        const long long n = (long long)nPts * nPts;
+        const std::string base(OUT_FILE_PATH);
+        const int nfeat = ref ? 2 : 3;
+        // all four (five) files get the two range lines first, at the
+        // precision 15 the reference sets before writing them
+        // (basinsCUDA.cu:557-590)
+        {
+            TextFile f(base);
+            write_range_header(f, ranges, 2, 15);
+            f.close();
+            for (int k = 1; k <= nfeat + 1; ++k) {
+                TextFile g(base + "_" + std::to_string(k) + ".csv");
+                write_range_header(g, ranges, 2, 15);
+                g.close();
+            }
+        }
+        std::vector<int> labels(n), flags(n);
+        std::vector<double> feat(nfeat * n);
+        int rc = chaos_basins2d(&o, tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges,
+                                indicesOfMutVars, writableVar, maxValue, transientTime, values, amountOfValues,
+                                preScaller, eps, labels.data(), feat.data(), flags.data());
+        if (rc != CHAOS_OK) throw Error(rc, chaos_last_error());
+        {
+            TextFile f(base, true);
+            write_map_int(f, labels.data(), n, nPts);
+            f.close();
+        }
+        // _1: feature 0, _2: feature 1, _3: flags (:733-792); CONFIG adds _4: feature 2
+        for (int k = 0; k < 2; ++k) {
+            TextFile g(base + "_" + std::to_string(k + 1) + ".csv", true);
+            write_map_double(g, feat.data() + k * n, n, nPts, 15, false);
+            g.close();
+        }
+        {
+            TextFile g(base + "_3.csv", true);
+            write_map_int(g, flags.data(), n, nPts);
+            g.close();
+        }
+        if (!ref) {
+            TextFile g(base + "_4.csv", true);
+            write_map_double(g, feat.data() + 2 * n, n, nPts, 15, false);
+            g.close();
+        }
+    });
+}

@@ -1,0 +1,570 @@
+// bifurcation.hip -- Bifurcation 1-D and 2-D (Bifurcation/src/bifurcationCUDA.cu).
+//
+// Pipeline per call:
+//   sweep_kernel<KIND_BIF>   transient + record + online peaks; every peak is
+//                            appended to its wave's chunk (ballot + mbcnt)
+//   kept_kernel + scan       per-point segment offsets (flag == 1 points)
+//   scatter_kernel           chunk entries -> per-point contiguous (value, time)
+//   1-D: host assembles the kept peaks / intervals (first peak dropped)
+//   2-D: one wave per point -- CONFIG: sort (LDS bitonic) + gap count =
+//        1-D DBSCAN(eps, minPts=1) cluster count; REFERENCE: the reference's
+//        per-point dbscan on the kept intervals.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "csv.hpp"
+#include "dispatch.hpp"
+#include "scan.hpp"
+#include "sweep.hpp"
+
+namespace chaos {
+
+struct BifInputs {
+    double tMax, h, maxValue, transientTime, eps;
+    int nPts, nx, na, writableVar, preScaller, dims;
+    const double *ic, *ranges, *values;
+    const int* idx;
+};
+
+struct BifDevice {
+    long long n = 0;         // points
+    long long total = 0;     // stored peaks of flag-1 points (first peak included)
+    double hp = 0;           // interval unit
+    DevBuf<int> flags, counts;
+    DevBuf<double> first;
+    DevBuf<long long> off;   // segment offsets
+    DevBuf<double> vals;
+    DevBuf<unsigned> times;
+    std::vector<int> h_flags, h_counts;
+    std::vector<double> h_first;
+};
+
+__global__ void kept_kernel(const int* __restrict__ flags, const int* __restrict__ counts,
+                            long long* __restrict__ kept, long long n)
+{
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) kept[p] = flags[p] == 1 ? counts[p] : 0;
+}
+
+__global__ void scatter_kernel(const PeakPool pool, unsigned used, const int* __restrict__ flags,
+                               const long long* __restrict__ off, double* __restrict__ vals,
+                               unsigned* __restrict__ times, long long n)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned c = (unsigned)(g / kChunk);
+    const int s = (int)(g % kChunk);
+    if (c >= used || s >= pool.fill[c]) return;
+    const PeakEntry e = pool.entries[(size_t)c * kChunk + s];
+    const long long p = (long long)pool.owner[c] * 64 + (e.kl & 63u);
+    if (p >= n || flags[p] != 1) return;
+    const long long pos = off[p] + (e.kl >> 6);
+    vals[pos] = e.v;
+    times[pos] = e.t;
+}
+
+static void run_bif(CallCtx& ctx, const chaos_options& o, const BifInputs& in, BifDevice& R)
+{
+    check_system(o.system, in.nx, in.na);
+    CHAOS_REQUIRE(in.nPts >= 1, "nPts must be >= 1 (got %d)", in.nPts);
+    CHAOS_REQUIRE(in.h > 0 && std::isfinite(in.h), "h must be positive");
+    CHAOS_REQUIRE(in.preScaller >= 1, "preScaller must be >= 1");
+    CHAOS_REQUIRE(in.ic && in.ranges && in.values && in.idx, "NULL input array");
+    for (int d = 0; d < in.dims; ++d)
+        CHAOS_REQUIRE(in.idx[d] >= 0 && in.idx[d] < in.na, "indicesOfMutVars[%d]=%d out of range [0,%d)", d,
+                      in.idx[d], in.na);
+    CHAOS_REQUIRE(in.writableVar >= 0 && in.writableVar < in.nx, "writableVar out of range");
+    const long long n = in.dims == 2 ? (long long)in.nPts * in.nPts : in.nPts;
+    CHAOS_REQUIRE(n <= (1LL << 31) - 64, "grid too large");
+
+    SweepArgs A{};
+    for (int k = 0; k < in.na; ++k) A.values[k] = in.values[k];
+    for (int k = 0; k < in.nx; ++k) A.ic[k] = in.ic[k];
+    for (int k = 0; k < 2 * in.dims; ++k) A.r[k] = in.ranges[k];
+    A.idx0 = in.idx[0];
+    A.idx1 = in.dims == 2 ? in.idx[1] : -1;
+    A.nPts = in.nPts;
+    A.dims = in.dims;
+    A.n_points = n;
+    A.S = horizon_steps(in.transientTime, in.h, o.semantics);
+    A.B = record_samples(in.tMax, in.h, in.preScaller, o.semantics);
+    CHAOS_REQUIRE(A.B >= 1, "tMax/h/preScaller must give at least one recorded sample");
+    A.pre = 1;  // Bifurcation steps once per sample (bifurcationCUDA.cu:435-443)
+    A.check = o.semantics == CHAOS_SEM_REFERENCE ? CHK_BIF_REF : CHK_CONFIG;
+    A.maxValue = in.maxValue;
+    A.L = divergence_fast_bound(A.check, in.maxValue);
+    A.hp = in.h * (double)in.preScaller;  // bifurcationCUDA.cu:196,726
+    A.sc = make_step_const(in.h);
+    R.n = n;
+    R.hp = A.hp;
+    R.flags.alloc(n);
+    R.counts.alloc(n);
+    R.first.alloc(n);
+    A.flags = R.flags.get();
+    A.counts = R.counts.get();
+    A.first = R.first.get();
+
+    const long long waves = (n + 63) / 64;
+    size_t free_b = 0, total_b = 0;
+    CHAOS_HIP(hipMemGetInfo(&free_b, &total_b));
+    // First guess: one peak per 32 samples (at most one per two is possible);
+    // an overflow is detected on the device and the sweep is rerun with the
+    // exact per-wave need.
+    const double max_pp = std::min((double)A.B / 2.0 + 1.0, std::max(256.0, (double)A.B / 32.0));
+    const long long budget_chunks = (long long)(0.25 * (double)free_b / (kChunk * sizeof(PeakEntry) + 8));
+    long long want = (long long)std::ceil((double)n * max_pp / (kChunk - 64)) + waves;
+    long long nchunks = std::max<long long>(waves, std::min(want, budget_chunks));
+
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        DevBuf<PeakEntry> entries((size_t)nchunks * kChunk);
+        DevBuf<int> owner((size_t)nchunks), fill((size_t)nchunks), ovf(1);
+        DevBuf<unsigned> head(1);
+        CHAOS_HIP(hipMemsetAsync(head.get(), 0, sizeof(unsigned), ctx.stream()));
+        CHAOS_HIP(hipMemsetAsync(ovf.get(), 0, sizeof(int), ctx.stream()));
+        A.pool.entries = entries.get();
+        A.pool.owner = owner.get();
+        A.pool.fill = fill.get();
+        A.pool.head = head.get();
+        A.pool.overflow = ovf.get();
+        A.pool.nchunks = (unsigned)nchunks;
+        ctx.begin();
+        launch_sweep(ctx, o.system, in.writableVar, KIND_BIF, A);
+        ctx.add_kernel_ms(ctx.end());
+        ctx.add_steps((double)n * ((double)A.S + (double)A.B * A.pre));
+        unsigned used = 0;
+        int overflow = 0;
+        CHAOS_HIP(hipMemcpyAsync(&used, head.get(), sizeof used, hipMemcpyDeviceToHost, ctx.stream()));
+        CHAOS_HIP(hipMemcpyAsync(&overflow, ovf.get(), sizeof overflow, hipMemcpyDeviceToHost, ctx.stream()));
+        R.h_flags.resize(n);
+        R.h_counts.resize(n);
+        R.h_first.resize(n);
+        CHAOS_HIP(hipMemcpyAsync(R.h_flags.data(), R.flags.get(), n * sizeof(int), hipMemcpyDeviceToHost, ctx.stream()));
+        CHAOS_HIP(hipMemcpyAsync(R.h_counts.data(), R.counts.get(), n * sizeof(int), hipMemcpyDeviceToHost, ctx.stream()));
+        CHAOS_HIP(hipMemcpyAsync(R.h_first.data(), R.first.get(), n * sizeof(double), hipMemcpyDeviceToHost, ctx.stream()));
+        ctx.sync();
+        if (overflow) {
+            // exact need from the per-wave totals (each chunk switch wastes < 64 slots)
+            long long need = 0;
+            for (long long w = 0; w < waves; ++w) {
+                long long s = 0;
+                for (long long p = w * 64; p < std::min(n, w * 64 + 64); ++p) s += R.h_counts[p];
+                need += (s + (kChunk - 64) - 1) / (kChunk - 64) + 1;
+            }
+            CHAOS_REQUIRE(attempt == 0, "peak store overflow after resize");
+            nchunks = need;
+            continue;
+        }
+        ctx.begin();
+        DevBuf<long long> kept((size_t)n);
+        R.off.alloc(n);
+        hipLaunchKernelGGL(kept_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx.stream(), R.flags.get(),
+                           R.counts.get(), kept.get(), n);
+        CHAOS_HIP(hipGetLastError());
+        R.total = exclusive_scan(ctx.stream(), kept.get(), R.off.get(), n);
+        R.vals.alloc((size_t)std::max(1LL, R.total));
+        R.times.alloc((size_t)std::max(1LL, R.total));
+        if (used > (unsigned)nchunks) used = (unsigned)nchunks;
+        if (used > 0) {
+            hipLaunchKernelGGL(scatter_kernel, dim3(ceil_div((unsigned long long)used * kChunk, 256)), dim3(256), 0,
+                               ctx.stream(), A.pool, used, R.flags.get(), R.off.get(), R.vals.get(), R.times.get(), n);
+            CHAOS_HIP(hipGetLastError());
+        }
+        ctx.add_analysis_ms(ctx.end());
+        ctx.add_launch(2);
+        ctx.add_peaks(R.total);
+        return;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 2-D analyses, one wave (64-thread block) per point
+// ---------------------------------------------------------------------------
+
+// In-place ascending bitonic sort of M (power of two) doubles shared by the
+// block's threads; sync() orders the stages.
+template <class T>
+__device__ __forceinline__ void bitonic_sort(T* v, int M)
+{
+    for (int size = 2; size <= M; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int idx = threadIdx.x; idx < (M >> 1); idx += blockDim.x) {
+                const int lo = 2 * idx - (idx & (stride - 1));
+                const int hi = lo + stride;
+                const bool up = (lo & size) == 0;
+                const T a = v[lo], b = v[hi];
+                if ((a > b) == up) {
+                    v[lo] = b;
+                    v[hi] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// 1 + #{sorted gaps > eps} over v[0..P) (sorted).
+__device__ __forceinline__ int gap_count(const double* v, int P, double eps)
+{
+    int gaps = 0;
+    for (int k = 1 + threadIdx.x; k < P; k += blockDim.x) gaps += (v[k] - v[k - 1] > eps) ? 1 : 0;
+    // block of one wave: reduce with ballot-free shuffle sum
+    for (int o = 32; o > 0; o >>= 1) gaps += __shfl_down(gaps, o);
+    return 1 + __shfl(gaps, 0);
+}
+
+// CONFIG 2-D: number of 1-D DBSCAN(eps, minPts=1) clusters of the kept peak
+// values (first peak dropped); points whose segment exceeds the LDS
+// capacity are left for dbscan1d_big_kernel (marked -2).
+__global__ __launch_bounds__(64) void dbscan1d_count_kernel(const int* __restrict__ flags,
+                                                            const int* __restrict__ counts,
+                                                            const long long* __restrict__ off,
+                                                            const double* __restrict__ vals, double eps, int cap,
+                                                            int* __restrict__ out, long long n)
+{
+    extern __shared__ double lds[];
+    const long long p = blockIdx.x;
+    if (p >= n) return;
+    const int fl = flags[p];
+    if (fl != 1) {
+        if (threadIdx.x == 0) out[p] = fl == -1 ? -1 : 0;
+        return;
+    }
+    const int P = counts[p] > 1 ? counts[p] - 1 : 0;
+    if (P == 0) {
+        if (threadIdx.x == 0) out[p] = 0;
+        return;
+    }
+    if (P > cap) {
+        if (threadIdx.x == 0) out[p] = -2;
+        return;
+    }
+    int M = 1;
+    while (M < P) M <<= 1;
+    const double* v = vals + off[p] + 1;
+    for (int k = threadIdx.x; k < M; k += blockDim.x) lds[k] = k < P ? v[k] : INFINITY;
+    __syncthreads();
+    bitonic_sort(lds, M);
+    const int c = gap_count(lds, P, eps);
+    if (threadIdx.x == 0) out[p] = c;
+}
+
+__global__ __launch_bounds__(64) void dbscan1d_big_kernel(const int* __restrict__ counts,
+                                                          const long long* __restrict__ off,
+                                                          const double* __restrict__ vals,
+                                                          const long long* __restrict__ big, int nbig,
+                                                          double* __restrict__ scratch,
+                                                          const long long* __restrict__ scratch_off, double eps,
+                                                          int* __restrict__ out)
+{
+    const int b = blockIdx.x;
+    if (b >= nbig) return;
+    const long long p = big[b];
+    const int P = counts[p] - 1;
+    int M = 1;
+    while (M < P) M <<= 1;
+    double* w = scratch + scratch_off[b];
+    const double* v = vals + off[p] + 1;
+    for (int k = threadIdx.x; k < M; k += blockDim.x) w[k] = k < P ? v[k] : INFINITY;
+    __syncthreads();
+    bitonic_sort(w, M);
+    const int c = gap_count(w, P, eps);
+    if (threadIdx.x == 0) out[p] = c;
+}
+
+// REFERENCE 2-D: Bifurcation/src/bifurcationCUDA.cu:516-572 over the kept
+// intervals.  The outer loop (with its stack quirk) is wave-uniform; each
+// pass over k is split across the 64 lanes, which is exact because pass k
+// only reads and writes helpful[k] and helpful[P+k].  distance() there sees
+// zeroed peak values, so hypotf(0, dy) == |float(dy)| (:575-583).
+__global__ __launch_bounds__(64) void dbscan_ref_kernel(const int* __restrict__ flags,
+                                                        const int* __restrict__ counts,
+                                                        const long long* __restrict__ off,
+                                                        const unsigned* __restrict__ times, double hp, double eps,
+                                                        int* __restrict__ helpful_all, int* __restrict__ out,
+                                                        long long n)
+{
+    const long long p = blockIdx.x;
+    if (p >= n) return;
+    const int fl = flags[p];
+    if (fl != 1) {
+        if (threadIdx.x == 0) out[p] = fl == -1 ? -1 : 0;  // dbscanCUDA, :387-395
+        return;
+    }
+    const int P = counts[p] > 1 ? counts[p] - 1 : 0;
+    if (P <= 0 || P == 1) {
+        if (threadIdx.x == 0) out[p] = P;  // :521-525
+        return;
+    }
+    const unsigned* t = times + off[p];
+    int* helpful = helpful_all + 2 * off[p];
+    for (int k = threadIdx.x; k < 2 * P; k += blockDim.x) helpful[k] = 0;
+    __syncthreads();
+    int cluster = 0, NumNeibor = 0;
+    auto iv = [&](int k) { return (double)((long long)t[k + 1] - (long long)t[k]) * hp; };
+    auto pass = [&](int i) {
+        const double yi = iv(i);
+        int found = 0;
+        for (int k = threadIdx.x; k < P - 1; k += blockDim.x) {
+            if (i != k && helpful[k] == 0) {
+                const double yk = iv(k);
+                const double d = (yi == yk) ? 0.0 : (double)fabsf((float)(yk - yi));
+                if (d < eps) {
+                    helpful[k] = cluster;
+                    helpful[P + k] = k;
+                    ++found;
+                }
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) found += __shfl_down(found, o);
+        NumNeibor += __shfl(found, 0);
+        __syncthreads();
+    };
+    for (int i = 0; i < P; i++) {
+        if (NumNeibor >= 1) {
+            i = helpful[P + NumNeibor - 1];
+            __syncthreads();
+            if (threadIdx.x == 0) helpful[P + NumNeibor - 1] = 0;
+            __syncthreads();
+            NumNeibor = NumNeibor - 1;
+            pass(i);
+        } else if (helpful[i] == 0) {
+            __syncthreads();
+            NumNeibor = 0;
+            ++cluster;
+            if (threadIdx.x == 0) helpful[i] = cluster;
+            __syncthreads();
+            pass(i);
+        }
+    }
+    if (threadIdx.x == 0) out[p] = cluster - 1;
+}
+
+static void bif2d_analysis(CallCtx& ctx, const chaos_options& o, BifDevice& R, double eps, int* d_out)
+{
+    const long long n = R.n;
+    ctx.begin();
+    if (o.semantics == CHAOS_SEM_REFERENCE) {
+        DevBuf<int> helpful((size_t)std::max(1LL, 2 * R.total));
+        hipLaunchKernelGGL(dbscan_ref_kernel, dim3((unsigned)n), dim3(64), 0, ctx.stream(), R.flags.get(),
+                           R.counts.get(), R.off.get(), R.times.get(), R.hp, eps, helpful.get(), d_out, n);
+        CHAOS_HIP(hipGetLastError());
+        ctx.sync();
+    } else {
+        int maxP = 0;
+        for (long long p = 0; p < n; ++p)
+            if (R.h_flags[p] == 1) maxP = std::max(maxP, R.h_counts[p] - 1);
+        const int cap_limit = 4096;
+        int M = 1;
+        while (M < std::min(maxP, cap_limit)) M <<= 1;
+        const int cap = std::min(M, cap_limit);
+        hipLaunchKernelGGL(dbscan1d_count_kernel, dim3((unsigned)n), dim3(64), (size_t)M * sizeof(double),
+                           ctx.stream(), R.flags.get(), R.counts.get(), R.off.get(), R.vals.get(), eps, cap, d_out, n);
+        CHAOS_HIP(hipGetLastError());
+        if (maxP > cap) {
+            std::vector<long long> big, soff;
+            long long tot = 0;
+            for (long long p = 0; p < n; ++p)
+                if (R.h_flags[p] == 1 && R.h_counts[p] - 1 > cap) {
+                    big.push_back(p);
+                    soff.push_back(tot);
+                    long long m = 1;
+                    while (m < R.h_counts[p] - 1) m <<= 1;
+                    tot += m;
+                }
+            DevBuf<long long> d_big(big.size()), d_soff(soff.size());
+            DevBuf<double> scratch((size_t)tot);
+            CHAOS_HIP(hipMemcpyAsync(d_big.get(), big.data(), big.size() * 8, hipMemcpyHostToDevice, ctx.stream()));
+            CHAOS_HIP(hipMemcpyAsync(d_soff.get(), soff.data(), soff.size() * 8, hipMemcpyHostToDevice, ctx.stream()));
+            hipLaunchKernelGGL(dbscan1d_big_kernel, dim3((unsigned)big.size()), dim3(64), 0, ctx.stream(),
+                               R.counts.get(), R.off.get(), R.vals.get(), d_big.get(), (int)big.size(),
+                               scratch.get(), d_soff.get(), eps, d_out);
+            CHAOS_HIP(hipGetLastError());
+            ctx.sync();
+        }
+    }
+    ctx.add_analysis_ms(ctx.end());
+    ctx.add_launch();
+}
+
+static BifInputs bif_inputs(double tMax, int nPts, double h, int nx, const double* ic, const double* ranges,
+                            const int* idx, int writableVar, double maxValue, double transientTime,
+                            const double* values, int na, int preScaller, double eps, int dims)
+{
+    BifInputs in{};
+    in.tMax = tMax;
+    in.nPts = nPts;
+    in.h = h;
+    in.nx = nx;
+    in.ic = ic;
+    in.ranges = ranges;
+    in.idx = idx;
+    in.writableVar = writableVar;
+    in.maxValue = maxValue;
+    in.transientTime = transientTime;
+    in.values = values;
+    in.na = na;
+    in.preScaller = preScaller;
+    in.eps = eps;
+    in.dims = dims;
+    return in;
+}
+
+}  // namespace chaos
+
+using namespace chaos;
+
+extern "C" int chaos_bif1d(const chaos_options* opt, double tMax, int nPts, double h, int amountOfInitialConditions,
+                           const double* initialConditions, const double* ranges, const int* indicesOfMutVars,
+                           int writableVar, double maxValue, double transientTime, const double* values,
+                           int amountOfValues, int preScaller, int* counts, double* first, double* peaks,
+                           double* intervals, long long capacity, long long* total)
+{
+    return guarded("chaos_bif1d", [&] {
+        chaos_options o = resolve_options(opt);
+        CHAOS_REQUIRE(counts && first && total, "NULL output");
+        CHAOS_REQUIRE(capacity == 0 || (peaks && intervals), "NULL peaks/intervals with capacity > 0");
+        BifInputs in = bif_inputs(tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges,
+                                  indicesOfMutVars, writableVar, maxValue, transientTime, values, amountOfValues,
+                                  preScaller, 0.0, 1);
+        CallCtx ctx(o);
+        BifDevice R;
+        run_bif(ctx, o, in, R);
+        std::vector<double> v((size_t)R.total);
+        std::vector<unsigned> t((size_t)R.total);
+        std::vector<long long> off((size_t)R.n);
+        if (R.total > 0) {
+            CHAOS_HIP(hipMemcpyAsync(v.data(), R.vals.get(), R.total * 8, hipMemcpyDeviceToHost, ctx.stream()));
+            CHAOS_HIP(hipMemcpyAsync(t.data(), R.times.get(), R.total * 4, hipMemcpyDeviceToHost, ctx.stream()));
+        }
+        CHAOS_HIP(hipMemcpyAsync(off.data(), R.off.get(), R.n * 8, hipMemcpyDeviceToHost, ctx.stream()));
+        ctx.sync();
+        // Reference output rules (bifurcationCUDA.cu:359-367,498-510,748-763):
+        // flag -1 -> one line with the first sample; flag 0 -> nothing;
+        // otherwise the peaks after the first, intervals (t[k+1]-t[k])*h*pre.
+        long long k_out = 0;
+        for (long long p = 0; p < R.n; ++p) {
+            first[p] = R.h_first[p];
+            const int fl = R.h_flags[p];
+            if (fl == -1) {
+                counts[p] = -1;
+                continue;
+            }
+            if (fl != 1 || R.h_counts[p] < 2) {
+                counts[p] = 0;
+                continue;
+            }
+            const int P = R.h_counts[p] - 1;
+            counts[p] = P;
+            for (int k = 0; k < P; ++k, ++k_out) {
+                if (k_out < capacity) {
+                    const long long e = off[p] + k;
+                    peaks[k_out] = v[e + 1];
+                    intervals[k_out] = (double)((long long)t[e + 1] - (long long)t[e]) * R.hp;
+                }
+            }
+        }
+        *total = k_out;
+        ctx.finish();
+        if (k_out > capacity) fail(CHAOS_E_CAPACITY, "need capacity %lld, got %lld", k_out, capacity);
+    });
+}
+
+extern "C" int chaos_bifurcation1D(const chaos_options* opt, double tMax, int nPts, double h,
+                                   int amountOfInitialConditions, const double* initialConditions,
+                                   const double* ranges, const int* indicesOfMutVars, int writableVar,
+                                   double maxValue, double transientTime, const double* values,
+                                   int amountOfValues, int preScaller, const char* OUT_FILE_PATH)
+{
+    return guarded("chaos_bifurcation1D", [&] {
+        CHAOS_REQUIRE(OUT_FILE_PATH, "OUT_FILE_PATH is NULL");
+        CHAOS_REQUIRE(nPts >= 1, "nPts must be >= 1");
+        TextFile f(OUT_FILE_PATH);
+        std::vector<int> counts(nPts);
+        std::vector<double> first(nPts);
+        long long total = 0;
+        int rc = chaos_bif1d(opt, tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges,
+                             indicesOfMutVars, writableVar, maxValue, transientTime, values, amountOfValues,
+                             preScaller, counts.data(), first.data(), nullptr, nullptr, 0, &total);
+        if (rc != CHAOS_OK && rc != CHAOS_E_CAPACITY) throw Error(rc, chaos_last_error());
+        std::vector<double> pk((size_t)std::max(1LL, total)), iv((size_t)std::max(1LL, total));
+        if (total > 0) {
+            rc = chaos_bif1d(opt, tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges,
+                             indicesOfMutVars, writableVar, maxValue, transientTime, values, amountOfValues,
+                             preScaller, counts.data(), first.data(), pk.data(), iv.data(), total, &total);
+            if (rc != CHAOS_OK) throw Error(rc, chaos_last_error());
+        }
+        // header at default precision, body at 12 digits (:645-647,746-763)
+        write_range_header(f, ranges, 1, 6);
+        long long k = 0;
+        for (int p = 0; p < nPts; ++p) {
+            const double x = grid_value(p, nPts, ranges[0], ranges[1]);
+            if (counts[p] == -1) {
+                f.put_g(x, 12);
+                f.put(", ", 2);
+                f.put_g(first[p], 12);
+                f.put(", 0\n");
+                continue;
+            }
+            for (int j = 0; j < counts[p]; ++j, ++k) {
+                f.put_g(x, 12);
+                f.put(", ", 2);
+                f.put_g(pk[k], 12);
+                f.put(", ", 2);
+                f.put_g(iv[k], 12);
+                f.put("\n", 1);
+            }
+        }
+        f.close();
+    });
+}
+
+extern "C" int chaos_bif2d(const chaos_options* opt, double tMax, int nPts, double h, int amountOfInitialConditions,
+                           const double* initialConditions, const double* ranges, const int* indicesOfMutVars,
+                           int writableVar, double maxValue, double transientTime, const double* values,
+                           int amountOfValues, int preScaller, double eps, int* out)
+{
+    return guarded("chaos_bif2d", [&] {
+        chaos_options o = resolve_options(opt);
+        CHAOS_REQUIRE(out, "out is NULL");
+        BifInputs in = bif_inputs(tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges,
+                                  indicesOfMutVars, writableVar, maxValue, transientTime, values, amountOfValues,
+                                  preScaller, eps, 2);
+        CallCtx ctx(o);
+        BifDevice R;
+        run_bif(ctx, o, in, R);
+        DevBuf<int> tmp;
+        int* d_out = out;
+        if (!o.device_buffers) {
+            tmp.alloc((size_t)R.n);
+            d_out = tmp.get();
+        }
+        bif2d_analysis(ctx, o, R, eps, d_out);
+        if (!o.device_buffers)
+            CHAOS_HIP(hipMemcpyAsync(out, d_out, R.n * sizeof(int), hipMemcpyDeviceToHost, ctx.stream()));
+        ctx.sync();
+        ctx.finish();
+    });
+}
+
+extern "C" int chaos_bifurcation2D(const chaos_options* opt, double tMax, int nPts, double h,
+                                   int amountOfInitialConditions, const double* initialConditions,
+                                   const double* ranges, const int* indicesOfMutVars, int writableVar,
+                                   double maxValue, double transientTime, const double* values,
+                                   int amountOfValues, int preScaller, double eps, const char* OUT_FILE_PATH)
+{
+    return guarded("chaos_bifurcation2D", [&] {
+        CHAOS_REQUIRE(OUT_FILE_PATH, "OUT_FILE_PATH is NULL");
+        CHAOS_REQUIRE(nPts >= 1, "nPts must be >= 1");
+        chaos_options o = resolve_options(opt);
+        o.device_buffers = 0;
+        TextFile f(OUT_FILE_PATH);
+        std::vector<int> res((size_t)nPts * nPts);
+        int rc = chaos_bif2d(&o, tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges,
+                             indicesOfMutVars, writableVar, maxValue, transientTime, values, amountOfValues,
+                             preScaller, eps, res.data());
+        if (rc != CHAOS_OK) throw Error(rc, chaos_last_error());
+        write_range_header(f, ranges, 2, 6);  // :109-110
+        write_map_int(f, res.data(), (long long)nPts * nPts, nPts);
+        f.close();
+    });
+}

@@ -1,0 +1,154 @@
+// peaks.hpp -- online peak detection over a lane's sample stream, plus the
+// wavefront-compacted peak store.
+//
+// The detector replays the reference peakFinder
+// (Bifurcation/src/bifurcationCUDA.cu:469-514, same text in
+// Basins/src/basinsCUDA.cu:382-440) one sample at a time, in registers,
+// instead of re-reading a stored trajectory:
+//   candidate at i:   d[i]-d[i-1] > 1e-13  &&  d[i] >= d[i+1]
+//   plateau scan j:   d[j] < d[j+1]          -> reject, next candidate i = j+2
+//                     d[j]-d[j+1] > 1e-13    -> peak (d[j], trunc((i+j)/2)),
+//                                               next candidate i = j+2
+// (the reference's i = j+1 followed by the loop's ++i skips index j+1).
+// All comparisons are on exact differences of adjacent samples:
+// d[c]-d[c+1] == -(d[c+1]-d[c]) and d[c] >= d[c+1] <=> d[c+1]-d[c] <= 0 for
+// finite samples, so one subtraction per sample feeds all three tests.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace chaos {
+
+struct PeakDetector {
+    double p;      // last sample d[t-1]
+    double dn;     // d[t-1] - d[t-2]
+    int pstart;    // plateau start index
+    bool plat;     // scanning a plateau
+    bool skip;     // next index is not a candidate
+
+    __device__ __forceinline__ void reset()
+    {
+        p = 0.0;
+        dn = 0.0;
+        pstart = 0;
+        plat = false;
+        skip = false;
+    }
+
+    // Feed sample d[t] = cur (t >= 0).  For t >= 2 the index c = t-1 is
+    // examined; on a peak, returns true with its value (d[c]) and time.
+    __device__ __forceinline__ bool feed(int t, double cur, double& val, int& time)
+    {
+        const double dnx = cur - p;  // d[c+1] - d[c]
+        bool emit = false;
+        if (t >= 2) {
+            const int c = t - 1;
+            const bool c1 = dn > 1e-13;     // d[c] - d[c-1] > 1e-13
+            const bool c2 = dnx <= 0.0;     // d[c] >= d[c+1]
+            const bool c3 = dnx < -1e-13;   // d[c] - d[c+1] > 1e-13
+            if (plat) {
+                if (!c2) {
+                    plat = false;
+                    skip = true;
+                } else if (c3) {
+                    emit = true;
+                    val = p;
+                    time = (pstart + c) >> 1;
+                    plat = false;
+                    skip = true;
+                }
+            } else if (skip) {
+                skip = false;
+            } else if (c1 && c2) {
+                if (c3) {
+                    emit = true;
+                    val = p;
+                    time = c;
+                    skip = true;
+                } else {
+                    plat = true;
+                    pstart = c;
+                }
+            }
+        }
+        dn = dnx;
+        p = cur;
+        return emit;
+    }
+};
+
+// One stored peak: value, sample index, and (sequence number << 6 | lane).
+struct PeakEntry {
+    double v;
+    unsigned t;
+    unsigned kl;
+};
+
+constexpr int kChunk = 256;  // entries per chunk (4 KiB)
+
+// Pool of fixed-size chunks handed out to waves by an atomic head.  Each
+// wave appends its lanes' peaks in time order with one ballot + mbcnt per
+// sample that produced any peak, so a chunk holds one wave's entries densely.
+struct PeakPool {
+    PeakEntry* entries;     // [nchunks * kChunk]
+    int* owner;             // [nchunks] global wave id
+    int* fill;              // [nchunks] entries used
+    unsigned* head;         // next free chunk
+    int* overflow;          // set when the pool ran out
+    unsigned nchunks;
+};
+
+struct WaveCursor {
+    unsigned chunk;  // current chunk (nchunks when none / exhausted)
+    int fill;        // entries used in it
+};
+
+__device__ __forceinline__ unsigned lane_prefix(unsigned long long mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+__device__ __forceinline__ int first_lane(unsigned long long active) { return __ffsll((long long)active) - 1; }
+
+// Close the cursor's chunk (record its fill) -- call from all active lanes.
+__device__ __forceinline__ void close_chunk(const PeakPool& pool, WaveCursor& cur)
+{
+    const unsigned long long act = __ballot(1);
+    if (cur.chunk < pool.nchunks && (int)__lane_id() == first_lane(act)) pool.fill[cur.chunk] = cur.fill;
+}
+
+// Append the emitting lanes' entries (wave-uniform control flow required).
+__device__ __forceinline__ void append_peaks(const PeakPool& pool, WaveCursor& cur, int wave, bool emit,
+                                             double v, int t, int k)
+{
+    const unsigned long long m = __ballot(emit);
+    if (m == 0ull) return;
+    const int n = __popcll(m);
+    if (cur.fill + n > kChunk) {
+        const unsigned long long act = __ballot(1);
+        const bool leader = (int)__lane_id() == first_lane(act);
+        if (cur.chunk < pool.nchunks && leader) pool.fill[cur.chunk] = cur.fill;
+        unsigned nc = 0;
+        if (leader) nc = atomicAdd(pool.head, 1u);
+        nc = __shfl(nc, first_lane(act));
+        if (nc < pool.nchunks) {
+            if (leader) pool.owner[nc] = wave;
+        } else {
+            if (leader) atomicExch(pool.overflow, 1);
+            nc = pool.nchunks;
+        }
+        cur.chunk = nc;
+        cur.fill = 0;
+    }
+    if (emit && cur.chunk < pool.nchunks) {
+        const unsigned pos = (unsigned)cur.fill + lane_prefix(m);
+        PeakEntry e;
+        e.v = v;
+        e.t = (unsigned)t;
+        e.kl = ((unsigned)k << 6) | (unsigned)__lane_id();
+        pool.entries[(size_t)cur.chunk * kChunk + pos] = e;
+    }
+    cur.fill += n;
+}
+
+}  // namespace chaos

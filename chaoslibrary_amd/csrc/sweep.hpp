@@ -1,0 +1,54 @@
+// sweep.hpp -- arguments of the transient+record sweep kernels that feed the
+// Bifurcation and Basins analyses.
+#pragma once
+
+#include "peaks.hpp"
+#include "runtime.hpp"
+#include "systems.hpp"
+
+namespace chaos {
+
+enum SweepKind : int {
+    KIND_BIF = 0,          // store every peak (value, time) -> Bifurcation 1D/2D
+    KIND_BASINS_REF = 1,   // mean peak / mean interval (+ sentinels)
+    KIND_BASINS_CFG = 2    // mean X[0], mean X[NX-1], peak std
+};
+
+enum CheckMode : int {
+    CHK_BIF_REF = 0,     // v!=v || v==v+1 ; (double)fabsf((float)v) > max ; sqrtf(fabsf((float)t)) < 1e-9
+    CHK_BASINS_REF = 1,  // isnan||isinf     ; (double)fabsf((float)v) > max ; sqrt(fabs(t)) < 1e-9
+    CHK_CONFIG = 2       // !isfinite        ; fabs(v) > max                 ; sqrt(t) < 1e-9
+};
+
+struct SweepArgs {
+    double values[8];
+    double ic[4];
+    double r[4];          // ranges (dim 0: r[0..1], dim 1: r[2..3])
+    int idx0, idx1;       // swept index into values (Bifurcation) or X (Basins); idx1 < 0 for 1-D
+    int nPts;             // grid side
+    int dims;             // 1 or 2
+    long long n_points;   // nPts or nPts^2
+    int S;                // transient steps
+    int B;                // recorded samples
+    int pre;              // steps per sample (Basins; Bifurcation records every step)
+    int check;            // CheckMode
+    double maxValue;      // divergence bound (0 = off)
+    double L;             // |v| < L guarantees "not diverged" (fast path)
+    double hp;            // interval unit: h * preScaller
+    StepConst sc;
+    // outputs (per point)
+    int* flags;
+    int* counts;          // peaks found (including the first, later dropped)
+    double* first;        // first recorded sample
+    double* f0;           // features (Basins)
+    double* f1;
+    double* f2;
+    PeakPool pool;        // Bifurcation peak store
+};
+
+// Fast-path bound for the divergence test of each check mode.
+double divergence_fast_bound(int check, double maxValue);
+
+void launch_sweep(CallCtx& ctx, int system, int writableVar, int kind, const SweepArgs& A);
+
+}  // namespace chaos

@@ -120,6 +120,13 @@ int chaos_device_synchronize(int device);
 int chaos_integrate(const chaos_options* opt, int n, const double* params,
                     double* states, long long nsteps, double h);
 
+/* Peak finder over caller rows (Bifurcation/src/bifurcationCUDA.cu:469-514,
+ * peakFinder): data[rows][len]; counts[r] = peaks kept after dropping the
+ * first, peaks[r*len+k] their values, intervals[r*len+k] = (t[k+1]-t[k])*h. */
+int chaos_peak_finder(const chaos_options* opt, const double* data, int rows,
+                      int len, double h, int* counts, double* peaks,
+                      double* intervals);
+
 /* ---------------- LLE (LLE/src/LLECUDA.cu:205-428) ---------------------- */
 /* out[j*nPts + i] = LLE(A_i, B_j), A = linspace(ranges[0], ranges[1], nPts)
  * on values[indicesOfMutVars[0]], B likewise on ranges[2..3] /
@@ -181,8 +188,9 @@ int chaos_bifurcation2D(const chaos_options* opt, double tMax, int nPts,
 /* labels[p], p = j*nPts + i over the IC grid (X[indicesOfMutVars[0]] over
  * ranges[0..1], X[indicesOfMutVars[1]] over ranges[2..3]); flags[p] in
  * {1,0,-1}; features f[k*nPts*nPts + p] (REFERENCE k<2: average peak,
- * average interval; CONFIG k<3: z-scored mean x, mean of the last state
- * component, peak std).  features/flags may be NULL. */
+ * average interval with the reference's sentinels; CONFIG k<3: mean X[0],
+ * mean X[NX-1], std of the kept peaks -- before the z-score that feeds the
+ * clustering).  features/flags may be NULL. */
 int chaos_basins2d(const chaos_options* opt, double tMax, int nPts, double h,
                    int amountOfInitialConditions,
                    const double* initialConditions, const double* ranges,

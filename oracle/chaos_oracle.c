@@ -666,9 +666,10 @@ int orc_lle2d_omp(int sys, int sem, double tMax, double NT, int nPts, double h,
  * checks, then B samples with preScaller sub-steps) and :818-864 (REFERENCE:
  * avg peak / avg interval with sentinels 999 / (last sample, -1) / 1000).
  * CONFIG: f0 = mean X[0], f1 = mean X[nx-1] over the B recorded samples
- * (sequential sums), f2 = population std of the kept writableVar peaks
- * (Welford: n+=1; d=p-m; m+=d/n; M2+=d*(p-m); std = sqrt(M2/n), 0 if n<2);
- * divergent points get (0,0,0) and flag 0.
+ * (sequential sums), f2 = population std of the n kept writableVar peaks
+ * from s0 = sum p, s1 = sum fma(p,p,.): m = s0/n, var = s1/n - m*m,
+ * std = sqrt(var) if var > 0 and n >= 2, else 0; divergent points get
+ * (0,0,0) and flag 0.
  */
 int orc_basins_features(int sys, int sem, double tMax, int nPts, double h,
                         const double* ic, int nx, const double* ranges,
@@ -750,13 +751,17 @@ int orc_basins_features(int sys, int sem, double tMax, int nPts, double h,
                     f0[q] = s0 / B;
                     f1[q] = s1 / B;
                     int P = orc_peak_finder(data, B, pk, iv, h * preScaller);
-                    double m = 0, M2 = 0;
+                    double p0 = 0, p1 = 0;
                     for (int k = 0; k < P; ++k) {
-                        double d = pk[k] - m;
-                        m = m + d / (k + 1);
-                        M2 = M2 + d * (pk[k] - m);
+                        p0 = p0 + pk[k];
+                        p1 = fma(pk[k], pk[k], p1);
                     }
-                    f2[q] = P >= 2 ? sqrt(M2 / P) : 0.0;
+                    f2[q] = 0.0;
+                    if (P >= 2) {
+                        double m = p0 / P;
+                        double var = p1 / P - m * m;
+                        f2[q] = var > 0 ? sqrt(var) : 0.0;
+                    }
                 }
             }
         }
