@@ -305,3 +305,220 @@ def basinsOfAttraction_2(tMax, nPts, h, amountOfInitialConditions, initialCondit
     _check(lib.chaos_basinsOfAttraction_2(_opt_ref(options), tMax, nPts, h, amountOfInitialConditions, _dp(ic),
                                           _dp(rg), _ip(ix), writableVar, maxValue, transientTime, _dp(vals),
                                           amountOfValues, preScaller, eps, str(OUT_FILE_PATH).encode()))
+
+
+# ---------------------------------------------------------------------------
+# batched engine / job server / mixed-job stream
+# ---------------------------------------------------------------------------
+
+JOB_LLE2D, JOB_BIF1D, JOB_BIF2D, JOB_BASINS2D = 0, 1, 2, 3
+
+
+class Job(C.Structure):
+    _fields_ = [("kind", C.c_int), ("system", C.c_int), ("semantics", C.c_int), ("nPts", C.c_int),
+                ("tMax", C.c_double), ("NT", C.c_double), ("h", C.c_double), ("eps", C.c_double),
+                ("maxValue", C.c_double), ("transientTime", C.c_double), ("preScaller", C.c_int),
+                ("writableVar", C.c_int), ("min_pts", C.c_int), ("amountOfInitialConditions", C.c_int),
+                ("seed", C.c_uint64), ("initialConditions", C.c_double * 4), ("values", C.c_double * 8),
+                ("amountOfValues", C.c_int), ("indicesOfMutVars", C.c_int * 2), ("reserved", C.c_int),
+                ("ranges", C.c_double * 4)]
+
+
+class EngineParams(C.Structure):
+    _fields_ = [("block", C.c_int), ("waves_per_simd", C.c_int), ("steps_per_launch", C.c_int),
+                ("max_jobs", C.c_int), ("max_tiles", C.c_int), ("use_graph", C.c_int),
+                ("hbm_fraction", C.c_double)]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [("status", C.c_int), ("kind", C.c_int), ("t_submit_ms", C.c_double),
+                ("t_admit_ms", C.c_double), ("t_done_ms", C.c_double), ("rk4_steps", C.c_double),
+                ("footprint_bytes", C.c_longlong), ("n_points", C.c_longlong), ("result_len", C.c_longlong),
+                ("reran", C.c_int), ("reserved", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("launches", C.c_longlong), ("graph_replays", C.c_longlong), ("tiles", C.c_longlong),
+                ("busy_ms", C.c_double), ("finalize_ms", C.c_double), ("rk4_steps", C.c_double),
+                ("active_jobs", C.c_int), ("pending_jobs", C.c_int), ("hbm_in_use", C.c_longlong),
+                ("hbm_budget", C.c_longlong)]
+
+
+class StreamConfig(C.Structure):
+    _fields_ = [("lambda_", C.c_double), ("duration_s", C.c_double), ("horizon_scale", C.c_double),
+                ("seed", C.c_uint64), ("bif_min", C.c_int), ("bif_max", C.c_int), ("lle_min", C.c_int),
+                ("lle_max", C.c_int), ("basins_min", C.c_int), ("basins_max", C.c_int), ("mode", C.c_int),
+                ("max_jobs", C.c_int)]
+
+
+class StreamReport(C.Structure):
+    _fields_ = [("jobs", C.c_int), ("jobs_kind", C.c_int * 3), ("makespan_s", C.c_double),
+                ("rk4_steps", C.c_double), ("steps_per_s", C.c_double), ("jobs_per_s", C.c_double),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
+                ("p99_kind_ms", C.c_double * 3), ("busy_ms", C.c_double), ("launches", C.c_longlong)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["jobs_kind"] = list(self.jobs_kind)
+        d["p99_kind_ms"] = list(self.p99_kind_ms)
+        return d
+
+
+_SIGS3 = {
+    "chaos_engine_params_default": (C.c_int, [C.POINTER(EngineParams)]),
+    "chaos_server_create": (C.c_int, [C.c_int, C.POINTER(EngineParams), C.POINTER(C.c_void_p)]),
+    "chaos_server_destroy": (C.c_int, [C.c_void_p]),
+    "chaos_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Job), C.POINTER(C.c_longlong)]),
+    "chaos_server_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "chaos_server_drain": (C.c_int, [C.c_void_p]),
+    "chaos_server_job_info": (C.c_int, [C.c_void_p, C.c_longlong, C.POINTER(JobInfo)]),
+    "chaos_server_stats_get": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    "chaos_server_result_lle": (C.c_int, [C.c_void_p, C.c_longlong, _D]),
+    "chaos_server_result_bif2d": (C.c_int, [C.c_void_p, C.c_longlong, _I]),
+    "chaos_server_result_bif1d": (C.c_int, [C.c_void_p, C.c_longlong, _I, _D, _D, _D, C.c_longlong,
+                                            C.POINTER(C.c_longlong)]),
+    "chaos_server_result_basins": (C.c_int, [C.c_void_p, C.c_longlong, _I, _D, _I]),
+    "chaos_server_release": (C.c_int, [C.c_void_p, C.c_longlong]),
+    "chaos_stream_config_default": (C.c_int, [C.POINTER(StreamConfig)]),
+    "chaos_stream_run": (C.c_int, [C.c_int, C.POINTER(StreamConfig), C.POINTER(EngineParams),
+                                   C.POINTER(StreamReport), _D, C.c_int]),
+    "chaos_autotune": (C.c_int, [C.c_int, C.c_char_p, C.POINTER(EngineParams), C.POINTER(C.c_double)]),
+    "chaos_autotune_load": (C.c_int, [C.c_char_p, C.POINTER(EngineParams)]),
+}
+for _n, (_r, _a) in _SIGS3.items():
+    _fn = getattr(lib, _n)
+    _fn.restype = _r
+    _fn.argtypes = _a
+
+
+def engine_params(**kw):
+    p = EngineParams()
+    _check(lib.chaos_engine_params_default(C.byref(p)))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def make_job(kind, system, semantics, nPts, initialConditions, values, ranges, indicesOfMutVars, h=0.01,
+             tMax=0.0, NT=0.5, eps=0.0, maxValue=0.0, transientTime=0.0, preScaller=1, writableVar=0,
+             min_pts=20, seed=12345):
+    j = Job()
+    j.kind = kind
+    j.system = SYSTEMS[system] if isinstance(system, str) else int(system)
+    j.semantics = semantics
+    j.nPts = nPts
+    j.tMax, j.NT, j.h, j.eps, j.maxValue, j.transientTime = tMax, NT, h, eps, maxValue, transientTime
+    j.preScaller, j.writableVar, j.min_pts, j.seed = preScaller, writableVar, min_pts, seed
+    j.amountOfInitialConditions = len(initialConditions)
+    for i, v in enumerate(initialConditions):
+        j.initialConditions[i] = v
+    j.amountOfValues = len(values)
+    for i, v in enumerate(values):
+        j.values[i] = v
+    for i, v in enumerate(ranges):
+        j.ranges[i] = v
+    for i, v in enumerate(indicesOfMutVars):
+        j.indicesOfMutVars[i] = v
+    return j
+
+
+class Server:
+    """Job server over the batched engine (chaos_server_*)."""
+
+    def __init__(self, device=0, params=None):
+        self.h = C.c_void_p()
+        _check(lib.chaos_server_create(device, C.byref(params) if params is not None else None, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            _check(lib.chaos_server_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, job):
+        jid = C.c_longlong()
+        _check(lib.chaos_server_submit(self.h, C.byref(job), C.byref(jid)))
+        return jid.value
+
+    def step(self):
+        c = C.c_int()
+        _check(lib.chaos_server_step(self.h, C.byref(c)))
+        return c.value
+
+    def drain(self):
+        _check(lib.chaos_server_drain(self.h))
+
+    def info(self, jid):
+        i = JobInfo()
+        _check(lib.chaos_server_job_info(self.h, jid, C.byref(i)))
+        return i
+
+    def stats(self):
+        s = ServerStats()
+        _check(lib.chaos_server_stats_get(self.h, C.byref(s)))
+        return s
+
+    def result(self, jid, job):
+        """Results in the layout of the one-shot wrappers for the job's kind."""
+        n = job.nPts
+        if job.kind == JOB_LLE2D:
+            out = np.zeros((n, n))
+            _check(lib.chaos_server_result_lle(self.h, jid, _dp(out)))
+            return out
+        if job.kind == JOB_BIF2D:
+            out = np.zeros((n, n), dtype=np.int32)
+            _check(lib.chaos_server_result_bif2d(self.h, jid, _ip(out)))
+            return out
+        if job.kind == JOB_BIF1D:
+            info = self.info(jid)
+            cap = max(1, info.result_len)
+            counts, first = np.zeros(n, dtype=np.int32), np.zeros(n)
+            pk, iv = np.zeros(cap), np.zeros(cap)
+            tot = C.c_longlong()
+            _check(lib.chaos_server_result_bif1d(self.h, jid, _ip(counts), _dp(first), _dp(pk), _dp(iv), cap,
+                                                 C.byref(tot)))
+            return counts, first, pk[:tot.value], iv[:tot.value]
+        labels, flags = np.zeros((n, n), dtype=np.int32), np.zeros((n, n), dtype=np.int32)
+        nf = 2 if job.semantics == SEM_REFERENCE else 3
+        feats = np.zeros((nf, n, n))
+        _check(lib.chaos_server_result_basins(self.h, jid, _ip(labels), _dp(feats), _ip(flags)))
+        return labels, feats, flags
+
+    def release(self, jid):
+        _check(lib.chaos_server_release(self.h, jid))
+
+
+def stream_config(**kw):
+    c = StreamConfig()
+    _check(lib.chaos_stream_config_default(C.byref(c)))
+    for k, v in kw.items():
+        setattr(c, "lambda_" if k == "lam" else k, v)
+    return c
+
+
+def stream_run(cfg, params=None, device=0):
+    """Run the seeded mixed-job stream; returns (report, latencies_ms)."""
+    rep = StreamReport()
+    cap = 1 << 20
+    lat = np.zeros(cap)
+    _check(lib.chaos_stream_run(device, C.byref(cfg), C.byref(params) if params is not None else None,
+                                C.byref(rep), _dp(lat), cap))
+    return rep, lat[:rep.jobs].copy()
+
+
+def autotune(cache_path=None, device=0):
+    p = EngineParams()
+    rate = C.c_double()
+    _check(lib.chaos_autotune(device, cache_path.encode() if cache_path else None, C.byref(p), C.byref(rate)))
+    return p, rate.value
+
+
+def autotune_load(cache_path):
+    p = EngineParams()
+    _check(lib.chaos_autotune_load(cache_path.encode(), C.byref(p)))
+    return p

@@ -15,9 +15,72 @@
 #include "csv.hpp"
 #include "dbscan.hpp"
 #include "dispatch.hpp"
+#include "basins.hpp"
 #include "sweep.hpp"
 
 using namespace chaos;
+
+namespace chaos {
+
+SweepArgs make_basins_args(const chaos_options& o, double tMax, int nPts, double h, int nx, const double* ic,
+                           const double* ranges, const int* idx, int writableVar, double maxValue,
+                           double transientTime, const double* values, int na, int preScaller, double eps)
+{
+    check_system(o.system, nx, na);
+    CHAOS_REQUIRE(nPts >= 2, "nPts must be >= 2 (got %d)", nPts);
+    CHAOS_REQUIRE(h > 0 && std::isfinite(h), "h must be positive");
+    CHAOS_REQUIRE(preScaller >= 1, "preScaller must be >= 1");
+    CHAOS_REQUIRE(ic && ranges && idx && values, "NULL input array");
+    CHAOS_REQUIRE(idx[0] >= 0 && idx[0] < nx && idx[1] >= 0 && idx[1] < nx,
+                  "indicesOfMutVars index the state vector here (basinsCUDA.cu:259) and must be in [0,%d)", nx);
+    CHAOS_REQUIRE(writableVar >= 0 && writableVar < nx, "writableVar out of range");
+    CHAOS_REQUIRE(eps > 0, "eps must be positive");
+    const long long n = (long long)nPts * nPts;
+    CHAOS_REQUIRE(n <= (1LL << 31) - 64, "grid too large");
+    SweepArgs A{};
+    for (int k = 0; k < na; ++k) A.values[k] = values[k];
+    for (int k = 0; k < nx; ++k) A.ic[k] = ic[k];
+    for (int k = 0; k < 4; ++k) A.r[k] = ranges[k];
+    A.idx0 = idx[0];
+    A.idx1 = idx[1];
+    A.nPts = nPts;
+    A.dims = 2;
+    A.n_points = n;
+    A.S = horizon_steps(transientTime, h, o.semantics);
+    A.B = record_samples(tMax, h, preScaller, o.semantics);
+    CHAOS_REQUIRE(A.B >= 1, "tMax/h/preScaller must give at least one recorded sample");
+    A.pre = preScaller;
+    A.check = o.semantics == CHAOS_SEM_REFERENCE ? CHK_BASINS_REF : CHK_CONFIG;
+    A.maxValue = maxValue;
+    A.L = divergence_fast_bound(A.check, maxValue);
+    A.hp = h * preScaller;  // basinsCUDA.cu:684
+    A.sc = make_step_const(h);
+    A.pool.nchunks = 0;
+    return A;
+}
+
+void basins_cluster(CallCtx& ctx, bool ref, long long n, const double* f0, const double* f1, const double* f2,
+                    const int* flags, double eps, int minPts, int* labels)
+{
+    if (ref) {
+        const double* pts[2] = {f0, f1};
+        DbscanParams prm{2, eps, 1, true};
+        dbscan_device(ctx, pts, flags, n, prm, labels);
+    } else {
+        // z-score a copy; the caller keeps the raw features
+        DevBuf<double> z(3 * (size_t)n);
+        CHAOS_HIP(hipMemcpyAsync(z.get(), f0, n * 8, hipMemcpyDeviceToDevice, ctx.stream()));
+        CHAOS_HIP(hipMemcpyAsync(z.get() + n, f1, n * 8, hipMemcpyDeviceToDevice, ctx.stream()));
+        CHAOS_HIP(hipMemcpyAsync(z.get() + 2 * n, f2, n * 8, hipMemcpyDeviceToDevice, ctx.stream()));
+        double* f[3] = {z.get(), z.get() + n, z.get() + 2 * n};
+        zscore_device(ctx, f, 3, flags, n, nullptr);
+        const double* pts[3] = {f[0], f[1], f[2]};
+        DbscanParams prm{3, eps, minPts, false};
+        dbscan_device(ctx, pts, flags, n, prm, labels);
+    }
+}
+
+}  // namespace chaos
 
 extern "C" int chaos_basins2d(const chaos_options* opt, double tMax, int nPts, double h, int amountOfInitialConditions,
                               const double* initialConditions, const double* ranges, const int* indicesOfMutVars,
@@ -27,41 +90,12 @@ extern "C" int chaos_basins2d(const chaos_options* opt, double tMax, int nPts, d
 {
     return guarded("chaos_basins2d", [&] {
         chaos_options o = resolve_options(opt);
-        const int nx = amountOfInitialConditions, na = amountOfValues;
-        check_system(o.system, nx, na);
         CHAOS_REQUIRE(labels, "labels is NULL");
-        CHAOS_REQUIRE(nPts >= 2, "nPts must be >= 2 (got %d)", nPts);
-        CHAOS_REQUIRE(h > 0 && std::isfinite(h), "h must be positive");
-        CHAOS_REQUIRE(preScaller >= 1, "preScaller must be >= 1");
-        CHAOS_REQUIRE(initialConditions && ranges && indicesOfMutVars && values, "NULL input array");
-        CHAOS_REQUIRE(indicesOfMutVars[0] >= 0 && indicesOfMutVars[0] < nx && indicesOfMutVars[1] >= 0 &&
-                          indicesOfMutVars[1] < nx,
-                      "indicesOfMutVars index the state vector here (basinsCUDA.cu:259) and must be in [0,%d)", nx);
-        CHAOS_REQUIRE(writableVar >= 0 && writableVar < nx, "writableVar out of range");
-        CHAOS_REQUIRE(eps > 0, "eps must be positive");
-        const long long n = (long long)nPts * nPts;
-        CHAOS_REQUIRE(n <= (1LL << 31) - 64, "grid too large");
+        SweepArgs A = make_basins_args(o, tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges,
+                                       indicesOfMutVars, writableVar, maxValue, transientTime, values,
+                                       amountOfValues, preScaller, eps);
+        const long long n = A.n_points;
         const bool ref = o.semantics == CHAOS_SEM_REFERENCE;
-
-        SweepArgs A{};
-        for (int k = 0; k < na; ++k) A.values[k] = values[k];
-        for (int k = 0; k < nx; ++k) A.ic[k] = initialConditions[k];
-        for (int k = 0; k < 4; ++k) A.r[k] = ranges[k];
-        A.idx0 = indicesOfMutVars[0];
-        A.idx1 = indicesOfMutVars[1];
-        A.nPts = nPts;
-        A.dims = 2;
-        A.n_points = n;
-        A.S = horizon_steps(transientTime, h, o.semantics);
-        A.B = record_samples(tMax, h, preScaller, o.semantics);
-        CHAOS_REQUIRE(A.B >= 1, "tMax/h/preScaller must give at least one recorded sample");
-        A.pre = preScaller;
-        A.check = ref ? CHK_BASINS_REF : CHK_CONFIG;
-        A.maxValue = maxValue;
-        A.L = divergence_fast_bound(A.check, maxValue);
-        A.hp = h * preScaller;  // basinsCUDA.cu:684
-        A.sc = make_step_const(h);
-
         CallCtx ctx(o);
         DevBuf<int> d_flags(n), d_labels;
         DevBuf<double> d_f(3 * (size_t)n);
@@ -69,7 +103,6 @@ extern "C" int chaos_basins2d(const chaos_options* opt, double tMax, int nPts, d
         A.f0 = d_f.get();
         A.f1 = d_f.get() + n;
         A.f2 = d_f.get() + 2 * n;
-        A.pool.nchunks = 0;
         int* out_labels = labels;
         if (!o.device_buffers) {
             d_labels.alloc(n);
@@ -79,22 +112,8 @@ extern "C" int chaos_basins2d(const chaos_options* opt, double tMax, int nPts, d
         launch_sweep(ctx, o.system, writableVar, ref ? KIND_BASINS_REF : KIND_BASINS_CFG, A);
         ctx.add_kernel_ms(ctx.end());
         ctx.add_steps((double)n * ((double)A.S + (double)A.B * A.pre));
-
         ctx.begin();
-        if (ref) {
-            const double* pts[2] = {A.f0, A.f1};
-            DbscanParams prm{2, eps, 1, true};
-            dbscan_device(ctx, pts, A.flags, n, prm, out_labels);
-        } else {
-            // z-score a copy; the caller gets the raw features
-            DevBuf<double> z(3 * (size_t)n);
-            CHAOS_HIP(hipMemcpyAsync(z.get(), d_f.get(), z.bytes(), hipMemcpyDeviceToDevice, ctx.stream()));
-            double* f[3] = {z.get(), z.get() + n, z.get() + 2 * n};
-            zscore_device(ctx, f, 3, A.flags, n, nullptr);
-            const double* pts[3] = {f[0], f[1], f[2]};
-            DbscanParams prm{3, eps, o.min_pts, false};
-            dbscan_device(ctx, pts, A.flags, n, prm, out_labels);
-        }
+        basins_cluster(ctx, ref, n, A.f0, A.f1, A.f2, A.flags, eps, o.min_pts, out_labels);
         ctx.add_analysis_ms(ctx.end());
         const hipMemcpyKind k = o.device_buffers ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
         if (!o.device_buffers)

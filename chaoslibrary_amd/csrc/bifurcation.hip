@@ -16,29 +16,10 @@
 #include "csv.hpp"
 #include "dispatch.hpp"
 #include "scan.hpp"
+#include "bifurcation.hpp"
 #include "sweep.hpp"
 
 namespace chaos {
-
-struct BifInputs {
-    double tMax, h, maxValue, transientTime, eps;
-    int nPts, nx, na, writableVar, preScaller, dims;
-    const double *ic, *ranges, *values;
-    const int* idx;
-};
-
-struct BifDevice {
-    long long n = 0;         // points
-    long long total = 0;     // stored peaks of flag-1 points (first peak included)
-    double hp = 0;           // interval unit
-    DevBuf<int> flags, counts;
-    DevBuf<double> first;
-    DevBuf<long long> off;   // segment offsets
-    DevBuf<double> vals;
-    DevBuf<unsigned> times;
-    std::vector<int> h_flags, h_counts;
-    std::vector<double> h_first;
-};
 
 __global__ void kept_kernel(const int* __restrict__ flags, const int* __restrict__ counts,
                             long long* __restrict__ kept, long long n)
@@ -63,7 +44,31 @@ __global__ void scatter_kernel(const PeakPool pool, unsigned used, const int* __
     times[pos] = e.t;
 }
 
-static void run_bif(CallCtx& ctx, const chaos_options& o, const BifInputs& in, BifDevice& R)
+void PeakPoolBuf::alloc(hipStream_t st, long long chunks)
+{
+    nchunks = chunks;
+    entries.alloc((size_t)chunks * kChunk);
+    owner.alloc((size_t)chunks);
+    fill.alloc((size_t)chunks);
+    ovf.alloc(1);
+    head.alloc(1);
+    CHAOS_HIP(hipMemsetAsync(head.get(), 0, sizeof(unsigned), st));
+    CHAOS_HIP(hipMemsetAsync(ovf.get(), 0, sizeof(int), st));
+}
+
+PeakPool PeakPoolBuf::view() const
+{
+    PeakPool p;
+    p.entries = entries.get();
+    p.owner = owner.get();
+    p.fill = fill.get();
+    p.head = head.get();
+    p.overflow = ovf.get();
+    p.nchunks = (unsigned)nchunks;
+    return p;
+}
+
+SweepArgs make_bif_args(const chaos_options& o, const BifInputs& in)
 {
     check_system(o.system, in.nx, in.na);
     CHAOS_REQUIRE(in.nPts >= 1, "nPts must be >= 1 (got %d)", in.nPts);
@@ -76,7 +81,6 @@ static void run_bif(CallCtx& ctx, const chaos_options& o, const BifInputs& in, B
     CHAOS_REQUIRE(in.writableVar >= 0 && in.writableVar < in.nx, "writableVar out of range");
     const long long n = in.dims == 2 ? (long long)in.nPts * in.nPts : in.nPts;
     CHAOS_REQUIRE(n <= (1LL << 31) - 64, "grid too large");
-
     SweepArgs A{};
     for (int k = 0; k < in.na; ++k) A.values[k] = in.values[k];
     for (int k = 0; k < in.nx; ++k) A.ic[k] = in.ic[k];
@@ -95,6 +99,75 @@ static void run_bif(CallCtx& ctx, const chaos_options& o, const BifInputs& in, B
     A.L = divergence_fast_bound(A.check, in.maxValue);
     A.hp = in.h * (double)in.preScaller;  // bifurcationCUDA.cu:196,726
     A.sc = make_step_const(in.h);
+    return A;
+}
+
+long long bif_initial_chunks(const SweepArgs& A)
+{
+    const long long n = A.n_points, waves = (n + 63) / 64;
+    size_t free_b = 0, total_b = 0;
+    CHAOS_HIP(hipMemGetInfo(&free_b, &total_b));
+    // First guess: one peak per 32 samples (at most one per two is possible);
+    // an overflow is detected on the device and the sweep rerun with the
+    // exact per-wave need.
+    const double max_pp = std::min((double)A.B / 2.0 + 1.0, std::max(256.0, (double)A.B / 32.0));
+    const long long budget_chunks = (long long)(0.25 * (double)free_b / (kChunk * sizeof(PeakEntry) + 8));
+    const long long want = (long long)std::ceil((double)n * max_pp / (kChunk - 64)) + waves;
+    return std::max<long long>(waves, std::min(want, budget_chunks));
+}
+
+long long bif_exact_chunks(const std::vector<int>& counts, long long n)
+{
+    long long need = 0;
+    for (long long w = 0; w * 64 < n; ++w) {
+        long long s = 0;
+        for (long long p = w * 64; p < std::min(n, w * 64 + 64); ++p) s += counts[p];
+        need += (s + (kChunk - 64) - 1) / (kChunk - 64) + 1;
+    }
+    return need;
+}
+
+bool bif_post(CallCtx& ctx, BifDevice& R, PeakPoolBuf& pool)
+{
+    const long long n = R.n;
+    unsigned used = 0;
+    int overflow = 0;
+    CHAOS_HIP(hipMemcpyAsync(&used, pool.head.get(), sizeof used, hipMemcpyDeviceToHost, ctx.stream()));
+    CHAOS_HIP(hipMemcpyAsync(&overflow, pool.ovf.get(), sizeof overflow, hipMemcpyDeviceToHost, ctx.stream()));
+    R.h_flags.resize(n);
+    R.h_counts.resize(n);
+    R.h_first.resize(n);
+    CHAOS_HIP(hipMemcpyAsync(R.h_flags.data(), R.flags.get(), n * sizeof(int), hipMemcpyDeviceToHost, ctx.stream()));
+    CHAOS_HIP(hipMemcpyAsync(R.h_counts.data(), R.counts.get(), n * sizeof(int), hipMemcpyDeviceToHost, ctx.stream()));
+    CHAOS_HIP(hipMemcpyAsync(R.h_first.data(), R.first.get(), n * sizeof(double), hipMemcpyDeviceToHost, ctx.stream()));
+    ctx.sync();
+    if (overflow) return false;
+    ctx.begin();
+    DevBuf<long long> kept((size_t)n);
+    R.off.alloc(n);
+    hipLaunchKernelGGL(kept_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx.stream(), R.flags.get(),
+                       R.counts.get(), kept.get(), n);
+    CHAOS_HIP(hipGetLastError());
+    R.total = exclusive_scan(ctx.stream(), kept.get(), R.off.get(), n);
+    R.vals.alloc((size_t)std::max(1LL, R.total));
+    R.times.alloc((size_t)std::max(1LL, R.total));
+    if (used > (unsigned)pool.nchunks) used = (unsigned)pool.nchunks;
+    if (used > 0) {
+        hipLaunchKernelGGL(scatter_kernel, dim3(ceil_div((unsigned long long)used * kChunk, 256)), dim3(256), 0,
+                           ctx.stream(), pool.view(), used, R.flags.get(), R.off.get(), R.vals.get(), R.times.get(),
+                           n);
+        CHAOS_HIP(hipGetLastError());
+    }
+    ctx.add_analysis_ms(ctx.end());
+    ctx.add_launch(2);
+    ctx.add_peaks(R.total);
+    return true;
+}
+
+static void run_bif(CallCtx& ctx, const chaos_options& o, const BifInputs& in, BifDevice& R)
+{
+    SweepArgs A = make_bif_args(o, in);
+    const long long n = A.n_points;
     R.n = n;
     R.hp = A.hp;
     R.flags.alloc(n);
@@ -103,76 +176,18 @@ static void run_bif(CallCtx& ctx, const chaos_options& o, const BifInputs& in, B
     A.flags = R.flags.get();
     A.counts = R.counts.get();
     A.first = R.first.get();
-
-    const long long waves = (n + 63) / 64;
-    size_t free_b = 0, total_b = 0;
-    CHAOS_HIP(hipMemGetInfo(&free_b, &total_b));
-    // First guess: one peak per 32 samples (at most one per two is possible);
-    // an overflow is detected on the device and the sweep is rerun with the
-    // exact per-wave need.
-    const double max_pp = std::min((double)A.B / 2.0 + 1.0, std::max(256.0, (double)A.B / 32.0));
-    const long long budget_chunks = (long long)(0.25 * (double)free_b / (kChunk * sizeof(PeakEntry) + 8));
-    long long want = (long long)std::ceil((double)n * max_pp / (kChunk - 64)) + waves;
-    long long nchunks = std::max<long long>(waves, std::min(want, budget_chunks));
-
+    long long nchunks = bif_initial_chunks(A);
     for (int attempt = 0; attempt < 2; ++attempt) {
-        DevBuf<PeakEntry> entries((size_t)nchunks * kChunk);
-        DevBuf<int> owner((size_t)nchunks), fill((size_t)nchunks), ovf(1);
-        DevBuf<unsigned> head(1);
-        CHAOS_HIP(hipMemsetAsync(head.get(), 0, sizeof(unsigned), ctx.stream()));
-        CHAOS_HIP(hipMemsetAsync(ovf.get(), 0, sizeof(int), ctx.stream()));
-        A.pool.entries = entries.get();
-        A.pool.owner = owner.get();
-        A.pool.fill = fill.get();
-        A.pool.head = head.get();
-        A.pool.overflow = ovf.get();
-        A.pool.nchunks = (unsigned)nchunks;
+        PeakPoolBuf pool;
+        pool.alloc(ctx.stream(), nchunks);
+        A.pool = pool.view();
         ctx.begin();
         launch_sweep(ctx, o.system, in.writableVar, KIND_BIF, A);
         ctx.add_kernel_ms(ctx.end());
         ctx.add_steps((double)n * ((double)A.S + (double)A.B * A.pre));
-        unsigned used = 0;
-        int overflow = 0;
-        CHAOS_HIP(hipMemcpyAsync(&used, head.get(), sizeof used, hipMemcpyDeviceToHost, ctx.stream()));
-        CHAOS_HIP(hipMemcpyAsync(&overflow, ovf.get(), sizeof overflow, hipMemcpyDeviceToHost, ctx.stream()));
-        R.h_flags.resize(n);
-        R.h_counts.resize(n);
-        R.h_first.resize(n);
-        CHAOS_HIP(hipMemcpyAsync(R.h_flags.data(), R.flags.get(), n * sizeof(int), hipMemcpyDeviceToHost, ctx.stream()));
-        CHAOS_HIP(hipMemcpyAsync(R.h_counts.data(), R.counts.get(), n * sizeof(int), hipMemcpyDeviceToHost, ctx.stream()));
-        CHAOS_HIP(hipMemcpyAsync(R.h_first.data(), R.first.get(), n * sizeof(double), hipMemcpyDeviceToHost, ctx.stream()));
-        ctx.sync();
-        if (overflow) {
-            // exact need from the per-wave totals (each chunk switch wastes < 64 slots)
-            long long need = 0;
-            for (long long w = 0; w < waves; ++w) {
-                long long s = 0;
-                for (long long p = w * 64; p < std::min(n, w * 64 + 64); ++p) s += R.h_counts[p];
-                need += (s + (kChunk - 64) - 1) / (kChunk - 64) + 1;
-            }
-            CHAOS_REQUIRE(attempt == 0, "peak store overflow after resize");
-            nchunks = need;
-            continue;
-        }
-        ctx.begin();
-        DevBuf<long long> kept((size_t)n);
-        R.off.alloc(n);
-        hipLaunchKernelGGL(kept_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx.stream(), R.flags.get(),
-                           R.counts.get(), kept.get(), n);
-        CHAOS_HIP(hipGetLastError());
-        R.total = exclusive_scan(ctx.stream(), kept.get(), R.off.get(), n);
-        R.vals.alloc((size_t)std::max(1LL, R.total));
-        R.times.alloc((size_t)std::max(1LL, R.total));
-        if (used > (unsigned)nchunks) used = (unsigned)nchunks;
-        if (used > 0) {
-            hipLaunchKernelGGL(scatter_kernel, dim3(ceil_div((unsigned long long)used * kChunk, 256)), dim3(256), 0,
-                               ctx.stream(), A.pool, used, R.flags.get(), R.off.get(), R.vals.get(), R.times.get(), n);
-            CHAOS_HIP(hipGetLastError());
-        }
-        ctx.add_analysis_ms(ctx.end());
-        ctx.add_launch(2);
-        ctx.add_peaks(R.total);
-        return;
+        if (bif_post(ctx, R, pool)) return;
+        CHAOS_REQUIRE(attempt == 0, "peak store overflow after resize");
+        nchunks = bif_exact_chunks(R.h_counts, n);
     }
 }
 
@@ -339,11 +354,11 @@ __global__ __launch_bounds__(64) void dbscan_ref_kernel(const int* __restrict__ 
     if (threadIdx.x == 0) out[p] = cluster - 1;
 }
 
-static void bif2d_analysis(CallCtx& ctx, const chaos_options& o, BifDevice& R, double eps, int* d_out)
+void bif2d_analysis(CallCtx& ctx, int semantics, BifDevice& R, double eps, int* d_out)
 {
     const long long n = R.n;
     ctx.begin();
-    if (o.semantics == CHAOS_SEM_REFERENCE) {
+    if (semantics == CHAOS_SEM_REFERENCE) {
         DevBuf<int> helpful((size_t)std::max(1LL, 2 * R.total));
         hipLaunchKernelGGL(dbscan_ref_kernel, dim3((unsigned)n), dim3(64), 0, ctx.stream(), R.flags.get(),
                            R.counts.get(), R.off.get(), R.times.get(), R.hp, eps, helpful.get(), d_out, n);
@@ -384,6 +399,46 @@ static void bif2d_analysis(CallCtx& ctx, const chaos_options& o, BifDevice& R, d
     }
     ctx.add_analysis_ms(ctx.end());
     ctx.add_launch();
+}
+
+void bif1d_collect(CallCtx& ctx, BifDevice& R, int* counts, double* first, double* peaks, double* intervals,
+                   long long capacity, long long* total)
+{
+    std::vector<double> v((size_t)R.total);
+    std::vector<unsigned> t((size_t)R.total);
+    std::vector<long long> off((size_t)R.n);
+    if (R.total > 0) {
+        CHAOS_HIP(hipMemcpyAsync(v.data(), R.vals.get(), R.total * 8, hipMemcpyDeviceToHost, ctx.stream()));
+        CHAOS_HIP(hipMemcpyAsync(t.data(), R.times.get(), R.total * 4, hipMemcpyDeviceToHost, ctx.stream()));
+    }
+    CHAOS_HIP(hipMemcpyAsync(off.data(), R.off.get(), R.n * 8, hipMemcpyDeviceToHost, ctx.stream()));
+    ctx.sync();
+    // Reference output rules (bifurcationCUDA.cu:359-367,498-510,748-763):
+    // flag -1 -> one line with the first sample; flag 0 -> nothing;
+    // otherwise the peaks after the first, intervals (t[k+1]-t[k])*h*pre.
+    long long k_out = 0;
+    for (long long p = 0; p < R.n; ++p) {
+        first[p] = R.h_first[p];
+        const int fl = R.h_flags[p];
+        if (fl == -1) {
+            counts[p] = -1;
+            continue;
+        }
+        if (fl != 1 || R.h_counts[p] < 2) {
+            counts[p] = 0;
+            continue;
+        }
+        const int P = R.h_counts[p] - 1;
+        counts[p] = P;
+        for (int k = 0; k < P; ++k, ++k_out) {
+            if (k_out < capacity) {
+                const long long e = off[p] + k;
+                peaks[k_out] = v[e + 1];
+                intervals[k_out] = (double)((long long)t[e + 1] - (long long)t[e]) * R.hp;
+            }
+        }
+    }
+    *total = k_out;
 }
 
 static BifInputs bif_inputs(double tMax, int nPts, double h, int nx, const double* ic, const double* ranges,
@@ -429,41 +484,8 @@ extern "C" int chaos_bif1d(const chaos_options* opt, double tMax, int nPts, doub
         CallCtx ctx(o);
         BifDevice R;
         run_bif(ctx, o, in, R);
-        std::vector<double> v((size_t)R.total);
-        std::vector<unsigned> t((size_t)R.total);
-        std::vector<long long> off((size_t)R.n);
-        if (R.total > 0) {
-            CHAOS_HIP(hipMemcpyAsync(v.data(), R.vals.get(), R.total * 8, hipMemcpyDeviceToHost, ctx.stream()));
-            CHAOS_HIP(hipMemcpyAsync(t.data(), R.times.get(), R.total * 4, hipMemcpyDeviceToHost, ctx.stream()));
-        }
-        CHAOS_HIP(hipMemcpyAsync(off.data(), R.off.get(), R.n * 8, hipMemcpyDeviceToHost, ctx.stream()));
-        ctx.sync();
-        // Reference output rules (bifurcationCUDA.cu:359-367,498-510,748-763):
-        // flag -1 -> one line with the first sample; flag 0 -> nothing;
-        // otherwise the peaks after the first, intervals (t[k+1]-t[k])*h*pre.
-        long long k_out = 0;
-        for (long long p = 0; p < R.n; ++p) {
-            first[p] = R.h_first[p];
-            const int fl = R.h_flags[p];
-            if (fl == -1) {
-                counts[p] = -1;
-                continue;
-            }
-            if (fl != 1 || R.h_counts[p] < 2) {
-                counts[p] = 0;
-                continue;
-            }
-            const int P = R.h_counts[p] - 1;
-            counts[p] = P;
-            for (int k = 0; k < P; ++k, ++k_out) {
-                if (k_out < capacity) {
-                    const long long e = off[p] + k;
-                    peaks[k_out] = v[e + 1];
-                    intervals[k_out] = (double)((long long)t[e + 1] - (long long)t[e]) * R.hp;
-                }
-            }
-        }
-        *total = k_out;
+        bif1d_collect(ctx, R, counts, first, peaks, intervals, capacity, total);
+        const long long k_out = *total;
         ctx.finish();
         if (k_out > capacity) fail(CHAOS_E_CAPACITY, "need capacity %lld, got %lld", k_out, capacity);
     });
@@ -538,7 +560,7 @@ extern "C" int chaos_bif2d(const chaos_options* opt, double tMax, int nPts, doub
             tmp.alloc((size_t)R.n);
             d_out = tmp.get();
         }
-        bif2d_analysis(ctx, o, R, eps, d_out);
+        bif2d_analysis(ctx, o.semantics, R, eps, d_out);
         if (!o.device_buffers)
             CHAOS_HIP(hipMemcpyAsync(out, d_out, R.n * sizeof(int), hipMemcpyDeviceToHost, ctx.stream()));
         ctx.sync();

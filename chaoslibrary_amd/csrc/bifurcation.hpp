@@ -1,0 +1,59 @@
+// bifurcation.hpp -- Bifurcation stages shared by the one-shot entry points
+// and the batched engine.
+#pragma once
+
+#include <vector>
+
+#include "peaks.hpp"
+#include "runtime.hpp"
+#include "sweep.hpp"
+
+namespace chaos {
+
+struct BifInputs {
+    double tMax, h, maxValue, transientTime, eps;
+    int nPts, nx, na, writableVar, preScaller, dims;
+    const double *ic, *ranges, *values;
+    const int* idx;
+};
+
+// Device results of one Bifurcation sweep (segments of flag-1 points).
+struct BifDevice {
+    long long n = 0;         // points
+    long long total = 0;     // stored peaks of flag-1 points (first peak included)
+    double hp = 0;           // interval unit
+    DevBuf<int> flags, counts;
+    DevBuf<double> first;
+    DevBuf<long long> off;   // segment offsets
+    DevBuf<double> vals;
+    DevBuf<unsigned> times;
+    std::vector<int> h_flags, h_counts;
+    std::vector<double> h_first;
+};
+
+// Chunk pool backing a PeakPool view.
+struct PeakPoolBuf {
+    DevBuf<PeakEntry> entries;
+    DevBuf<int> owner, fill, ovf;
+    DevBuf<unsigned> head;
+    long long nchunks = 0;
+    void alloc(hipStream_t st, long long chunks);
+    PeakPool view() const;
+};
+
+// Validated sweep arguments (outputs and pool left unset).
+SweepArgs make_bif_args(const chaos_options& o, const BifInputs& in);
+// Initial pool size for n points (overflow is detected and retried).
+long long bif_initial_chunks(const SweepArgs& A);
+// Exact pool need from per-point counts (each chunk switch wastes < 64 slots).
+long long bif_exact_chunks(const std::vector<int>& counts, long long n);
+// Reads flags/counts/first back, builds segment offsets and scatters the
+// pooled peaks into per-point segments.  Returns false on pool overflow.
+bool bif_post(CallCtx& ctx, BifDevice& R, PeakPoolBuf& pool);
+// Per-point 2-D analysis (REFERENCE dbscan / CONFIG 1-D DBSCAN count).
+void bif2d_analysis(CallCtx& ctx, int semantics, BifDevice& R, double eps, int* d_out);
+// Host assembly of the 1-D outputs (first peak dropped, intervals).
+void bif1d_collect(CallCtx& ctx, BifDevice& R, int* counts, double* first, double* peaks, double* intervals,
+                   long long capacity, long long* total);
+
+}  // namespace chaos

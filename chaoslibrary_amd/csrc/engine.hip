@@ -1,0 +1,683 @@
+// engine.hip -- batched execution of many sweeps in one persistent kernel.
+//
+// Work unit: a tile = (job slot, 64-point group, tick range).  Each launch
+// carries one tile per 64-point group of every active job; waves of the
+// persistent kernel pull tiles from a device queue (one atomic per tile),
+// resume the group's lane state from HBM, advance it, and park it again (or
+// write the job's outputs when its last tick is done).  The per-lane
+// programs are the ones the one-shot kernels run (lanes.hpp), so a job's
+// results do not depend on how it was batched.  The host side admits jobs
+// FIFO while job slots and the HBM budget allow, replays the launch sequence
+// from a captured hipGraph, and finishes completed jobs with the same
+// post-processing stages as the one-shot entry points.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <deque>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "basins.hpp"
+#include "bifurcation.hpp"
+#include "dispatch.hpp"
+#include "engine.hpp"
+#include "lanes.hpp"
+#include "lle.hpp"
+
+namespace chaos {
+
+// ---------------------------------------------------------------------------
+// device side
+// ---------------------------------------------------------------------------
+
+template <int SYS>
+__device__ __forceinline__ void lle_tile(const JobDev& J, const Tile& T, int lane)
+{
+    const LleArgs p = J.lle;
+    const long long q = (long long)T.group * 64 + lane;
+    const bool valid = q < J.n_points;
+    const long long qq = valid ? q : 0;
+    const int i = (int)(qq % p.nPts), j = (int)(qq / p.nPts);
+    LleProg<SYS> L;
+    L.params(p, i, j);
+    if (T.g0 == 0) L.init(p);
+    else L.load(J.st, q);
+    L.ticks(p, (long long)j * p.nPts + i, T.g0, T.g1);
+    if (T.g1 == J.total) {
+        if (valid) J.lle_out[q] = L.result(p);
+    } else {
+        L.store(J.st, q);
+    }
+}
+
+template <int SYS, int KIND>
+__device__ __forceinline__ void sweep_tile(const JobDev& J, const Tile& T, int lane)
+{
+    const SweepArgs A = J.sw;
+    const long long q = (long long)T.group * 64 + lane;
+    const bool in_range = q < A.n_points;
+    const long long qq = in_range ? q : 0;
+    SweepProg<SYS, KIND> L;
+    WaveCursor cur{A.pool.nchunks, kChunk};
+    if (T.g0 == 0) {
+        L.init(A, qq, in_range);
+    } else {
+        L.params(A, qq);
+        L.load(J.st, q);
+        if (KIND == KIND_BIF) cur = J.cursors[T.group];
+    }
+    L.template ticks<-1>(A, J.wv, cur, T.group, T.g0, T.g1);
+    if (T.g1 == J.total) {
+        if (KIND == KIND_BIF) close_chunk(A.pool, cur);
+        if (in_range) L.finish(A, q);
+    } else {
+        L.store(J.st, q);
+        if (KIND == KIND_BIF && lane == 0) J.cursors[T.group] = cur;
+    }
+}
+
+__global__ void reset_head_kernel(unsigned* head)
+{
+    if (threadIdx.x == 0) *head = 0u;
+}
+
+// Persistent workers: every wave pulls tiles until the queue is empty, so
+// the grid drains as soon as the last tile is taken.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
+void engine_kernel(const JobDev* __restrict__ jobs, const Tile* __restrict__ tiles, const int* __restrict__ ntiles,
+                   unsigned* __restrict__ head)
+{
+    const int lane = threadIdx.x & 63;
+    const int n = *ntiles;
+    for (;;) {
+        unsigned t = 0;
+        if (lane == 0) t = atomicAdd(head, 1u);
+        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
+        if ((int)t >= n) break;
+        const Tile T = tiles[t];
+        const JobDev& J = jobs[T.slot];
+        switch (J.prog * 2 + J.lorenz) {
+        case 0: lle_tile<SYS_ROSSLER_RK4>(J, T, lane); break;
+        case 1: lle_tile<SYS_LORENZ_RK4>(J, T, lane); break;
+        case 2: sweep_tile<SYS_ROSSLER_RK4, KIND_BIF>(J, T, lane); break;
+        case 3: sweep_tile<SYS_LORENZ_RK4, KIND_BIF>(J, T, lane); break;
+        case 4: sweep_tile<SYS_ROSSLER_RK4, KIND_BASINS_REF>(J, T, lane); break;
+        case 5: sweep_tile<SYS_LORENZ_RK4, KIND_BASINS_REF>(J, T, lane); break;
+        case 6: sweep_tile<SYS_ROSSLER_RK4, KIND_BASINS_CFG>(J, T, lane); break;
+        case 7: sweep_tile<SYS_LORENZ_RK4, KIND_BASINS_CFG>(J, T, lane); break;
+        default: break;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+static double steady_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+static chaos_options job_options(const chaos_job& j, int device, hipStream_t st)
+{
+    chaos_options o = resolve_options(nullptr);
+    o.system = j.system;
+    o.semantics = j.semantics;
+    o.seed = j.seed;
+    o.min_pts = j.min_pts > 0 ? j.min_pts : 20;
+    o.device = device;
+    o.stream = st;
+    o.device_buffers = 0;
+    o.block = 0;
+    o.stats = nullptr;
+    return o;
+}
+
+Engine::Engine(int device, const chaos_engine_params& prm) : prm_(prm), device_(device)
+{
+    CHAOS_REQUIRE(prm_.block == 64 || prm_.block == 128 || prm_.block == 256, "block must be 64, 128 or 256");
+    CHAOS_REQUIRE(prm_.waves_per_simd >= 1 && prm_.waves_per_simd <= 8, "waves_per_simd must be in [1,8]");
+    CHAOS_REQUIRE(prm_.steps_per_launch >= 2, "steps_per_launch must be >= 2");
+    CHAOS_REQUIRE(prm_.max_jobs >= 1 && prm_.max_tiles >= 1, "max_jobs and max_tiles must be >= 1");
+    chaos_options o = resolve_options(nullptr);
+    o.device = device;
+    ctx_.reset(new CallCtx(o));
+    CHAOS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));  // default priority
+    o.stream = stream_;
+    ctx_.reset(new CallCtx(o));
+    CHAOS_HIP(hipEventCreate(&ev0_));
+    CHAOS_HIP(hipEventCreate(&ev1_));
+    d_jobs_.alloc(prm_.max_jobs);
+    d_tiles_.alloc(prm_.max_tiles);
+    d_ntiles_.alloc(1);
+    d_head_.alloc(1);
+    CHAOS_HIP(hipHostMalloc((void**)&h_jobs_, sizeof(JobDev) * prm_.max_jobs, hipHostMallocDefault));
+    CHAOS_HIP(hipHostMalloc((void**)&h_tiles_, sizeof(Tile) * prm_.max_tiles, hipHostMallocDefault));
+    CHAOS_HIP(hipHostMalloc((void**)&h_ntiles_, sizeof(int), hipHostMallocDefault));
+    for (int s = prm_.max_jobs - 1; s >= 0; --s) free_slots_.push_back(s);
+    const hipDeviceProp_t& props = device_props(device);
+    const int waves = props.multiProcessorCount * 4 * prm_.waves_per_simd;
+    grid_ = std::max(1, waves / (prm_.block / 64));
+    size_t free_b = 0, total_b = 0;
+    CHAOS_HIP(hipMemGetInfo(&free_b, &total_b));
+    hbm_budget_ = (long long)(prm_.hbm_fraction * (double)free_b);
+    t0_ = steady_ms();
+}
+
+Engine::~Engine()
+{
+    jobs_.clear();
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    if (h_jobs_) (void)hipHostFree(h_jobs_);
+    if (h_tiles_) (void)hipHostFree(h_tiles_);
+    if (h_ntiles_) (void)hipHostFree(h_ntiles_);
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    d_jobs_.release();
+    d_tiles_.release();
+    d_ntiles_.release();
+    d_head_.release();
+    ctx_.reset();
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+double Engine::now_ms() const { return steady_ms() - t0_; }
+
+// Validate and size a job (no device work).
+void Engine::plan(JobRec& r)
+{
+    const chaos_job& j = r.spec;
+    CHAOS_REQUIRE(j.system == CHAOS_SYS_ROSSLER_RK4 || j.system == CHAOS_SYS_LORENZ_RK4,
+                  "the engine runs the RK4 systems only (got system %d); use the one-shot entry points", j.system);
+    const chaos_options o = job_options(j, device_, stream_);
+    int nx = 0, na = 0;
+    chaos_system_dims(j.system, &nx, &na);
+    r.lorenz = j.system == CHAOS_SYS_LORENZ_RK4 ? 1 : 0;
+    const long long K = prm_.steps_per_launch;
+    if (j.kind == CHAOS_JOB_LLE2D) {
+        r.lle = make_lle_args(o, j.tMax, j.NT, j.nPts, j.h, j.eps, j.initialConditions, j.amountOfInitialConditions,
+                              j.ranges, j.indicesOfMutVars, j.transientTime, j.values, j.amountOfValues);
+        r.lle.row0 = 0;
+        r.n = (long long)j.nPts * j.nPts;
+        r.lle.n_points = r.n;
+        r.total = (long long)r.lle.S + (long long)r.lle.W * r.lle.NTp;
+        r.ticks_per_launch = std::max<long long>(1, K / 2);
+        r.rk4_steps = (double)r.n * ((double)r.lle.S + 2.0 * r.lle.W * r.lle.NTp);
+        r.prog = PROG_LLE;
+    } else if (j.kind == CHAOS_JOB_BIF1D || j.kind == CHAOS_JOB_BIF2D) {
+        BifInputs in{};
+        in.tMax = j.tMax;
+        in.nPts = j.nPts;
+        in.h = j.h;
+        in.nx = j.amountOfInitialConditions;
+        in.ic = j.initialConditions;
+        in.ranges = j.ranges;
+        in.idx = j.indicesOfMutVars;
+        in.writableVar = j.writableVar;
+        in.maxValue = j.maxValue;
+        in.transientTime = j.transientTime;
+        in.values = j.values;
+        in.na = j.amountOfValues;
+        in.preScaller = j.preScaller;
+        in.eps = j.eps;
+        in.dims = j.kind == CHAOS_JOB_BIF2D ? 2 : 1;
+        r.sw = make_bif_args(o, in);
+        r.n = r.sw.n_points;
+        r.total = (long long)r.sw.S + r.sw.B;
+        r.ticks_per_launch = K;
+        r.rk4_steps = (double)r.n * ((double)r.sw.S + (double)r.sw.B);
+        r.prog = PROG_BIF;
+        r.pool_chunks = bif_initial_chunks(r.sw);
+    } else if (j.kind == CHAOS_JOB_BASINS2D) {
+        r.sw = make_basins_args(o, j.tMax, j.nPts, j.h, j.amountOfInitialConditions, j.initialConditions, j.ranges,
+                                j.indicesOfMutVars, j.writableVar, j.maxValue, j.transientTime, j.values,
+                                j.amountOfValues, j.preScaller, j.eps);
+        r.n = r.sw.n_points;
+        r.total = (long long)r.sw.S + r.sw.B;
+        r.ticks_per_launch = std::max<long long>(1, K / std::max(1, r.sw.pre));
+        r.rk4_steps = (double)r.n * ((double)r.sw.S + (double)r.sw.B * r.sw.pre);
+        r.prog = j.semantics == CHAOS_SEM_REFERENCE ? PROG_BASINS_REF : PROG_BASINS_CFG;
+    } else {
+        fail(CHAOS_E_INVALID, "unknown job kind %d", j.kind);
+    }
+    r.groups = (r.n + 63) / 64;
+    r.np = r.groups * 64;
+    CHAOS_REQUIRE(r.groups <= prm_.max_tiles, "job needs %lld tiles, queue holds %d", r.groups, prm_.max_tiles);
+    // HBM footprint: lane state + outputs + per-kind working sets
+    long long fp = 0;
+    if (r.prog == PROG_LLE) {
+        fp = r.np * (2 * nx + 1) * 8 + r.n * 8;
+    } else {
+        fp = r.np * ((nx + 8) * 8 + 4 * 4);
+        if (r.prog == PROG_BIF)
+            fp += r.groups * (long long)sizeof(WaveCursor) + r.n * 16 + r.pool_chunks * (kChunk * 16 + 8) +
+                  r.n * 8 + r.pool_chunks * kChunk * 12;
+        else
+            fp += r.n * (3 * 8 + 4 + 4) + r.n * 200;  // features, flags, labels, clustering scratch
+    }
+    r.footprint = fp;
+}
+
+long long Engine::submit(const chaos_job& j)
+{
+    std::unique_ptr<JobRec> r(new JobRec());
+    r->spec = j;
+    plan(*r);
+    CHAOS_REQUIRE(r->footprint <= hbm_budget_, "job footprint %lld bytes exceeds the HBM budget %lld", r->footprint,
+                  hbm_budget_);
+    r->id = next_id_++;
+    r->t_submit = now_ms();
+    const long long id = r->id;
+    pending_.push_back(id);
+    jobs_[id] = std::move(r);
+    return id;
+}
+
+void Engine::allocate(JobRec& r)
+{
+    const int nx = r.lorenz ? System<SYS_LORENZ_RK4>::NX : System<SYS_ROSSLER_RK4>::NX;
+    JobDev& D = h_jobs_[r.slot];
+    D = JobDev{};
+    D.prog = r.prog;
+    D.lorenz = r.lorenz;
+    D.wv = r.spec.writableVar;
+    D.np = r.np;
+    D.n_points = r.n;
+    D.total = r.total;
+    if (r.prog == PROG_LLE) {
+        r.dstate.alloc((size_t)r.np * (2 * nx + 1));
+        r.out.alloc((size_t)r.n);
+        D.st = LaneStore{r.dstate.get(), nullptr, r.np};
+        D.lle = r.lle;
+        D.lle_out = r.out.get();
+    } else {
+        r.dstate.alloc((size_t)r.np * (nx + 8));
+        r.istate.alloc((size_t)r.np * 4);
+        D.st = LaneStore{r.dstate.get(), r.istate.get(), r.np};
+        D.sw = r.sw;
+        if (r.prog == PROG_BIF) {
+            r.bif.n = r.n;
+            r.bif.hp = r.sw.hp;
+            r.bif.flags.alloc(r.n);
+            r.bif.counts.alloc(r.n);
+            r.bif.first.alloc(r.n);
+            r.pool.alloc(stream_, r.pool_chunks);
+            r.cursors.alloc((size_t)r.groups);
+            D.sw.flags = r.bif.flags.get();
+            D.sw.counts = r.bif.counts.get();
+            D.sw.first = r.bif.first.get();
+            D.sw.pool = r.pool.view();
+            D.cursors = r.cursors.get();
+        } else {
+            r.flags.alloc(r.n);
+            r.feats.alloc(3 * (size_t)r.n);
+            D.sw.flags = r.flags.get();
+            D.sw.f0 = r.feats.get();
+            D.sw.f1 = r.feats.get() + r.n;
+            D.sw.f2 = r.feats.get() + 2 * r.n;
+            D.sw.counts = nullptr;
+        }
+    }
+    CHAOS_HIP(hipMemcpyAsync(d_jobs_.get() + r.slot, &D, sizeof(JobDev), hipMemcpyHostToDevice, stream_));
+}
+
+void Engine::admit()
+{
+    while (!pending_.empty() && !free_slots_.empty()) {
+        JobRec& r = *jobs_[pending_.front()];
+        if (hbm_used_ + r.footprint > hbm_budget_) break;
+        if (active_groups_ + r.groups > prm_.max_tiles) break;
+        r.slot = free_slots_.back();
+        free_slots_.pop_back();
+        hbm_used_ += r.footprint;
+        active_groups_ += r.groups;
+        r.status = 1;
+        r.t_admit = now_ms();
+        allocate(r);
+        active_.push_back(r.id);
+        pending_.pop_front();
+    }
+}
+
+void Engine::capture()
+{
+    CHAOS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    enqueue_launch();
+    CHAOS_HIP(hipStreamEndCapture(stream_, &graph_));
+    CHAOS_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+}
+
+// The per-round launch sequence: tile list and count to the device, queue
+// head reset by a kernel (no memset node), persistent kernel.
+void Engine::enqueue_launch()
+{
+    CHAOS_HIP(hipMemcpyAsync(d_tiles_.get(), h_tiles_, sizeof(Tile) * (size_t)prm_.max_tiles, hipMemcpyHostToDevice,
+                             stream_));
+    CHAOS_HIP(hipMemcpyAsync(d_ntiles_.get(), h_ntiles_, sizeof(int), hipMemcpyHostToDevice, stream_));
+    hipLaunchKernelGGL(reset_head_kernel, dim3(1), dim3(64), 0, stream_, d_head_.get());
+    hipLaunchKernelGGL(engine_kernel, dim3(grid_), dim3(prm_.block), 0, stream_, d_jobs_.get(), d_tiles_.get(),
+                       d_ntiles_.get(), d_head_.get());
+    CHAOS_HIP(hipGetLastError());
+}
+
+void Engine::finalize(JobRec& r)
+{
+    const double t0 = steady_ms();
+    const chaos_job& j = r.spec;
+    if (r.prog == PROG_LLE) {
+        r.r_lle.resize(r.n);
+        CHAOS_HIP(hipMemcpyAsync(r.r_lle.data(), r.out.get(), r.n * 8, hipMemcpyDeviceToHost, stream_));
+        CHAOS_HIP(hipStreamSynchronize(stream_));
+    } else if (r.prog == PROG_BIF) {
+        bool ok = bif_post(*ctx_, r.bif, r.pool);
+        if (!ok) {
+            // peak store overflow: rerun this job alone with the exact pool
+            r.reran = 1;
+            chaos_options o = job_options(j, device_, stream_);
+            if (j.kind == CHAOS_JOB_BIF1D) {
+                r.r_counts.resize(r.n);
+                r.r_first.resize(r.n);
+                long long total = 0;
+                int rc = chaos_bif1d(&o, j.tMax, j.nPts, j.h, j.amountOfInitialConditions, j.initialConditions,
+                                     j.ranges, j.indicesOfMutVars, j.writableVar, j.maxValue, j.transientTime,
+                                     j.values, j.amountOfValues, j.preScaller, r.r_counts.data(), r.r_first.data(),
+                                     nullptr, nullptr, 0, &total);
+                if (rc != CHAOS_OK && rc != CHAOS_E_CAPACITY) throw Error(rc, chaos_last_error());
+                r.r_peaks.resize(std::max(1LL, total));
+                r.r_intervals.resize(std::max(1LL, total));
+                rc = chaos_bif1d(&o, j.tMax, j.nPts, j.h, j.amountOfInitialConditions, j.initialConditions, j.ranges,
+                                 j.indicesOfMutVars, j.writableVar, j.maxValue, j.transientTime, j.values,
+                                 j.amountOfValues, j.preScaller, r.r_counts.data(), r.r_first.data(), r.r_peaks.data(),
+                                 r.r_intervals.data(), total, &total);
+                if (rc != CHAOS_OK) throw Error(rc, chaos_last_error());
+                r.r_peaks.resize(total);
+                r.r_intervals.resize(total);
+            } else {
+                r.r_map.resize(r.n);
+                int rc = chaos_bif2d(&o, j.tMax, j.nPts, j.h, j.amountOfInitialConditions, j.initialConditions,
+                                     j.ranges, j.indicesOfMutVars, j.writableVar, j.maxValue, j.transientTime,
+                                     j.values, j.amountOfValues, j.preScaller, j.eps, r.r_map.data());
+                if (rc != CHAOS_OK) throw Error(rc, chaos_last_error());
+            }
+        } else if (j.kind == CHAOS_JOB_BIF1D) {
+            r.r_counts.resize(r.n);
+            r.r_first.resize(r.n);
+            r.r_peaks.resize(std::max(1LL, r.bif.total));
+            r.r_intervals.resize(std::max(1LL, r.bif.total));
+            long long total = 0;
+            bif1d_collect(*ctx_, r.bif, r.r_counts.data(), r.r_first.data(), r.r_peaks.data(), r.r_intervals.data(),
+                          r.bif.total, &total);
+            r.r_peaks.resize(total);
+            r.r_intervals.resize(total);
+        } else {
+            DevBuf<int> map((size_t)r.n);
+            bif2d_analysis(*ctx_, j.semantics, r.bif, j.eps, map.get());
+            r.r_map.resize(r.n);
+            CHAOS_HIP(hipMemcpyAsync(r.r_map.data(), map.get(), r.n * 4, hipMemcpyDeviceToHost, stream_));
+            CHAOS_HIP(hipStreamSynchronize(stream_));
+        }
+    } else {
+        const bool ref = r.prog == PROG_BASINS_REF;
+        DevBuf<int> labels((size_t)r.n);
+        basins_cluster(*ctx_, ref, r.n, r.feats.get(), r.feats.get() + r.n, r.feats.get() + 2 * r.n, r.flags.get(),
+                       j.eps, j.min_pts > 0 ? j.min_pts : 20, labels.get());
+        const int nf = ref ? 2 : 3;
+        r.r_labels.resize(r.n);
+        r.r_flags.resize(r.n);
+        r.r_feats.resize(nf * r.n);
+        CHAOS_HIP(hipMemcpyAsync(r.r_labels.data(), labels.get(), r.n * 4, hipMemcpyDeviceToHost, stream_));
+        CHAOS_HIP(hipMemcpyAsync(r.r_flags.data(), r.flags.get(), r.n * 4, hipMemcpyDeviceToHost, stream_));
+        CHAOS_HIP(hipMemcpyAsync(r.r_feats.data(), r.feats.get(), nf * r.n * 8, hipMemcpyDeviceToHost, stream_));
+        CHAOS_HIP(hipStreamSynchronize(stream_));
+    }
+    // release the device working set
+    r.dstate.release();
+    r.istate.release();
+    r.out.release();
+    r.cursors.release();
+    r.flags.release();
+    r.feats.release();
+    r.bif = BifDevice();
+    r.pool = PeakPoolBuf();
+    finalize_ms_ += steady_ms() - t0;
+}
+
+int Engine::step()
+{
+    admit();
+    int nt = 0;
+    for (long long id : active_) {
+        JobRec& r = *jobs_[id];
+        const long long g0 = r.done, g1 = std::min(r.total, r.done + r.ticks_per_launch);
+        for (long long g = 0; g < r.groups; ++g) h_tiles_[nt++] = Tile{r.slot, (int)g, g0, g1};
+    }
+    if (nt == 0) return 0;
+    *h_ntiles_ = nt;
+    CHAOS_HIP(hipEventRecord(ev0_, stream_));
+    if (prm_.use_graph) {
+        if (!exec_) capture();
+        CHAOS_HIP(hipGraphLaunch(exec_, stream_));
+        ++graph_replays_;
+    } else {
+        enqueue_launch();
+    }
+    CHAOS_HIP(hipEventRecord(ev1_, stream_));
+    CHAOS_HIP(hipEventSynchronize(ev1_));
+    float ms = 0;
+    CHAOS_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    busy_ms_ += ms;
+    ++launches_;
+    tiles_ += nt;
+    int completed = 0;
+    std::vector<long long> still;
+    still.reserve(active_.size());
+    for (long long id : active_) {
+        JobRec& r = *jobs_[id];
+        r.done = std::min(r.total, r.done + r.ticks_per_launch);
+        if (r.done < r.total) {
+            still.push_back(id);
+            continue;
+        }
+        finalize(r);
+        r.status = 2;
+        r.t_done = now_ms();
+        steps_done_ += r.rk4_steps;
+        hbm_used_ -= r.footprint;
+        active_groups_ -= r.groups;
+        free_slots_.push_back(r.slot);
+        r.slot = -1;
+        ++completed;
+    }
+    active_.swap(still);
+    return completed;
+}
+
+void Engine::drain()
+{
+    while (!pending_.empty() || !active_.empty()) {
+        const size_t before = pending_.size() + active_.size();
+        step();
+        if (active_.empty() && !pending_.empty() && pending_.size() + active_.size() == before)
+            fail(CHAOS_E_BUSY, "pending job cannot be admitted");
+    }
+}
+
+JobRec& Engine::job(long long id)
+{
+    auto it = jobs_.find(id);
+    if (it == jobs_.end()) fail(CHAOS_E_NOTFOUND, "unknown job id %lld", id);
+    return *it->second;
+}
+
+void Engine::release(long long id)
+{
+    JobRec& r = job(id);
+    CHAOS_REQUIRE(r.status == 2 || r.status == -1, "job %lld is still running", id);
+    jobs_.erase(id);
+}
+
+chaos_server_stats Engine::stats() const
+{
+    chaos_server_stats s{};
+    s.launches = launches_;
+    s.graph_replays = graph_replays_;
+    s.tiles = tiles_;
+    s.busy_ms = busy_ms_;
+    s.finalize_ms = finalize_ms_;
+    s.rk4_steps = steps_done_;
+    s.active_jobs = (int)active_.size();
+    s.pending_jobs = (int)pending_.size();
+    s.hbm_in_use = hbm_used_;
+    s.hbm_budget = hbm_budget_;
+    return s;
+}
+
+}  // namespace chaos
+
+using namespace chaos;
+
+extern "C" int chaos_engine_params_default(chaos_engine_params* p)
+{
+    if (!p) return CHAOS_E_INVALID;
+    p->block = 256;
+    p->waves_per_simd = 8;
+    p->steps_per_launch = 20000;
+    p->max_jobs = 256;
+    p->max_tiles = 65536;
+    p->use_graph = 1;
+    p->hbm_fraction = 0.6;
+    return CHAOS_OK;
+}
+
+extern "C" int chaos_server_create(int device, const chaos_engine_params* p, void** server)
+{
+    return guarded("chaos_server_create", [&] {
+        CHAOS_REQUIRE(server, "server is NULL");
+        chaos_engine_params prm;
+        chaos_engine_params_default(&prm);
+        if (p) prm = *p;
+        *server = new Engine(device, prm);
+    });
+}
+
+extern "C" int chaos_server_destroy(void* server)
+{
+    return guarded("chaos_server_destroy", [&] { delete static_cast<Engine*>(server); });
+}
+
+extern "C" int chaos_server_submit(void* server, const chaos_job* job, long long* job_id)
+{
+    return guarded("chaos_server_submit", [&] {
+        CHAOS_REQUIRE(server && job && job_id, "NULL argument");
+        *job_id = static_cast<Engine*>(server)->submit(*job);
+    });
+}
+
+extern "C" int chaos_server_step(void* server, int* completed)
+{
+    return guarded("chaos_server_step", [&] {
+        CHAOS_REQUIRE(server, "server is NULL");
+        const int c = static_cast<Engine*>(server)->step();
+        if (completed) *completed = c;
+    });
+}
+
+extern "C" int chaos_server_drain(void* server)
+{
+    return guarded("chaos_server_drain", [&] {
+        CHAOS_REQUIRE(server, "server is NULL");
+        static_cast<Engine*>(server)->drain();
+    });
+}
+
+extern "C" int chaos_server_job_info(void* server, long long job_id, chaos_job_info* info)
+{
+    return guarded("chaos_server_job_info", [&] {
+        CHAOS_REQUIRE(server && info, "NULL argument");
+        const JobRec& r = static_cast<Engine*>(server)->job(job_id);
+        *info = chaos_job_info{};
+        info->status = r.status;
+        info->kind = r.spec.kind;
+        info->t_submit_ms = r.t_submit;
+        info->t_admit_ms = r.t_admit;
+        info->t_done_ms = r.t_done;
+        info->rk4_steps = r.rk4_steps;
+        info->footprint_bytes = r.footprint;
+        info->n_points = r.n;
+        info->result_len = r.spec.kind == CHAOS_JOB_BIF1D ? (long long)r.r_peaks.size() : r.n;
+        info->reran = r.reran;
+    });
+}
+
+extern "C" int chaos_server_stats_get(void* server, chaos_server_stats* stats)
+{
+    return guarded("chaos_server_stats_get", [&] {
+        CHAOS_REQUIRE(server && stats, "NULL argument");
+        *stats = static_cast<Engine*>(server)->stats();
+    });
+}
+
+static const JobRec& done_job(void* server, long long id, int kind)
+{
+    CHAOS_REQUIRE(server, "server is NULL");
+    const JobRec& r = static_cast<Engine*>(server)->job(id);
+    CHAOS_REQUIRE(r.status == 2, "job %lld is not finished", id);
+    CHAOS_REQUIRE(r.spec.kind == kind, "job %lld is of kind %d", id, r.spec.kind);
+    return r;
+}
+
+extern "C" int chaos_server_result_lle(void* server, long long job_id, double* out)
+{
+    return guarded("chaos_server_result_lle", [&] {
+        const JobRec& r = done_job(server, job_id, CHAOS_JOB_LLE2D);
+        CHAOS_REQUIRE(out, "out is NULL");
+        std::copy(r.r_lle.begin(), r.r_lle.end(), out);
+    });
+}
+
+extern "C" int chaos_server_result_bif2d(void* server, long long job_id, int* out)
+{
+    return guarded("chaos_server_result_bif2d", [&] {
+        const JobRec& r = done_job(server, job_id, CHAOS_JOB_BIF2D);
+        CHAOS_REQUIRE(out, "out is NULL");
+        std::copy(r.r_map.begin(), r.r_map.end(), out);
+    });
+}
+
+extern "C" int chaos_server_result_bif1d(void* server, long long job_id, int* counts, double* first, double* peaks,
+                                         double* intervals, long long capacity, long long* total)
+{
+    return guarded("chaos_server_result_bif1d", [&] {
+        const JobRec& r = done_job(server, job_id, CHAOS_JOB_BIF1D);
+        CHAOS_REQUIRE(counts && first && total, "NULL output");
+        std::copy(r.r_counts.begin(), r.r_counts.end(), counts);
+        std::copy(r.r_first.begin(), r.r_first.end(), first);
+        *total = (long long)r.r_peaks.size();
+        if (*total > capacity) fail(CHAOS_E_CAPACITY, "need capacity %lld", *total);
+        std::copy(r.r_peaks.begin(), r.r_peaks.end(), peaks);
+        std::copy(r.r_intervals.begin(), r.r_intervals.end(), intervals);
+    });
+}
+
+extern "C" int chaos_server_result_basins(void* server, long long job_id, int* labels, double* features, int* flags)
+{
+    return guarded("chaos_server_result_basins", [&] {
+        const JobRec& r = done_job(server, job_id, CHAOS_JOB_BASINS2D);
+        CHAOS_REQUIRE(labels, "labels is NULL");
+        std::copy(r.r_labels.begin(), r.r_labels.end(), labels);
+        if (features) std::copy(r.r_feats.begin(), r.r_feats.end(), features);
+        if (flags) std::copy(r.r_flags.begin(), r.r_flags.end(), flags);
+    });
+}
+
+extern "C" int chaos_server_release(void* server, long long job_id)
+{
+    return guarded("chaos_server_release", [&] {
+        CHAOS_REQUIRE(server, "server is NULL");
+        static_cast<Engine*>(server)->release(job_id);
+    });
+}

@@ -1,0 +1,413 @@
+// lanes.hpp -- resumable per-lane programs shared by the one-shot kernels and
+// the batched engine.
+//
+// Every program advances one grid point over a range of "ticks" [g0, g1) of
+// its job and can stop anywhere: the one-shot kernels run [0, total) in one
+// launch, the engine runs a few thousand ticks per launch and parks the lane
+// state in HBM in between.  Both paths execute the same arithmetic in the
+// same order, so a point's result does not depend on how its work was split.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lle.hpp"
+#include "peaks.hpp"
+#include "sweep.hpp"
+#include "systems.hpp"
+
+namespace chaos {
+
+// ---------------------------------------------------------------------------
+// shared checks (Bifurcation/src/bifurcationCUDA.cu:445-466,
+// Basins/src/basinsCUDA.cu:350-376)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ bool diverged_exact(double v, double M, int check)
+{
+    if (check == CHK_BIF_REF) {
+        if (v != v || v == v + 1.0) return true;
+        if (M != 0 && (double)fabsf((float)v) > M) return true;
+    } else if (check == CHK_BASINS_REF) {
+        if (isnan(v) || isinf(v)) return true;
+        if (M != 0 && (double)fabsf((float)v) > M) return true;
+    } else {
+        if (!isfinite(v)) return true;
+        if (M != 0 && fabs(v) > M) return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ bool fixed_point(double t, int check)
+{
+    if (check == CHK_BIF_REF) return (double)sqrtf(fabsf((float)t)) < 1e-9;
+    if (check == CHK_BASINS_REF) return sqrt(fabs(t)) < 1e-9;
+    return sqrt(t) < 1e-9;
+}
+
+template <int NX>
+__device__ __forceinline__ double step_delta2(const double* X, const double* xp)
+{
+    double t = 0;
+#pragma unroll
+    for (int j = 0; j < NX; ++j) t += ((X[j] - xp[j]) * (X[j] - xp[j]));
+    return t;
+}
+
+// X[wv] for a compile-time wv (WVT >= 0) or a run-time one (WVT < 0).
+template <int NX, int WVT>
+__device__ __forceinline__ double pick(const double* X, int wv)
+{
+    if constexpr (WVT >= 0) {
+        return X[WVT];
+    } else {
+        double v = X[0];
+#pragma unroll
+        for (int k = 1; k < NX; ++k) v = (wv == k) ? X[k] : v;
+        return v;
+    }
+}
+
+// Lane state parked in HBM between engine launches (SoA, k-major).
+struct LaneStore {
+    double* d;     // [nd][np]
+    int* i;        // [ni][np]
+    long long np;  // lanes (multiple of 64)
+};
+
+// ---------------------------------------------------------------------------
+// LLE: Benettin with two trajectories (LLE/src/LLECUDA.cu:56-166)
+// ticks [0, S): transient on X; tick S starts with the perturbation; ticks
+// [S, S + W*NTp) step X and Xp once each, renormalising after every NTp.
+// ---------------------------------------------------------------------------
+template <int SYS>
+struct LleProg {
+    using Sy = System<SYS>;
+    static constexpr int NX = Sy::NX, NA = Sy::NA;
+    static constexpr int ND = 2 * NX + 1;
+    double X[NX], Xp[NX], a[NA], sum;
+
+    __device__ __forceinline__ static long long total(const LleArgs& p) { return (long long)p.S + (long long)p.W * p.NTp; }
+
+    __device__ __forceinline__ void params(const LleArgs& p, int i, int j)
+    {
+        build_params<NA>(a, p.values, p.idxA, grid_value(i, p.nPts, p.rA0, p.rA1), p.idxB,
+                         grid_value(j, p.nPts, p.rB0, p.rB1));
+    }
+    __device__ __forceinline__ void init(const LleArgs& p)
+    {
+#pragma unroll
+        for (int k = 0; k < NX; ++k) {
+            X[k] = p.ic[k];
+            Xp[k] = 0.0;
+        }
+        sum = 0.0;
+    }
+    __device__ __forceinline__ void ticks(const LleArgs& p, long long point, long long g0, long long g1)
+    {
+        long long g = g0;
+        if (g < p.S) {
+            const long long e = g1 < p.S ? g1 : p.S;
+            for (long long s = g; s < e; ++s) Sy::step(X, a, p.sc);
+            g = e;
+        }
+        if (g == p.S && g < g1) {
+            double v[NX];
+            unit_vector<NX>(p.seed, point, v);
+#pragma unroll
+            for (int k = 0; k < NX; ++k) {
+                const double d = p.eps * v[k];
+                Xp[k] = X[k] + d;
+            }
+        }
+        int local = g < g1 ? (int)((g - p.S) % p.NTp) : 0;  // position in the first window
+        while (g < g1) {
+            const long long left = g1 - g;
+            const int n = (p.NTp - local) < left ? (p.NTp - local) : (int)left;
+            for (int s = 0; s < n; ++s) {
+                Sy::step(X, a, p.sc);
+                Sy::step(Xp, a, p.sc);
+            }
+            g += n;
+            const bool window_end = local + n == p.NTp;
+            local = 0;
+            if (window_end) {
+                double d2 = 0.0;
+#pragma unroll
+                for (int l = 0; l < NX; ++l) {
+                    const double diff = X[l] - Xp[l];
+                    d2 = d2 + diff * diff;
+                }
+                const double dist = sqrt(d2) / p.eps;
+                sum = sum + log(dist);
+#pragma unroll
+                for (int l = 0; l < NX; ++l) Xp[l] = X[l] - ((X[l] - Xp[l]) / dist);
+            }
+        }
+    }
+    __device__ __forceinline__ double result(const LleArgs& p) const { return sum / p.tMax; }
+
+    __device__ __forceinline__ void load(const LaneStore& L, long long q)
+    {
+#pragma unroll
+        for (int k = 0; k < NX; ++k) {
+            X[k] = L.d[k * L.np + q];
+            Xp[k] = L.d[(NX + k) * L.np + q];
+        }
+        sum = L.d[2 * NX * L.np + q];
+    }
+    __device__ __forceinline__ void store(const LaneStore& L, long long q) const
+    {
+#pragma unroll
+        for (int k = 0; k < NX; ++k) {
+            L.d[k * L.np + q] = X[k];
+            L.d[(NX + k) * L.np + q] = Xp[k];
+        }
+        L.d[2 * NX * L.np + q] = sum;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Sweep (Bifurcation / Basins): ticks [0, S) transient, [S, S+B) samples
+// (each sample = pre steps).  Kernels and engine must keep all 64 lanes of
+// a wave in the loop (append_peaks is a wave-level operation).
+// ---------------------------------------------------------------------------
+template <int SYS, int KIND>
+struct SweepProg {
+    using Sy = System<SYS>;
+    static constexpr int NX = Sy::NX, NA = Sy::NA;
+    static constexpr int ND = NX + 8;  // X, p, dn, first, last, s0, s1, sx, sz
+    static constexpr int NI = 4;       // pstart, cnt, prev_t, bits
+    double X[NX], a[NA];
+    PeakDetector det;
+    int cnt, prev_t;
+    bool alive, fixed;
+    double s0, s1, sx, sz, first, last;
+
+    __device__ __forceinline__ static long long total(const SweepArgs& A) { return (long long)A.S + A.B; }
+
+    __device__ __forceinline__ static void grid(const SweepArgs& A, long long pq, double& g0, double& g1)
+    {
+        const int i = (int)(pq % A.nPts);
+        const int j = A.dims == 2 ? (int)(pq / A.nPts) : 0;
+        g0 = grid_value(i, A.nPts, A.r[0], A.r[1]);
+        g1 = A.dims == 2 ? grid_value(j, A.nPts, A.r[2], A.r[3]) : 0.0;
+    }
+    // Parameter vector of point pq.
+    __device__ __forceinline__ void params(const SweepArgs& A, long long pq)
+    {
+        if (KIND == KIND_BIF) {
+            // swept entries of the parameter vector (bifurcationCUDA.cu:282-284)
+            double g0, g1;
+            grid(A, pq, g0, g1);
+            build_params<NA>(a, A.values, A.idx0, g0, A.idx1, g1);
+        } else {
+#pragma unroll
+            for (int k = 0; k < NA; ++k) a[k] = A.values[k];
+        }
+    }
+    // Parameters and the initial lane state of point pq.
+    __device__ __forceinline__ void init(const SweepArgs& A, long long pq, bool in_range)
+    {
+        params(A, pq);
+        if (KIND == KIND_BIF) {
+#pragma unroll
+            for (int k = 0; k < NX; ++k) X[k] = A.ic[k];
+        } else {
+            // swept entries of the initial condition (basinsCUDA.cu:253-262)
+            double g0, g1;
+            grid(A, pq, g0, g1);
+#pragma unroll
+            for (int k = 0; k < NX; ++k) X[k] = (k == A.idx0) ? g0 : ((k == A.idx1) ? g1 : A.ic[k]);
+        }
+        det.reset();
+        cnt = 0;
+        prev_t = 0;
+        alive = in_range;
+        fixed = false;
+        s0 = s1 = sx = sz = first = last = 0.0;
+    }
+
+    template <int WVT>
+    __device__ __forceinline__ void check(const SweepArgs& A, int wv)
+    {
+        const double v = pick<NX, WVT>(X, wv);
+        if (alive && !(fabs(v) < A.L)) {
+            if (diverged_exact(v, A.maxValue, A.check)) alive = false;
+        }
+    }
+
+    template <int WVT>
+    __device__ __forceinline__ void ticks(const SweepArgs& A, int wv, WaveCursor& cur, int wave, long long g0,
+                                          long long g1)
+    {
+        long long g = g0;
+        // ---- transient ----
+        if (KIND == KIND_BIF) {
+            // checked every step; a transient that ends diverged or at a
+            // fixed point is not recorded and reports 0 (bifurcationCUDA.cu:334-342)
+            if (g < A.S - 1) {
+                const long long e = g1 < A.S - 1 ? g1 : A.S - 1;
+                for (long long s = g; s < e; ++s) {
+                    Sy::step(X, a, A.sc);
+                    check<WVT>(A, wv);
+                }
+                g = e;
+            }
+            if (g == A.S - 1 && g < g1) {
+                double xp[NX];
+#pragma unroll
+                for (int k = 0; k < NX; ++k) xp[k] = X[k];
+                Sy::step(X, a, A.sc);
+                check<WVT>(A, wv);
+                if (alive && fixed_point(step_delta2<NX>(X, xp), A.check)) alive = false;
+                ++g;
+            }
+        } else if (g < A.S) {
+            // no checks during the Basins transient (basinsCUDA.cu:264-271)
+            const long long e = g1 < A.S ? g1 : A.S;
+            for (long long s = g; s < e; ++s) Sy::step(X, a, A.sc);
+            g = e;
+        }
+        // ---- record ----
+        const int B = A.B;
+        const int tb = (int)(g - A.S), te = (int)(g1 - A.S);
+        for (int t = tb; t < te; ++t) {
+            const double v = pick<NX, WVT>(X, wv);
+            if (t == 0) first = v;
+            if (KIND == KIND_BASINS_CFG) {
+                sx = sx + X[0];
+                sz = sz + X[NX - 1];
+            }
+            double pv = 0.0;
+            int pt = 0;
+            const bool e = det.feed(t, v, pv, pt) && alive;
+            if (KIND == KIND_BIF) {
+                append_peaks(A.pool, cur, wave, e, pv, pt, cnt);
+                if (e) ++cnt;
+            } else if (KIND == KIND_BASINS_REF) {
+                // running sums of the kept peaks and intervals
+                // (avgPeakFinderCUDA, basinsCUDA.cu:841-861)
+                if (e) {
+                    if (cnt >= 1) {
+                        s0 = s0 + pv;
+                        s1 = s1 + (double)(pt - prev_t) * A.hp;
+                    }
+                    prev_t = pt;
+                    ++cnt;
+                }
+            } else {
+                if (e) {
+                    if (cnt >= 1) {
+                        s0 = s0 + pv;
+                        s1 = fma(pv, pv, s1);
+                    }
+                    ++cnt;
+                }
+            }
+            if (t == B - 1) {
+                last = v;
+                double xp[NX];
+#pragma unroll
+                for (int k = 0; k < NX; ++k) xp[k] = X[k];
+                for (int r = 0; r < A.pre; ++r) Sy::step(X, a, A.sc);
+                check<WVT>(A, wv);
+                fixed = alive && fixed_point(step_delta2<NX>(X, xp), A.check);
+            } else {
+                for (int r = 0; r < A.pre; ++r) Sy::step(X, a, A.sc);
+                check<WVT>(A, wv);
+            }
+        }
+    }
+
+    __device__ __forceinline__ void finish(const SweepArgs& A, long long q) const
+    {
+        const int B = A.B;
+        const int flag = alive ? (fixed ? -1 : 1) : 0;
+        if (A.flags) A.flags[q] = flag;
+        if (KIND == KIND_BIF) {
+            A.counts[q] = cnt;
+            A.first[q] = first;
+        } else if (KIND == KIND_BASINS_REF) {
+            double f0, f1;
+            if (flag == 0) {
+                f0 = 999;
+                f1 = 999;
+            } else if (flag == -1) {
+                f0 = last;
+                f1 = -1.0;
+            } else {
+                const int P = cnt > 1 ? cnt - 1 : 0;
+                if (P <= 0) {
+                    f0 = 1000;
+                    f1 = 1000;
+                } else {
+                    f0 = s0 / P;
+                    f1 = s1 / P;
+                }
+            }
+            A.f0[q] = f0;
+            A.f1[q] = f1;
+            if (A.f2) A.f2[q] = 0.0;
+            if (A.counts) A.counts[q] = cnt;
+        } else {
+            double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+            if (flag != 0) {
+                f0 = sx / B;
+                f1 = sz / B;
+                const int n = cnt > 1 ? cnt - 1 : 0;
+                if (n >= 2) {
+                    const double m = s0 / n;
+                    const double var = s1 / n - m * m;
+                    f2 = var > 0 ? sqrt(var) : 0.0;
+                }
+            }
+            A.f0[q] = f0;
+            A.f1[q] = f1;
+            A.f2[q] = f2;
+            if (A.counts) A.counts[q] = cnt;
+        }
+    }
+
+    __device__ __forceinline__ void load(const LaneStore& L, long long q)
+    {
+#pragma unroll
+        for (int k = 0; k < NX; ++k) X[k] = L.d[k * L.np + q];
+        const double* d = L.d + NX * L.np + q;
+        det.p = d[0];
+        det.dn = d[L.np];
+        first = d[2 * L.np];
+        last = d[3 * L.np];
+        s0 = d[4 * L.np];
+        s1 = d[5 * L.np];
+        sx = d[6 * L.np];
+        sz = d[7 * L.np];
+        det.pstart = L.i[q];
+        cnt = L.i[L.np + q];
+        prev_t = L.i[2 * L.np + q];
+        const int bits = L.i[3 * L.np + q];
+        alive = bits & 1;
+        fixed = bits & 2;
+        det.plat = bits & 4;
+        det.skip = bits & 8;
+    }
+    __device__ __forceinline__ void store(const LaneStore& L, long long q) const
+    {
+#pragma unroll
+        for (int k = 0; k < NX; ++k) L.d[k * L.np + q] = X[k];
+        double* d = L.d + NX * L.np + q;
+        d[0] = det.p;
+        d[L.np] = det.dn;
+        d[2 * L.np] = first;
+        d[3 * L.np] = last;
+        d[4 * L.np] = s0;
+        d[5 * L.np] = s1;
+        d[6 * L.np] = sx;
+        d[7 * L.np] = sz;
+        L.i[q] = det.pstart;
+        L.i[L.np + q] = cnt;
+        L.i[2 * L.np + q] = prev_t;
+        L.i[3 * L.np + q] = (alive ? 1 : 0) | (fixed ? 2 : 0) | (det.plat ? 4 : 0) | (det.skip ? 8 : 0);
+    }
+};
+
+}  // namespace chaos

@@ -11,6 +11,7 @@
 
 #include "csv.hpp"
 #include "dispatch.hpp"
+#include "lanes.hpp"
 #include "lle.hpp"
 
 namespace chaos {
@@ -21,49 +22,17 @@ template <int SYS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Occupancy<SYS>::waves)))
 void lle_map_kernel(const LleArgs p, double* __restrict__ out)
 {
-    using S = System<SYS>;
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= p.n_points) return;
     const int i = (int)(q % p.nPts);
     const int j = (int)(p.row0 + q / p.nPts);
-
-    double a[S::NA];
-    build_params<S::NA>(a, p.values, p.idxA, grid_value(i, p.nPts, p.rA0, p.rA1), p.idxB,
-                        grid_value(j, p.nPts, p.rB0, p.rB1));
-    double X[S::NX], Xp[S::NX], v[S::NX];
-#pragma unroll
-    for (int k = 0; k < S::NX; ++k) X[k] = p.ic[k];
-
-    // Transient (LLE/src/LLECUDA.cu:86).
-    for (int s = 0; s < p.S; ++s) S::step(X, a, p.sc);
-
-    // Perturbation eps * unit vector (LLE/src/LLECUDA.cu:88-102).
-    unit_vector<S::NX>(p.seed, (long long)j * p.nPts + i, v);
-#pragma unroll
-    for (int k = 0; k < S::NX; ++k) {
-        const double d = p.eps * v[k];
-        Xp[k] = X[k] + d;
-    }
-
-    // Windows (LLE/src/LLECUDA.cu:147-163).
-    double sum = 0.0;
-    for (int w = 0; w < p.W; ++w) {
-        for (int s = 0; s < p.NTp; ++s) {
-            S::step(X, a, p.sc);
-            S::step(Xp, a, p.sc);
-        }
-        double d2 = 0.0;
-#pragma unroll
-        for (int l = 0; l < S::NX; ++l) {
-            const double diff = X[l] - Xp[l];
-            d2 = d2 + diff * diff;
-        }
-        const double dist = sqrt(d2) / p.eps;
-        sum = sum + log(dist);
-#pragma unroll
-        for (int l = 0; l < S::NX; ++l) Xp[l] = X[l] - ((X[l] - Xp[l]) / dist);
-    }
-    out[q] = sum / p.tMax;
+    // transient (LLE/src/LLECUDA.cu:86), perturbation (:88-102) and the
+    // Benettin windows (:147-163) in one pass
+    LleProg<SYS> L;
+    L.params(p, i, j);
+    L.init(p);
+    L.ticks(p, (long long)j * p.nPts + i, 0, LleProg<SYS>::total(p));
+    out[q] = L.result(p);
 }
 
 LleArgs make_lle_args(const chaos_options& o, double tMax, double NT, int nPts, double h, double eps,

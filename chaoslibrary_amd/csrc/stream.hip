@@ -1,0 +1,402 @@
+// stream.hip -- the mixed-job stream driver and the engine autotuner.
+//
+// chaos_stream_run generates a seeded Poisson stream of bifurcation / LLE /
+// basins jobs and either serves it through the engine (continuous batching:
+// jobs are submitted when their arrival time passes and admitted at the next
+// round) or runs every job alone with the one-shot entry points in arrival
+// order.  Latency is completion time minus scheduled arrival time.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <thread>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace chaos {
+
+namespace {
+
+double clock_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Rng {
+    unsigned long long s;
+    double uniform()
+    {
+        const unsigned long long z = splitmix64(s);
+        return (double)(z >> 11) * 0x1p-53;
+    }
+};
+
+int log_uniform(Rng& r, int lo, int hi)
+{
+    const double u = r.uniform();
+    const double v = std::exp(std::log((double)lo) + u * (std::log((double)hi) - std::log((double)lo)));
+    return std::max(lo, std::min(hi, (int)std::lround(v)));
+}
+
+struct StreamJob {
+    chaos_job job;
+    double arrival_ms;
+    int kind3;   // 0 bifurcation, 1 LLE, 2 basins
+};
+
+chaos_job rossler_job()
+{
+    chaos_job j{};
+    j.system = CHAOS_SYS_ROSSLER_RK4;
+    j.semantics = CHAOS_SEM_CONFIG;
+    j.h = 0.01;
+    j.amountOfInitialConditions = 3;
+    j.initialConditions[0] = j.initialConditions[1] = j.initialConditions[2] = 0.1;
+    j.amountOfValues = 3;
+    j.values[0] = 0.2;
+    j.values[1] = 0.2;
+    j.values[2] = 5.7;
+    j.preScaller = 1;
+    j.seed = 12345;
+    j.min_pts = 20;
+    return j;
+}
+
+// Job physics of the benchmark configurations, horizons scaled.
+chaos_job make_stream_job(int kind3, int size, double scale)
+{
+    if (kind3 == 0) {  // 1-D Rossler bifurcation over c in [2,6]
+        chaos_job j = rossler_job();
+        j.kind = CHAOS_JOB_BIF1D;
+        j.nPts = size;
+        j.transientTime = 1000.0 * scale;
+        j.tMax = 1000.0 * scale;
+        j.ranges[0] = 2.0;
+        j.ranges[1] = 6.0;
+        j.indicesOfMutVars[0] = 2;
+        return j;
+    }
+    if (kind3 == 1) {  // Rossler LLE map over a in [0.05,0.35] x c in [2,10]
+        chaos_job j = rossler_job();
+        j.kind = CHAOS_JOB_LLE2D;
+        j.nPts = std::max(2, (int)std::lround(std::sqrt((double)size)));
+        j.transientTime = 500.0 * scale;
+        j.tMax = 2000.0 * scale;
+        j.NT = 0.5;
+        j.eps = 1e-8;
+        j.ranges[0] = 0.05;
+        j.ranges[1] = 0.35;
+        j.ranges[2] = 2.0;
+        j.ranges[3] = 10.0;
+        j.indicesOfMutVars[0] = 0;
+        j.indicesOfMutVars[1] = 2;
+        return j;
+    }
+    // Lorenz basins at rho = 24.5 over x0, y0 in [-20,20], z0 = 25
+    chaos_job j{};
+    j.kind = CHAOS_JOB_BASINS2D;
+    j.system = CHAOS_SYS_LORENZ_RK4;
+    j.semantics = CHAOS_SEM_CONFIG;
+    j.nPts = std::max(2, (int)std::lround(std::sqrt((double)size)));
+    j.h = 0.01;
+    j.transientTime = 300.0 * scale;
+    j.tMax = 300.0 * scale;
+    j.eps = 0.5;
+    j.min_pts = 20;
+    j.preScaller = 1;
+    j.amountOfInitialConditions = 3;
+    j.initialConditions[2] = 25.0;
+    j.amountOfValues = 3;
+    j.values[0] = 10.0;
+    j.values[1] = 24.5;
+    j.values[2] = 8.0 / 3.0;
+    j.ranges[0] = -20.0;
+    j.ranges[1] = 20.0;
+    j.ranges[2] = -20.0;
+    j.ranges[3] = 20.0;
+    j.indicesOfMutVars[0] = 0;
+    j.indicesOfMutVars[1] = 1;
+    j.seed = 12345;
+    return j;
+}
+
+std::vector<StreamJob> make_stream(const chaos_stream_config& c)
+{
+    Rng r{c.seed};
+    std::vector<StreamJob> v;
+    double t = 0.0;
+    for (;;) {
+        const double u = r.uniform();
+        t += -std::log(1.0 - u) / c.lambda;  // exponential inter-arrival
+        if (t > c.duration_s) break;
+        if (c.max_jobs > 0 && (int)v.size() >= c.max_jobs) break;
+        const int kind3 = std::min(2, (int)(r.uniform() * 3.0));
+        int size;
+        if (kind3 == 0) size = log_uniform(r, c.bif_min, c.bif_max);
+        else if (kind3 == 1) size = log_uniform(r, c.lle_min, c.lle_max);
+        else size = log_uniform(r, c.basins_min, c.basins_max);
+        v.push_back(StreamJob{make_stream_job(kind3, size, c.horizon_scale), t * 1000.0, kind3});
+    }
+    return v;
+}
+
+double job_steps(const chaos_job& j)
+{
+    const double h = j.h;
+    if (j.kind == CHAOS_JOB_LLE2D) {
+        const double S = std::llround(j.transientTime / h), W = std::llround(j.tMax / h) / std::llround(j.NT / h);
+        return (double)j.nPts * j.nPts * (S + 2.0 * W * std::llround(j.NT / h));
+    }
+    const double n = j.kind == CHAOS_JOB_BIF1D ? j.nPts : (double)j.nPts * j.nPts;
+    return n * ((double)std::llround(j.transientTime / h) + (double)std::llround(j.tMax / h));
+}
+
+// One job through the one-shot entry points with host outputs.
+void run_one_shot(const chaos_job& j, int device)
+{
+    chaos_options o;
+    chaos_options_init(&o);
+    o.system = j.system;
+    o.semantics = j.semantics;
+    o.seed = j.seed;
+    o.min_pts = j.min_pts;
+    o.device = device;
+    int rc = CHAOS_OK;
+    if (j.kind == CHAOS_JOB_LLE2D) {
+        std::vector<double> out((size_t)j.nPts * j.nPts);
+        rc = chaos_lle2d(&o, j.tMax, j.NT, j.nPts, j.h, j.eps, j.initialConditions, j.amountOfInitialConditions,
+                         j.ranges, j.indicesOfMutVars, j.writableVar, j.maxValue, j.transientTime, j.values,
+                         j.amountOfValues, out.data());
+    } else if (j.kind == CHAOS_JOB_BIF1D) {
+        std::vector<int> counts(j.nPts);
+        std::vector<double> first(j.nPts);
+        long long cap = (long long)j.nPts * (std::llround(j.tMax / j.h) / 64 + 16), total = 0;
+        std::vector<double> pk((size_t)cap), iv((size_t)cap);
+        rc = chaos_bif1d(&o, j.tMax, j.nPts, j.h, j.amountOfInitialConditions, j.initialConditions, j.ranges,
+                         j.indicesOfMutVars, j.writableVar, j.maxValue, j.transientTime, j.values, j.amountOfValues,
+                         j.preScaller, counts.data(), first.data(), pk.data(), iv.data(), cap, &total);
+        if (rc == CHAOS_E_CAPACITY) {
+            pk.resize((size_t)total);
+            iv.resize((size_t)total);
+            rc = chaos_bif1d(&o, j.tMax, j.nPts, j.h, j.amountOfInitialConditions, j.initialConditions, j.ranges,
+                             j.indicesOfMutVars, j.writableVar, j.maxValue, j.transientTime, j.values,
+                             j.amountOfValues, j.preScaller, counts.data(), first.data(), pk.data(), iv.data(), total,
+                             &total);
+        }
+    } else {
+        const size_t n = (size_t)j.nPts * j.nPts;
+        std::vector<int> labels(n), flags(n);
+        std::vector<double> feats(3 * n);
+        rc = chaos_basins2d(&o, j.tMax, j.nPts, j.h, j.amountOfInitialConditions, j.initialConditions, j.ranges,
+                            j.indicesOfMutVars, j.writableVar, j.maxValue, j.transientTime, j.values,
+                            j.amountOfValues, j.preScaller, j.eps, labels.data(), feats.data(), flags.data());
+    }
+    if (rc != CHAOS_OK) throw Error(rc, chaos_last_error());
+}
+
+double percentile(std::vector<double> v, double p)
+{
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    const double idx = p * (double)(v.size() - 1);
+    const size_t lo = (size_t)std::floor(idx), hi = (size_t)std::ceil(idx);
+    return v[lo] + (v[hi] - v[lo]) * (idx - (double)lo);
+}
+
+void sleep_until_ms(double t_abs)
+{
+    const double d = t_abs - clock_ms();
+    if (d > 0.05) std::this_thread::sleep_for(std::chrono::microseconds((long long)(d * 1000.0)));
+}
+
+}  // namespace
+
+void stream_run(int device, const chaos_stream_config& c, const chaos_engine_params& prm, chaos_stream_report& rep,
+                std::vector<double>& lat)
+{
+    CHAOS_REQUIRE(c.lambda > 0 && c.duration_s > 0 && c.horizon_scale > 0, "lambda, duration and scale must be > 0");
+    const std::vector<StreamJob> jobs = make_stream(c);
+    const size_t N = jobs.size();
+    lat.assign(N, 0.0);
+    rep = chaos_stream_report{};
+    rep.jobs = (int)N;
+    for (const StreamJob& s : jobs) {
+        rep.jobs_kind[s.kind3]++;
+        rep.rk4_steps += job_steps(s.job);
+    }
+    if (N == 0) return;
+    std::vector<double> done_ms(N, 0.0);
+    double t_start = 0.0;
+    if (c.mode == 0) {
+        Engine eng(device, prm);
+        std::vector<long long> ids(N, -1);
+        size_t next = 0, finished = 0;
+        t_start = clock_ms();
+        while (finished < N) {
+            const double now = clock_ms() - t_start;
+            while (next < N && jobs[next].arrival_ms <= now) ids[next] = eng.submit(jobs[next].job), ++next;
+            if (eng.idle()) {
+                if (next < N) sleep_until_ms(t_start + jobs[next].arrival_ms);
+                continue;
+            }
+            const int k = eng.step();
+            if (k > 0) {
+                const double t = clock_ms() - t_start;
+                for (size_t i = 0; i < next; ++i)
+                    if (ids[i] >= 0 && done_ms[i] == 0.0 && eng.job(ids[i]).status == 2) {
+                        done_ms[i] = t;
+                        eng.release(ids[i]);
+                        ids[i] = -1;
+                        ++finished;
+                    }
+            }
+        }
+        const chaos_server_stats st = eng.stats();
+        rep.busy_ms = st.busy_ms;
+        rep.launches = st.launches;
+    } else {
+        t_start = clock_ms();
+        for (size_t i = 0; i < N; ++i) {
+            sleep_until_ms(t_start + jobs[i].arrival_ms);
+            run_one_shot(jobs[i].job, device);
+            done_ms[i] = clock_ms() - t_start;
+        }
+    }
+    std::vector<double> per[3];
+    double last = 0.0;
+    for (size_t i = 0; i < N; ++i) {
+        lat[i] = done_ms[i] - jobs[i].arrival_ms;
+        per[jobs[i].kind3].push_back(lat[i]);
+        last = std::max(last, done_ms[i]);
+    }
+    rep.makespan_s = (last - jobs[0].arrival_ms) / 1000.0;
+    rep.steps_per_s = rep.rk4_steps / rep.makespan_s;
+    rep.jobs_per_s = (double)N / rep.makespan_s;
+    rep.p50_ms = percentile(lat, 0.50);
+    rep.p99_ms = percentile(lat, 0.99);
+    double s = 0;
+    for (double v : lat) s += v;
+    rep.mean_ms = s / (double)N;
+    rep.max_ms = *std::max_element(lat.begin(), lat.end());
+    for (int k = 0; k < 3; ++k) rep.p99_kind_ms[k] = percentile(per[k], 0.99);
+}
+
+}  // namespace chaos
+
+using namespace chaos;
+
+extern "C" int chaos_stream_config_default(chaos_stream_config* c)
+{
+    if (!c) return CHAOS_E_INVALID;
+    *c = chaos_stream_config{};
+    c->lambda = 200.0;      // jobs/s
+    c->duration_s = 60.0;
+    c->horizon_scale = 1.0;
+    c->seed = 2024;
+    c->bif_min = 256;
+    c->bif_max = 8192;
+    c->lle_min = 64 * 64;
+    c->lle_max = 1024 * 1024;
+    c->basins_min = 128 * 128;
+    c->basins_max = 1024 * 1024;
+    c->mode = 0;
+    c->max_jobs = 0;
+    return CHAOS_OK;
+}
+
+extern "C" int chaos_stream_run(int device, const chaos_stream_config* cfg, const chaos_engine_params* p,
+                                chaos_stream_report* rep, double* latencies_ms, int latencies_cap)
+{
+    return guarded("chaos_stream_run", [&] {
+        CHAOS_REQUIRE(cfg && rep, "NULL argument");
+        chaos_engine_params prm;
+        chaos_engine_params_default(&prm);
+        if (p) prm = *p;
+        std::vector<double> lat;
+        stream_run(device, *cfg, prm, *rep, lat);
+        if (latencies_ms)
+            for (int i = 0; i < latencies_cap && i < (int)lat.size(); ++i) latencies_ms[i] = lat[i];
+    });
+}
+
+// ---------------------------------------------------------------------------
+// autotuner
+// ---------------------------------------------------------------------------
+
+static double calibrate(int device, const chaos_engine_params& prm)
+{
+    // fixed mix: all three kinds, several sizes, submitted together
+    std::vector<chaos_job> mix;
+    const int sizes[4] = {4096, 9216, 16384, 36864};
+    for (int k = 0; k < 4; ++k) {
+        mix.push_back(make_stream_job(1, sizes[k], 0.02));
+        mix.push_back(make_stream_job(0, 512 * (k + 1), 0.05));
+        mix.push_back(make_stream_job(2, sizes[k], 0.05));
+    }
+    double steps = 0;
+    for (const chaos_job& j : mix) steps += job_steps(j);
+    Engine eng(device, prm);
+    for (const chaos_job& j : mix) eng.submit(j);
+    const double t0 = clock_ms();
+    eng.drain();
+    return steps / ((clock_ms() - t0) / 1000.0);
+}
+
+extern "C" int chaos_autotune(int device, const char* cache_path, chaos_engine_params* best,
+                              double* best_steps_per_s)
+{
+    return guarded("chaos_autotune", [&] {
+        CHAOS_REQUIRE(best, "best is NULL");
+        chaos_engine_params base;
+        chaos_engine_params_default(&base);
+        calibrate(device, base);  // warm-up: code objects, allocator
+        double best_rate = -1.0;
+        chaos_engine_params b = base;
+        const int blocks[2] = {64, 256};
+        const int waves[3] = {4, 6, 8};
+        const int steps[3] = {4000, 16000, 64000};
+        for (int bl : blocks)
+            for (int w : waves)
+                for (int k : steps) {
+                    chaos_engine_params p = base;
+                    p.block = bl;
+                    p.waves_per_simd = w;
+                    p.steps_per_launch = k;
+                    const double r = calibrate(device, p);
+                    if (r > best_rate) {
+                        best_rate = r;
+                        b = p;
+                    }
+                }
+        *best = b;
+        if (best_steps_per_s) *best_steps_per_s = best_rate;
+        if (cache_path) {
+            FILE* f = std::fopen(cache_path, "w");
+            if (!f) fail(CHAOS_E_IO, "cannot write autotune cache '%s'", cache_path);
+            std::fprintf(f, "block %d\nwaves_per_simd %d\nsteps_per_launch %d\nsteps_per_s %.6e\n", b.block,
+                         b.waves_per_simd, b.steps_per_launch, best_rate);
+            std::fclose(f);
+        }
+    });
+}
+
+extern "C" int chaos_autotune_load(const char* cache_path, chaos_engine_params* p)
+{
+    return guarded("chaos_autotune_load", [&] {
+        CHAOS_REQUIRE(cache_path && p, "NULL argument");
+        FILE* f = std::fopen(cache_path, "r");
+        if (!f) fail(CHAOS_E_IO, "cannot read autotune cache '%s'", cache_path);
+        chaos_engine_params_default(p);
+        char key[64];
+        double val;
+        while (std::fscanf(f, "%63s %lf", key, &val) == 2) {
+            const std::string k(key);
+            if (k == "block") p->block = (int)val;
+            else if (k == "waves_per_simd") p->waves_per_simd = (int)val;
+            else if (k == "steps_per_launch") p->steps_per_launch = (int)val;
+        }
+        std::fclose(f);
+    });
+}

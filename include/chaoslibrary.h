@@ -208,6 +208,127 @@ int chaos_basinsOfAttraction_2(const chaos_options* opt, double tMax, int nPts,
                                int preScaller, double eps,
                                const char* OUT_FILE_PATH);
 
+/* ---------------- Batched engine and job server ------------------------- */
+/* No counterpart in the reference (one synchronous call per sweep, one
+ * system per build).  Jobs of any of the four kinds run together in one
+ * persistent kernel that pulls (job, 64-point group, tick range) tiles from
+ * a device queue; lane state stays in HBM between launches; the launch
+ * sequence is replayed from a captured hipGraph.  Jobs are admitted FIFO as
+ * job slots and the HBM budget allow (continuous batching).  The engine runs
+ * the RK4 systems (CHAOS_SYS_ROSSLER_RK4, CHAOS_SYS_LORENZ_RK4); results are
+ * bit-identical to the one-shot entry points for the same job. */
+typedef enum chaos_job_kind {
+    CHAOS_JOB_LLE2D    = 0,   /* = chaos_lle2d      */
+    CHAOS_JOB_BIF1D    = 1,   /* = chaos_bif1d      */
+    CHAOS_JOB_BIF2D    = 2,   /* = chaos_bif2d      */
+    CHAOS_JOB_BASINS2D = 3    /* = chaos_basins2d   */
+} chaos_job_kind;
+
+typedef struct chaos_job {
+    int kind;                      /* chaos_job_kind                          */
+    int system;                    /* chaos_system (RK4 systems only)         */
+    int semantics;                 /* chaos_semantics                         */
+    int nPts;
+    double tMax, NT, h, eps, maxValue, transientTime;
+    int preScaller, writableVar, min_pts, amountOfInitialConditions;
+    uint64_t seed;
+    double initialConditions[4];
+    double values[8];
+    int amountOfValues;
+    int indicesOfMutVars[2];
+    int reserved;
+    double ranges[4];
+} chaos_job;
+
+typedef struct chaos_engine_params {
+    int block;              /* threads per persistent workgroup (64|128|256)  */
+    int waves_per_simd;     /* persistent waves per SIMD (1..8)               */
+    int steps_per_launch;   /* RK4 steps a tile advances per launch           */
+    int max_jobs;           /* active job slots                               */
+    int max_tiles;          /* tile queue capacity per launch                 */
+    int use_graph;          /* replay the launch sequence from a hipGraph     */
+    double hbm_fraction;    /* share of free HBM admitted jobs may occupy     */
+} chaos_engine_params;
+
+typedef struct chaos_job_info {
+    int status;             /* 0 pending, 1 active, 2 done, -1 failed         */
+    int kind;
+    double t_submit_ms, t_admit_ms, t_done_ms;   /* server clock              */
+    double rk4_steps;
+    long long footprint_bytes;
+    long long n_points;
+    long long result_len;   /* 1-D: kept peaks; others: n_points             */
+    int reran;              /* peak store overflowed: rerun one-shot          */
+    int reserved;
+} chaos_job_info;
+
+typedef struct chaos_server_stats {
+    long long launches;     /* engine launches (graph replays included)       */
+    long long graph_replays;
+    long long tiles;
+    double busy_ms;         /* device time of the engine launches             */
+    double finalize_ms;     /* host wall time in per-job finishing stages     */
+    double rk4_steps;       /* steps completed                                */
+    int active_jobs, pending_jobs;
+    long long hbm_in_use, hbm_budget;
+} chaos_server_stats;
+
+int chaos_engine_params_default(chaos_engine_params* p);
+int chaos_server_create(int device, const chaos_engine_params* p, void** server);
+int chaos_server_destroy(void* server);
+int chaos_server_submit(void* server, const chaos_job* job, long long* job_id);
+/* One scheduling round: admit, launch every active job's next tiles, finish
+ * completed jobs.  *completed receives the number finished this round. */
+int chaos_server_step(void* server, int* completed);
+int chaos_server_drain(void* server);
+int chaos_server_job_info(void* server, long long job_id, chaos_job_info* info);
+int chaos_server_stats_get(void* server, chaos_server_stats* stats);
+/* Results (host buffers, same layouts as the one-shot entry points). */
+int chaos_server_result_lle(void* server, long long job_id, double* out);
+int chaos_server_result_bif2d(void* server, long long job_id, int* out);
+int chaos_server_result_bif1d(void* server, long long job_id, int* counts,
+                              double* first, double* peaks, double* intervals,
+                              long long capacity, long long* total);
+int chaos_server_result_basins(void* server, long long job_id, int* labels,
+                               double* features, int* flags);
+int chaos_server_release(void* server, long long job_id);
+
+/* Seeded mixed-job stream: Poisson arrivals at rate lambda for duration_s,
+ * kinds uniform over {bifurcation 1-D Rossler, LLE Rossler, basins Lorenz},
+ * sizes log-uniform in [*_min, *_max] points, physics of the benchmark
+ * configurations with every horizon multiplied by horizon_scale.  mode 0
+ * serves them through the engine, mode 1 runs each job alone with the
+ * one-shot entry points in arrival order.  Latency = completion - arrival. */
+typedef struct chaos_stream_config {
+    double lambda, duration_s, horizon_scale;
+    uint64_t seed;
+    int bif_min, bif_max, lle_min, lle_max, basins_min, basins_max;
+    int mode;
+    int max_jobs;           /* cap on generated jobs (0 = none)               */
+} chaos_stream_config;
+
+typedef struct chaos_stream_report {
+    int jobs;
+    int jobs_kind[3];
+    double makespan_s, rk4_steps, steps_per_s, jobs_per_s;
+    double p50_ms, p99_ms, mean_ms, max_ms;
+    double p99_kind_ms[3];
+    double busy_ms;
+    long long launches;
+} chaos_stream_report;
+
+int chaos_stream_config_default(chaos_stream_config* cfg);
+int chaos_stream_run(int device, const chaos_stream_config* cfg,
+                     const chaos_engine_params* p, chaos_stream_report* rep,
+                     double* latencies_ms /* [jobs] or NULL */, int latencies_cap);
+
+/* Autotuner: times a fixed calibration mix over block size, waves per SIMD
+ * and steps per launch; writes the fastest to *best and, if cache_path is
+ * not NULL, to that file (read back by chaos_autotune_load). */
+int chaos_autotune(int device, const char* cache_path, chaos_engine_params* best,
+                   double* best_steps_per_s);
+int chaos_autotune_load(const char* cache_path, chaos_engine_params* p);
+
 #ifdef __cplusplus
 }
 #endif

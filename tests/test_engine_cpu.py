@@ -1,0 +1,42 @@
+"""Host-side checks of the engine/server API (no GPU needed)."""
+import pytest
+
+
+def test_engine_params_default(chaos):
+    p = chaos.engine_params()
+    assert (p.block, p.waves_per_simd, p.steps_per_launch, p.use_graph) == (256, 8, 20000, 1)
+    assert p.max_jobs >= 1 and p.max_tiles >= 1 and 0 < p.hbm_fraction <= 1
+
+
+def test_stream_config_default_matches_stream_spec(chaos):
+    c = chaos.stream_config()
+    assert c.lambda_ == 200.0 and c.duration_s == 60.0 and c.horizon_scale == 1.0
+    assert (c.bif_min, c.bif_max) == (256, 8192)
+    assert (c.lle_min, c.lle_max) == (64 * 64, 1024 * 1024)
+    assert (c.basins_min, c.basins_max) == (128 * 128, 1024 * 1024)
+
+
+def test_autotune_cache_roundtrip(chaos, tmp_path):
+    f = tmp_path / "tune.txt"
+    f.write_text("block 64\nwaves_per_simd 6\nsteps_per_launch 16000\nsteps_per_s 1.0e+12\n")
+    p = chaos.autotune_load(str(f))
+    assert (p.block, p.waves_per_simd, p.steps_per_launch) == (64, 6, 16000)
+    with pytest.raises(chaos.ChaosError):
+        chaos.autotune_load(str(tmp_path / "missing.txt"))
+
+
+def test_server_without_device_fails_cleanly(chaos):
+    if chaos.device_count() > 0:
+        pytest.skip("a device is present")
+    with pytest.raises(chaos.ChaosError) as e:
+        chaos.Server()
+    assert e.value.code in (-6, -2)
+
+
+def test_job_struct_layout(chaos):
+    import ctypes as C
+    j = chaos.make_job(chaos.JOB_LLE2D, "rossler_rk4", 1, 8, [0.1] * 3, [0.2, 0.2, 5.7], [0, 1, 2, 3], [0, 2])
+    # sizes and offsets of include/chaoslibrary.h structs on x86-64 (gcc)
+    assert C.sizeof(j) == 232 and chaos.Job.seed.offset == 80 and chaos.Job.ranges.offset == 200
+    assert C.sizeof(chaos.StreamConfig) == 64 and C.sizeof(chaos.StreamReport) == 120
+    assert j.amountOfInitialConditions == 3 and j.amountOfValues == 3 and list(j.indicesOfMutVars) == [0, 2]
