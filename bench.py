@@ -1,20 +1,27 @@
 #!/usr/bin/env python3
-"""Benchmark: FP64 RK4 steps/s of the 2048x2048 Rossler LLE map on one MI355X.
+"""Benchmarks on one MI355X.  Prints ONE JSON line on stdout (progress on stderr).
 
-Workload (the headline metric): Benettin LLE of the Rossler system
-x'=-y-z, y'=x+a*y, z'=b+z*(x-c) with b=0.2, a over linspace(0.05,0.35,2048)
-and c over linspace(2,10,2048), FP64 RK4 h=0.01 from (0.1,0.1,0.1), 500 time
-units of transient, then 4000 windows of 0.5 time units on the base and the
-perturbed trajectory (perturbation 1e-8 along a seeded unit vector), i.e.
-450,000 RK4 steps per grid point and 1.887e12 per map.  One "step" of this
-benchmark is one full map; its output (33.5 MB of FP64 exponents) is written
-to a device buffer, so nothing crosses PCIe inside the timed region.
+Default workload (the headline metric): FP64 RK4 steps/s of the 2048x2048
+Rossler LLE map -- Benettin LLE of x'=-y-z, y'=x+a*y, z'=b+z*(x-c), b=0.2,
+a over linspace(0.05,0.35,2048), c over linspace(2,10,2048), RK4 h=0.01 from
+(0.1,0.1,0.1), 500 time units of transient, then 4000 windows of 0.5 time
+units on the base and the perturbed trajectory (perturbation 1e-8 along a
+seeded unit vector): 450,000 RK4 steps per point, 1.887e12 per map.  One
+benchmark "step" is one full map written to a device buffer, so nothing
+crosses PCIe in the timed region.
 
-Prints ONE JSON line on stdout (progress goes to stderr).  The roofline block
-prices the map kernel's algorithmic FP64 work (63 FLOP per Rossler RK4 step,
-FMA = 2) against MI355X's vector-FP64 peak (78.6 TFLOP/s), using the kernel's
-own HIP-event time; cpu_baseline times the host oracle (the same per-point
-code, OpenMP over grid points) on a strided sample of the grid rows.
+--workload selects the other configurations (reported the same way):
+  bif1d   1-D Rossler bifurcation, c in [2,6] x 1000, 1000 + 1000 t.u.
+  bif2d   Lorenz peak-count map 1024^2 (rho x beta), 500 + 500 t.u., 1-D DBSCAN
+  basins  Lorenz basins 2048^2 ICs, rho=24.5, 300 + 300 t.u., 3-D DBSCAN
+  stream  the served mixed-job stream, engine vs one launch per job
+  copy    streaming copy of --bytes (reference for HBM traffic counters)
+
+The roofline block prices the dominant kernel's algorithmic FP64 work
+(63 FLOP per Rossler RK4 step, 71 per Lorenz step, FMA = 2) against MI355X's
+vector-FP64 peak (78.6 TFLOP/s) using the kernel's own HIP-event time;
+cpu_baseline times the host oracle (the same per-point code, OpenMP over grid
+points) on a bounded sample of the same workload.
 """
 import argparse
 import ctypes as C
@@ -28,17 +35,13 @@ sys.path.insert(0, ROOT)
 
 METRIC = "FP64 RHS steps/sec on 2048² LLE map (% FP64 peak); p50/p99 latency, mixed job stream"
 FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
 FLOP_PER_STEP = {"rossler_rk4": 63, "lorenz_rk4": 71}
+SYS_ID = {"rossler_rk4": 5, "lorenz_rk4": 6}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def lle_workload(args):
-    return dict(system="rossler_rk4", nPts=args.grid, tMax=2000.0, NT=0.5, h=0.01, eps=1e-8,
-                ic=[0.1, 0.1, 0.1], ranges=[0.05, 0.35, 2.0, 10.0], idx=[0, 2], transient=500.0,
-                values=[0.2, 0.2, 5.7], seed=12345, semantics=1)
 
 
 def cpu_threads():
@@ -52,115 +55,273 @@ def cpu_threads():
     return max(1, min(16, n))
 
 
-def cpu_baseline(w, target_s):
-    """Oracle LLE over a strided row sample, OpenMP over points."""
+def oracle():
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import numpy as np
-    import _oracle as O
-    n = w["nPts"]
-    threads = cpu_threads()
-    sid = O.SYS[w["system"]]
-    ic = np.asarray(w["ic"], dtype=np.float64)
-    rg = np.asarray(w["ranges"], dtype=np.float64)
-    ix = np.asarray(w["idx"], dtype=np.int32)
-    vals = np.asarray(w["values"], dtype=np.float64)
-    steps_pt = round(w["transient"] / w["h"]) + 2 * round(w["tMax"] / w["h"])
-
-    def run(rows, stride, cols_only=None):
-        out = np.zeros(rows * n)
-        t0 = time.perf_counter()
-        used = O.lib.orc_lle2d_omp(sid, w["semantics"], w["tMax"], w["NT"], n, w["h"], w["eps"], O.dptr(ic),
-                                   3, O.dptr(rg), O.iptr(ix), w["transient"], O.dptr(vals), 3, w["seed"],
-                                   O.dptr(out), 0, rows * stride, stride, threads)
-        return time.perf_counter() - t0, used
-
-    # probe: one row, to size the sample to about target_s seconds
-    dt1, used = run(1, n)
-    rate = n * steps_pt / dt1
-    rows = max(1, min(n, int(target_s * rate / (n * steps_pt))))
-    stride = max(1, n // rows)
-    rows = min(rows, n // stride)
-    dt, used = run(rows, stride)
-    pts = rows * n
-    return {"value": pts * steps_pt / dt, "unit": "RK4 steps/s", "cores": used, "kind": "port",
-            "sample": f"{rows} rows (every {stride}th) x {n} points = {pts} points of the same map, "
-                      f"{steps_pt} RK4 steps each, {dt:.1f} s on {used} OpenMP threads"}
+    import _oracle
+    return _oracle
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3, help="timed maps")
-    ap.add_argument("--warmup", type=int, default=1, help="untimed maps")
-    ap.add_argument("--grid", type=int, default=2048)
-    ap.add_argument("--block", type=int, default=0)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
+def roofline(system, steps, kernel_s, name, kernel_ms):
+    achieved = steps * FLOP_PER_STEP[system] / kernel_s / 1e12
+    return {"bound": "valu_fp64", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None, "kernel": name,
+            "kernel_ms": round(kernel_ms, 3)}
 
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only: the LLE map shards by grid rows "
-                          "with no exchange, but multi-GPU runs are not part of it; use --gpus 1"}))
-        sys.exit(2)
 
+def base_result(args, value, wall, steps, dtype="f64"):
+    return {"metric": METRIC, "value": value, "unit": "RK4 steps/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": dtype}
+
+
+def timed(args, fn):
+    import chaoslibrary_amd as ch
+    for i in range(args.warmup):
+        st = fn()
+        log(f"warmup {i}: kernel {st[0]:.1f} ms analysis {st[1]:.1f} ms")
+    ch._check(ch.lib.chaos_device_synchronize(0))
+    kms, ams, steps = [], [], 0.0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        k, a, s = fn()
+        kms.append(k)
+        ams.append(a)
+        steps += s
+        log(f"step {i}: kernel {k:.1f} ms analysis {a:.1f} ms")
+    ch._check(ch.lib.chaos_device_synchronize(0))
+    wall = time.perf_counter() - t0
+    return wall, sum(kms) / len(kms), sum(ams) / len(ams), steps
+
+
+# ---------------------------------------------------------------------------
+# LLE map (headline)
+# ---------------------------------------------------------------------------
+
+def run_lle(args):
     import numpy as np
     import chaoslibrary_amd as ch
-
-    w = lle_workload(args)
-    n = w["nPts"]
+    n = args.grid
+    w = dict(system="rossler_rk4", tMax=2000.0, NT=0.5, h=0.01, eps=1e-8, ic=[0.1, 0.1, 0.1],
+             ranges=[0.05, 0.35, 2.0, 10.0], idx=[0, 2], transient=500.0, values=[0.2, 0.2, 5.7], seed=12345)
     stats = ch.Stats()
     dptr = C.c_void_p()
     ch._check(ch.lib.chaos_device_malloc(C.byref(dptr), n * n * 8, 0))
 
-    def one_map():
+    def one():
         ch.lle2d(w["tMax"], w["NT"], n, w["h"], w["eps"], w["ic"], w["ranges"], w["idx"], w["transient"],
-                 w["values"], system=w["system"], semantics=w["semantics"], seed=w["seed"],
-                 block=args.block, stats=stats, out_device_ptr=dptr.value)
-        return stats.kernel_ms, stats.rk4_steps
+                 w["values"], system=w["system"], semantics=1, seed=w["seed"], block=args.block, stats=stats,
+                 out_device_ptr=dptr.value)
+        return stats.kernel_ms, 0.0, stats.rk4_steps
 
-    for i in range(args.warmup):
-        kms, _ = one_map()
-        log(f"warmup {i}: kernel {kms:.1f} ms")
-    ch._check(ch.lib.chaos_device_synchronize(0))
-    kernel_ms = []
-    steps_total = 0.0
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        kms, st = one_map()
-        kernel_ms.append(kms)
-        steps_total += st
-        log(f"step {i}: kernel {kms:.1f} ms")
-    ch._check(ch.lib.chaos_device_synchronize(0))
-    wall = time.perf_counter() - t0
-
+    wall, kms, _, steps = timed(args, one)
     host = np.empty(n * n)
     ch._check(ch.lib.chaos_memcpy_d2h(host.ctypes.data, dptr, n * n * 8))
     ch._check(ch.lib.chaos_device_free(dptr))
     if not np.isfinite(host).all():
         log("warning: non-finite exponents in the map")
-
-    value = steps_total / wall
-    avg_kernel_s = sum(kernel_ms) / len(kernel_ms) / 1e3
-    steps_per_map = steps_total / args.steps
-    achieved = steps_per_map * FLOP_PER_STEP[w["system"]] / avg_kernel_s / 1e12
-    roof = {"bound": "valu_fp64", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
-            "kernel": f"lle_map_kernel<{5 if w['system'] == 'rossler_rk4' else 6}>",
-            "kernel_ms": round(avg_kernel_s * 1e3, 3)}
-    res = {
-        "metric": METRIC, "value": value, "unit": "RK4 steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic: seeded 2048x2048 Rossler (a,c) parameter grid, no input data",
-        "config": {"workload": f"lle2d_rossler_rk4_{n}x{n}", "grid": n, "h": w["h"],
-                   "transient_time": w["transient"], "tMax": w["tMax"], "NT": w["NT"], "eps": w["eps"],
-                   "seed": w["seed"], "rk4_steps_per_map": steps_per_map,
-                   "flop_per_step": FLOP_PER_STEP[w["system"]]},
-        "roofline": roof,
-    }
+    per_map = steps / args.steps
+    res = base_result(args, steps / wall, wall, steps)
+    res["data"] = "synthetic: seeded 2048x2048 Rossler (a,c) parameter grid, no input data"
+    res["config"] = {"workload": f"lle2d_rossler_rk4_{n}x{n}", "grid": n, "h": w["h"],
+                     "transient_time": w["transient"], "tMax": w["tMax"], "NT": w["NT"], "eps": w["eps"],
+                     "seed": w["seed"], "rk4_steps_per_map": per_map, "flop_per_step": 63}
+    res["roofline"] = roofline(w["system"], per_map, kms / 1e3, "lle_map_kernel<5>", kms)
     if not args.no_cpu:
         log("cpu baseline ...")
-        res["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
+        O = oracle()
+        ic = np.asarray(w["ic"])
+        rg = np.asarray(w["ranges"])
+        ix = np.asarray(w["idx"], dtype=np.int32)
+        vals = np.asarray(w["values"])
+        steps_pt = round(w["transient"] / w["h"]) + 2 * round(w["tMax"] / w["h"])
+        threads = cpu_threads()
+
+        def cpu(rows, stride):
+            out = np.zeros(rows * n)
+            t0 = time.perf_counter()
+            used = O.lib.orc_lle2d_omp(SYS_ID[w["system"]], 1, w["tMax"], w["NT"], n, w["h"], w["eps"],
+                                       O.dptr(ic), 3, O.dptr(rg), O.iptr(ix), w["transient"], O.dptr(vals), 3,
+                                       w["seed"], O.dptr(out), 0, rows * stride, stride, threads)
+            return time.perf_counter() - t0, used
+
+        dt1, _ = cpu(1, n)  # probe one row to size the sample
+        rows = max(1, min(n, int(args.cpu_seconds * n * steps_pt / dt1 / (n * steps_pt))))
+        stride = max(1, n // rows)
+        rows = min(rows, n // stride)
+        dt, used = cpu(rows, stride)
+        res["cpu_baseline"] = {
+            "value": rows * n * steps_pt / dt, "unit": "RK4 steps/s", "cores": used, "kind": "port",
+            "sample": f"{rows} rows (every {stride}th) x {n} points of the same map = {rows * n} points, "
+                      f"{steps_pt} RK4 steps each, {dt:.1f} s on {used} OpenMP threads"}
+    return res
+
+
+# ---------------------------------------------------------------------------
+# Bifurcation / basins configurations
+# ---------------------------------------------------------------------------
+
+def run_bif1d(args):
+    import chaoslibrary_amd as ch
+    stats = ch.Stats()
+    n = 1000
+
+    def one():
+        ch.bif1d(1000.0, n, 0.01, [0.1, 0.1, 0.1], [2.0, 6.0], [2], 0, 0.0, 1000.0, [0.2, 0.2, 5.7], 1,
+                 system="rossler_rk4", semantics=1, stats=stats)
+        return stats.kernel_ms, stats.analysis_ms, stats.rk4_steps
+
+    wall, kms, ams, steps = timed(args, one)
+    res = base_result(args, steps / wall, wall, steps)
+    res["metric"] = "FP64 RK4 steps/s, 1-D Rossler bifurcation (1000 points)"
+    res["data"] = "synthetic: c = linspace(2,6,1000), a = b = 0.2"
+    res["config"] = {"workload": "bif1d_rossler_rk4_1000", "points": n, "transient_time": 1000.0, "tMax": 1000.0,
+                     "h": 0.01, "peaks": stats.peaks}
+    res["roofline"] = roofline("rossler_rk4", steps / args.steps, kms / 1e3, "sweep_kernel<5,0,0>", kms)
+    res["roofline"]["note"] = "1000 lanes = 16 waves on a 1024-SIMD GPU: latency-bound by design"
+    res["analysis_ms"] = ams
+    if not args.no_cpu:
+        O = oracle()
+        t0 = time.perf_counter()
+        O.bif1d(O.SYS["rossler_rk4"], 1, 1000.0, 100, 0.01, [0.1, 0.1, 0.1], [2.0, 6.0], [2], 0, 0.0, 1000.0,
+                [0.2, 0.2, 5.7], 1)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": 100 * 200000 / dt, "unit": "RK4 steps/s", "cores": 1, "kind": "port",
+                               "sample": f"100 of the points (c over [2,6] x 100), {dt:.1f} s single-threaded"}
+    return res
+
+
+def run_bif2d(args):
+    import chaoslibrary_amd as ch
+    stats = ch.Stats()
+    n = args.grid if args.grid != 2048 else 1024
+    dptr = C.c_void_p()
+    ch._check(ch.lib.chaos_device_malloc(C.byref(dptr), n * n * 4, 0))
+
+    def one():
+        ch.bif2d(500.0, n, 0.01, [1.0, 1.0, 1.0], [20.0, 200.0, 0.5, 4.0], [1, 2], 0, 0.0, 500.0,
+                 [10.0, 28.0, 8 / 3], 1, 1e-2, system="lorenz_rk4", semantics=1, stats=stats,
+                 out_device_ptr=dptr.value)
+        return stats.kernel_ms, stats.analysis_ms, stats.rk4_steps
+
+    wall, kms, ams, steps = timed(args, one)
+    ch._check(ch.lib.chaos_device_free(dptr))
+    res = base_result(args, steps / wall, wall, steps)
+    res["metric"] = f"FP64 RK4 steps/s, Lorenz peak-count map {n}^2 (whole pipeline)"
+    res["data"] = "synthetic: rho = linspace(20,200), beta = linspace(0.5,4), sigma = 10"
+    res["config"] = {"workload": f"bif2d_lorenz_rk4_{n}x{n}", "grid": n, "transient_time": 500.0, "tMax": 500.0,
+                     "h": 0.01, "dbscan_eps": 1e-2, "peaks": stats.peaks}
+    res["roofline"] = roofline("lorenz_rk4", steps / args.steps, kms / 1e3, "sweep_kernel<6,0,0>", kms)
+    res["analysis_ms"] = ams
+    peak_bytes = stats.peaks * (16 + 16 + 12)  # store + scatter read + segment write
+    res["analysis_hbm_gbs"] = peak_bytes / (ams / 1e3) / 1e9 if ams > 0 else None
+    return res
+
+
+def run_basins(args):
+    import chaoslibrary_amd as ch
+    stats = ch.Stats()
+    n = args.grid
+    dptr = C.c_void_p()
+    ch._check(ch.lib.chaos_device_malloc(C.byref(dptr), n * n * 4, 0))
+
+    def one():
+        o = ch.make_options("lorenz_rk4", 1, stats=stats, device_buffers=True)
+        ic, rg, ix = ch._f64([0.0, 0.0, 25.0]), ch._f64([-20.0, 20.0, -20.0, 20.0]), ch._i32([0, 1])
+        vals = ch._f64([10.0, 24.5, 8 / 3])
+        ch._check(ch.lib.chaos_basins2d(C.byref(o), 300.0, n, 0.01, 3, ch._dp(ic), ch._dp(rg), ch._ip(ix), 0, 0.0,
+                                        300.0, ch._dp(vals), 3, 1, 0.5, dptr, None, None))
+        return stats.kernel_ms, stats.analysis_ms, stats.rk4_steps
+
+    wall, kms, ams, steps = timed(args, one)
+    ch._check(ch.lib.chaos_device_free(dptr))
+    res = base_result(args, steps / wall, wall, steps)
+    res["metric"] = f"FP64 RK4 steps/s, Lorenz basins {n}^2 (features + z-score + 3-D DBSCAN)"
+    res["data"] = "synthetic: IC grid x0, y0 in [-20,20], z0 = 25; rho = 24.5"
+    res["config"] = {"workload": f"basins_lorenz_rk4_{n}x{n}", "grid": n, "transient_time": 300.0, "tMax": 300.0,
+                     "h": 0.01, "eps": 0.5, "min_pts": 20}
+    res["roofline"] = roofline("lorenz_rk4", steps / args.steps, kms / 1e3, "sweep_kernel<6,0,2>", kms)
+    res["analysis_ms"] = ams
+    feat_bytes = n * n * (24 + 4) * 4  # features read by ~4 passes + labels
+    res["dbscan_hbm_gbs"] = feat_bytes / (ams / 1e3) / 1e9 if ams > 0 else None
+    return res
+
+
+def run_stream(args):
+    import numpy as np
+    import chaoslibrary_amd as ch
+    prm = ch.engine_params(steps_per_launch=args.spl) if args.spl else ch.engine_params()
+    if args.tune:
+        prm, rate = ch.autotune()
+        log(f"autotune: block {prm.block} waves {prm.waves_per_simd} steps {prm.steps_per_launch} ({rate:.3e} steps/s)")
+    out = {}
+    for mode, name in ((0, "batched"), (1, "per_job")):
+        cfg = ch.stream_config(lambda_=args.lam, duration_s=args.duration, horizon_scale=args.scale, seed=2024,
+                               mode=mode)
+        log(f"stream {name}: lambda {args.lam}/s for {args.duration} s, horizons x{args.scale}")
+        rep, lat = ch.stream_run(cfg, prm)
+        out[name] = {k: v for k, v in rep.as_dict().items()}
+        log(f"  jobs {rep.jobs} makespan {rep.makespan_s:.2f} s steps/s {rep.steps_per_s:.3e} "
+            f"p50 {rep.p50_ms:.1f} ms p99 {rep.p99_ms:.1f} ms")
+    ratio = out["batched"]["steps_per_s"] / out["per_job"]["steps_per_s"]
+    res = {"metric": "served mixed-job stream: RK4 steps/s (engine) with p50/p99 job latency",
+           "value": out["batched"]["steps_per_s"], "unit": "RK4 steps/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+           "ms_per_step": out["batched"]["makespan_s"] * 1e3, "higher_is_better": True, "scaling": "strong",
+           "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic seeded Poisson stream (bifurcation / LLE / basins jobs, log-uniform sizes)",
+           "config": {"workload": "served_stream", "lambda": args.lam, "duration_s": args.duration,
+                      "horizon_scale": args.scale, "engine": {"block": prm.block,
+                                                              "waves_per_simd": prm.waves_per_simd,
+                                                              "steps_per_launch": prm.steps_per_launch}},
+           "batched": out["batched"], "per_job": out["per_job"], "throughput_ratio": ratio,
+           "p99_ratio": out["batched"]["p99_ms"] / out["per_job"]["p99_ms"]}
+    return res
+
+
+def run_copy(args):
+    import chaoslibrary_amd as ch
+    nb = args.bytes
+    a, b = C.c_void_p(), C.c_void_p()
+    ch._check(ch.lib.chaos_device_malloc(C.byref(a), nb, 0))
+    ch._check(ch.lib.chaos_device_malloc(C.byref(b), nb, 0))
+    for _ in range(args.warmup):
+        ch._check(ch.lib.chaos_device_copy(b, a, nb, 0))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ch._check(ch.lib.chaos_device_copy(b, a, nb, 0))
+    wall = time.perf_counter() - t0
+    ch._check(ch.lib.chaos_device_free(a))
+    ch._check(ch.lib.chaos_device_free(b))
+    gbs = 2 * nb * args.steps / wall / 1e9
+    return {"metric": "streaming copy GB/s (read + write)", "value": gbs, "unit": "GB/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+            "data": "device buffers", "config": {"workload": "copy", "bytes": nb}}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed runs")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed runs")
+    ap.add_argument("--workload", default="lle", choices=["lle", "bif1d", "bif2d", "basins", "stream", "copy"])
+    ap.add_argument("--grid", type=int, default=2048)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--lam", type=float, default=200.0, help="stream arrival rate (jobs/s)")
+    ap.add_argument("--duration", type=float, default=60.0, help="stream arrival window (s)")
+    ap.add_argument("--scale", type=float, default=1.0, help="stream horizon scale")
+    ap.add_argument("--spl", type=int, default=0, help="engine steps per launch (0 = default)")
+    ap.add_argument("--tune", action="store_true", help="autotune the engine before the stream")
+    ap.add_argument("--bytes", type=int, default=1 << 30)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only: the maps shard by grid rows with no "
+                          "exchange, but multi-GPU runs are not part of it; use --gpus 1"}))
+        sys.exit(2)
+    res = {"lle": run_lle, "bif1d": run_bif1d, "bif2d": run_bif2d, "basins": run_basins, "stream": run_stream,
+           "copy": run_copy}[args.workload](args)
     print(json.dumps(res), flush=True)
 
 

@@ -74,6 +74,7 @@ _SIGS = {
     "chaos_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "chaos_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "chaos_device_synchronize": (C.c_int, [C.c_int]),
+    "chaos_device_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
     "chaos_integrate": (C.c_int, [C.POINTER(Options), C.c_int, _D, _D, C.c_longlong, C.c_double]),
     "chaos_lle2d": (C.c_int, [C.POINTER(Options), C.c_double, C.c_double, C.c_int, C.c_double,
                               C.c_double, _D, C.c_int, _D, _I, C.c_int, C.c_double, C.c_double, _D,

@@ -113,6 +113,9 @@ int chaos_device_free(void* ptr);
 int chaos_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int chaos_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int chaos_device_synchronize(int device);
+/* Device-to-device streaming copy (bytes a multiple of 16); also the
+ * known-size reference kernel for HBM traffic counters. */
+int chaos_device_copy(void* dst, const void* src, size_t bytes, int device);
 
 /* Advance n trajectories nsteps each: states[n][NX] in/out, params[n][NA].
  * The per-step map is the selected system's calcDiscreteModel
