@@ -82,6 +82,8 @@ _SIGS = {
     "chaos_LLE2D": (C.c_int, [C.POINTER(Options), C.c_double, C.c_double, C.c_int, C.c_double,
                               C.c_double, _D, C.c_int, _D, _I, C.c_int, C.c_double, C.c_double, _D,
                               C.c_int, C.c_char_p]),
+    "chaos_dbscan": (C.c_int, [C.POINTER(Options), _D, _D, _D, C.c_longlong, C.c_int, C.c_double, C.c_int,
+                               C.c_int, _I, _I, _I]),
 }
 
 
@@ -298,6 +300,21 @@ def basins2d(tMax, nPts, h, initialConditions, ranges, indicesOfMutVars, writabl
     return labels, feats, flags
 
 
+def dbscan(points, eps, min_pts=20, sign_rule=False, include=None, **kw):
+    """chaos_dbscan over points[n, dim] (dim 2 or 3): (labels[n], K)."""
+    pts = _f64(points)
+    n, dim = pts.shape
+    cols = [np.ascontiguousarray(pts[:, d]) for d in range(dim)]
+    inc = None if include is None else _i32(include)
+    labels = np.zeros(n, dtype=np.int32)
+    k = C.c_int(0)
+    o = make_options(**kw)
+    _check(lib.chaos_dbscan(C.byref(o), _dp(cols[0]), _dp(cols[1]), _dp(cols[2]) if dim == 3 else None, n, dim,
+                            float(eps), int(min_pts), 1 if sign_rule else 0, None if inc is None else _ip(inc),
+                            _ip(labels), C.byref(k)))
+    return labels, k.value
+
+
 def basinsOfAttraction_2(tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges, indicesOfMutVars,
                          writableVar, maxValue, transientTime, values, amountOfValues, preScaller, eps,
                          OUT_FILE_PATH, options=None):
@@ -356,7 +373,8 @@ class StreamReport(C.Structure):
     _fields_ = [("jobs", C.c_int), ("jobs_kind", C.c_int * 3), ("makespan_s", C.c_double),
                 ("rk4_steps", C.c_double), ("steps_per_s", C.c_double), ("jobs_per_s", C.c_double),
                 ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
-                ("p99_kind_ms", C.c_double * 3), ("busy_ms", C.c_double), ("launches", C.c_longlong)]
+                ("p99_kind_ms", C.c_double * 3), ("busy_ms", C.c_double), ("launches", C.c_longlong),
+                ("finalize_ms", C.c_double)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_}

@@ -105,8 +105,7 @@ SweepArgs make_bif_args(const chaos_options& o, const BifInputs& in)
 long long bif_initial_chunks(const SweepArgs& A)
 {
     const long long n = A.n_points, waves = (n + 63) / 64;
-    size_t free_b = 0, total_b = 0;
-    CHAOS_HIP(hipMemGetInfo(&free_b, &total_b));
+    const size_t free_b = dev_mem_available();
     // First guess: one peak per 32 samples (at most one per two is possible);
     // an overflow is detected on the device and the sweep rerun with the
     // exact per-wave need.

@@ -6,13 +6,23 @@
 // per visited point, O(N) launches and O(N^2) work) with a fixed sequence of
 // launches:
 //   bbox -> hash cells of side eps/sqrt(dim) -> per-cell point lists
-//   core test (a cell holding >= minPts points is all-core: its diagonal is
-//   below eps; otherwise count neighbours in the 5^dim cell block, early exit)
+//   per-cell exact bounding box of the member points: a neighbour cell whose
+//   box lies wholly beyond eps of a point is skipped, one wholly within eps is
+//   taken as a whole, only the rest are scanned point by point
+//   core test: a cell holding >= minPts points is all-core (its diagonal is
+//   below eps); every other point is tested by one wavefront that sums whole
+//   neighbour cells and scans the undecided ones 64 points at a time
 //   union-find: cell members to the cell's smallest core index, then one
-//   witness pair per neighbouring cell pair; links always go larger -> smaller
-//   index, so every root is its component's smallest core index
+//   witness pair per neighbouring cell pair, searched by one wavefront per
+//   cell in 64 x 64 tiles, nearest neighbour rings first so that most far
+//   pairs are already connected; links always go larger -> smaller index, so every
+//   root is its component's smallest core index
 //   border points take their smallest-index core neighbour's root
 //   numbering by mark + exclusive scan over point indices.
+// Crowded cells (a basin that converges to a fixed point puts millions of
+// points into one cell) would serialise the per-point atomics on a few
+// addresses: lanes of a wavefront that share a cell combine their update and
+// one lane issues it.
 #include <climits>
 #include <cmath>
 #include <vector>
@@ -37,6 +47,8 @@ struct GridDev {
     long long* hstart;
     int* hcur;
     int* hrep;      // smallest core index in the cell (INT_MAX if none)
+    unsigned long long* bbox;  // [6][H]: per-cell min x,y,z then max x,y,z (order-preserving encoding)
+    long long H;
     int* pslot;     // slot of each point's cell
     int* cellpts;   // points grouped by cell
     const int* include;
@@ -73,6 +85,19 @@ __device__ __forceinline__ int lookup(const GridDev& G, unsigned long long key)
     }
 }
 
+// Order-preserving map double -> u64 (for atomicMin/atomicMax on doubles).
+__device__ __forceinline__ unsigned long long ord_enc(double v)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ double ord_dec(unsigned long long e)
+{
+    const unsigned long long u = (e >> 63) ? (e & 0x7fffffffffffffffull) : ~e;
+    return __longlong_as_double((long long)u);
+}
+
 __device__ __forceinline__ bool within(const GridDev& G, long long p, long long q)
 {
     double d2 = 0.0;
@@ -95,6 +120,28 @@ __device__ __forceinline__ double box_gap2(const GridDev& G, long long p, const 
         g2 += g * g;
     }
     return g2;
+}
+
+// Relation of point p to the member points of cell `slot`, from the cell's
+// exact bounding box: 0 = none within eps, 2 = all within eps, 1 = undecided.
+// The gap and far distances are formed with the same operations, in the same
+// order, as within(); rounding is monotone, so every member's computed
+// distance lies between them and both verdicts agree with within() exactly.
+__device__ __forceinline__ int box_relation(const GridDev& G, long long p, int slot)
+{
+    double g2 = 0.0, f2 = 0.0;
+    for (int d = 0; d < G.dim; ++d) {
+        const double lo = ord_dec(G.bbox[d * G.H + slot]);
+        const double hi = ord_dec(G.bbox[(3 + d) * G.H + slot]);
+        const double v = G.x[d][p];
+        const double g = v < lo ? lo - v : (v > hi ? v - hi : 0.0);
+        const double f = fmax(v - lo, hi - v);
+        g2 = g2 + g * g;
+        f2 = f2 + f * f;
+    }
+    if (sqrt(g2) > G.eps) return 0;
+    if (sqrt(f2) <= G.eps) return 2;
+    return 1;
 }
 
 // Visit every cell of the 5^dim block around p's cell that may hold a point
@@ -177,61 +224,243 @@ __global__ void bbox_kernel(const GridDev G, long long n, double* out)
         }
 }
 
+// Slot of `key` in the open-addressing table, inserting it if absent.  A
+// non-empty slot never changes, so a plain load that sees a key is final; an
+// empty one is claimed by compare-and-swap.
+__device__ __forceinline__ int hash_insert(const GridDev& G, unsigned long long key)
+{
+    unsigned long long h = mix64(key) & G.hmask;
+    while (true) {
+        const unsigned long long k = G.hkey[h];
+        if (k == key) return (int)h;
+        if (k == kEmpty) {
+            const unsigned long long prev = atomicCAS(&G.hkey[h], kEmpty, key);
+            if (prev == kEmpty || prev == key) return (int)h;
+        }
+        h = (h + 1) & G.hmask;
+    }
+}
+
+// Wavefront groups: up to kAggGroups rounds, each taking the lowest pending
+// lane as leader and every pending lane with the leader's key as its group.
+// Lanes left pending afterwards (many distinct keys) update on their own.
+constexpr int kAggGroups = 4;
+
+// Grid of the wavefront-per-item kernels (blocks of 256): 8192 waves, a full
+// complement for 256 CUs; the waves stride over the device-side item count.
+constexpr unsigned kWaveBlocks = 2048;
+
+__device__ __forceinline__ unsigned long long lanes_below() { return (1ull << __lane_id()) - 1ull; }
+
 __global__ void insert_kernel(const GridDev G, long long n)
 {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    if (!G.include[p]) {
-        G.pslot[p] = -1;
-        return;
+    const bool valid = p < n && G.include[p];
+    unsigned long long key = kEmpty;
+    if (valid) {
+        long long c[3] = {0, 0, 0};
+        cell_coords(G, p, c);
+        key = pack_cell(G.dim, c);
     }
-    long long c[3] = {0, 0, 0};
-    cell_coords(G, p, c);
-    const unsigned long long key = pack_cell(G.dim, c);
-    unsigned long long h = mix64(key) & G.hmask;
-    while (true) {
-        const unsigned long long prev = atomicCAS(&G.hkey[h], kEmpty, key);
-        if (prev == kEmpty || prev == key) break;
-        h = (h + 1) & G.hmask;
+    const int lane = __lane_id();
+    bool pending = valid;
+    int slot = -1;
+    for (int it = 0; it < kAggGroups; ++it) {
+        const unsigned long long act = __ballot(pending);
+        if (!act) break;
+        const int leader = __builtin_ctzll(act);
+        const unsigned long long lk = __shfl(key, leader);
+        const bool mine = pending && key == lk;
+        const unsigned long long grp = __ballot(mine);
+        int h = 0;
+        if (lane == leader) {
+            h = hash_insert(G, lk);
+            atomicAdd(&G.hcount[h], (unsigned long long)__popcll(grp));
+        }
+        h = __shfl(h, leader);
+        if (mine) {
+            slot = h;
+            pending = false;
+        }
     }
-    G.pslot[p] = (int)h;
-    atomicAdd(&G.hcount[h], 1ull);
+    if (pending) {
+        slot = hash_insert(G, key);
+        atomicAdd(&G.hcount[slot], 1ull);
+    }
+    if (p < n) G.pslot[p] = valid ? slot : -1;
 }
 
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long w = __shfl_xor(v, o);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long w = __shfl_xor(v, o);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+// Places every included point in its cell's segment of cellpts and grows the
+// cell's bounding box.
 __global__ void fill_kernel(const GridDev G, long long n)
 {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const int s = G.pslot[p];
-    if (s < 0) return;
-    const long long pos = G.hstart[s] + atomicAdd(&G.hcur[s], 1);
-    G.cellpts[pos] = (int)p;
+    const int s = p < n ? G.pslot[p] : -1;
+    const bool valid = s >= 0;
+    unsigned long long e[3] = {0, 0, 0};
+    if (valid)
+        for (int d = 0; d < G.dim; ++d) e[d] = ord_enc(G.x[d][p]);
+    const int lane = __lane_id();
+    bool pending = valid;
+    for (int it = 0; it < kAggGroups; ++it) {
+        const unsigned long long act = __ballot(pending);
+        if (!act) break;
+        const int leader = __builtin_ctzll(act);
+        const int ls = __shfl(s, leader);
+        const bool mine = pending && s == ls;
+        const unsigned long long grp = __ballot(mine);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&G.hcur[ls], __popcll(grp));
+        base = __shfl(base, leader);
+        if (mine) G.cellpts[G.hstart[ls] + base + __popcll(grp & lanes_below())] = (int)p;
+        for (int d = 0; d < G.dim; ++d) {
+            const unsigned long long mn = wave_min_u64(mine ? e[d] : ~0ull);
+            const unsigned long long mx = wave_max_u64(mine ? e[d] : 0ull);
+            if (lane == leader) {
+                atomicMin(&G.bbox[d * G.H + ls], mn);
+                atomicMax(&G.bbox[(3 + d) * G.H + ls], mx);
+            }
+        }
+        if (mine) pending = false;
+    }
+    if (pending) {
+        G.cellpts[G.hstart[s] + atomicAdd(&G.hcur[s], 1)] = (int)p;
+        for (int d = 0; d < G.dim; ++d) {
+            atomicMin(&G.bbox[d * G.H + s], e[d]);
+            atomicMax(&G.bbox[(3 + d) * G.H + s], e[d]);
+        }
+    }
 }
 
-__global__ void core_kernel(const GridDev G, long long n, int minPts)
+// Core test, dense part: a cell holding >= minPts points is all-core (its
+// diagonal is below eps).  Points of sparser cells go to a candidate list for
+// the wavefront-per-point test.
+__global__ void core_dense_kernel(const GridDev G, long long n, int minPts, int* cand, int* ncand)
 {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const int s = G.pslot[p];
-    if (s < 0) {
-        G.core[p] = 0;
-        G.parent[p] = -1;
-        return;
+    const int s = p < n ? G.pslot[p] : -1;
+    const bool dense = s >= 0 && (long long)G.hcount[s] >= minPts;
+    const bool sparse = s >= 0 && !dense;
+    if (p < n) {
+        G.core[p] = dense ? 1 : 0;
+        G.parent[p] = dense ? (int)p : -1;
     }
-    bool core = (long long)G.hcount[s] >= minPts;
-    if (!core) {
-        int cnt = 0;
-        for_neighbour_cells(G, p, [&](int slot, const long long*) {
-            const long long b = G.hstart[slot], e = b + (long long)G.hcount[slot];
-            for (long long k = b; k < e; ++k)
-                if (within(G, p, G.cellpts[k]) && ++cnt >= minPts) return false;
-            return true;
-        });
-        core = cnt >= minPts;
+    const int lane = __lane_id();
+    const unsigned long long sb = __ballot(sparse);
+    int base = 0;
+    if (lane == 0 && sb) base = atomicAdd(ncand, __popcll(sb));
+    base = __shfl(base, 0);
+    if (sparse) cand[base + __popcll(sb & lanes_below())] = (int)p;
+    // hrep = smallest core index of the cell.  Indices rise with the lane, so
+    // a group's leader holds the group's smallest index.
+    bool pending = dense;
+    for (int it = 0; it < kAggGroups; ++it) {
+        const unsigned long long act = __ballot(pending);
+        if (!act) break;
+        const int leader = __builtin_ctzll(act);
+        const int ls = __shfl(s, leader);
+        const bool mine = pending && s == ls;
+        if (lane == leader) atomicMin(&G.hrep[ls], (int)p);
+        if (mine) pending = false;
     }
-    G.core[p] = core ? 1 : 0;
-    G.parent[p] = core ? (int)p : -1;
-    if (core) atomicMin(&G.hrep[s], (int)p);
+    if (pending) atomicMin(&G.hrep[s], (int)p);
+}
+
+__device__ __forceinline__ long long wave_sum_ll(long long v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Neighbour cell o (0 <= o < 5^dim) of cell c0: coordinates, false if any is
+// negative (no such cell).
+__device__ __forceinline__ bool neighbour_cell(const GridDev& G, const long long* c0, int o, long long* c)
+{
+    c[0] = c0[0] + o % 5 - 2;
+    c[1] = c0[1] + (o / 5) % 5 - 2;
+    c[2] = G.dim == 3 ? c0[2] + o / 25 - 2 : 0;
+    return c[0] >= 0 && c[1] >= 0 && c[2] >= 0;
+}
+
+__device__ __forceinline__ void unpack_cell(int dim, unsigned long long key, long long* c)
+{
+    if (dim == 3) {
+        c[0] = (long long)(key & 0x1FFFFFull);
+        c[1] = (long long)((key >> 21) & 0x1FFFFFull);
+        c[2] = (long long)(key >> 42);
+    } else {
+        c[0] = (long long)(key & 0x7FFFFFFFull);
+        c[1] = (long long)(key >> 31);
+        c[2] = 0;
+    }
+}
+
+// Core test, sparse part: one wavefront per candidate point.  The lanes look
+// up the 5^dim neighbour cells, add whole cells whose box lies within eps and
+// scan the undecided ones 64 points at a time, stopping at minPts.
+__global__ void core_sparse_kernel(const GridDev G, const int* cand, const int* ncand, int minPts)
+{
+    const int lane = __lane_id();
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+    const int nc = *ncand;
+    const int noff = G.dim == 3 ? 125 : 25;
+    for (long long i = wave; i < nc; i += nw) {
+        const int p = cand[i];
+        long long c0[3] = {0, 0, 0};
+        cell_coords(G, p, c0);
+        int sl[2] = {-1, -1}, rl[2] = {0, 0};
+        for (int j = 0; j < 2; ++j) {
+            const int o = lane + 64 * j;
+            long long c[3];
+            if (o >= noff || !neighbour_cell(G, c0, o, c)) continue;
+            if (box_gap2(G, p, c) > G.prune2) continue;
+            const int slot = lookup(G, pack_cell(G.dim, c));
+            if (slot < 0) continue;
+            sl[j] = slot;
+            rl[j] = box_relation(G, p, slot);
+        }
+        long long whole = 0;
+        for (int j = 0; j < 2; ++j)
+            if (rl[j] == 2) whole += (long long)G.hcount[sl[j]];
+        long long cnt = wave_sum_ll(whole);
+        for (int j = 0; j < 2 && cnt < minPts; ++j) {
+            unsigned long long m = __ballot(rl[j] == 1);
+            while (m && cnt < minPts) {
+                const int ld = __builtin_ctzll(m);
+                m &= m - 1;
+                const int slot = __shfl(sl[j], ld);
+                const long long b = G.hstart[slot], e = b + (long long)G.hcount[slot];
+                for (long long k = b; k < e && cnt < minPts; k += 64) {
+                    const bool hit = k + lane < e && within(G, p, G.cellpts[k + lane]);
+                    cnt += __popcll(__ballot(hit));
+                }
+            }
+        }
+        if (cnt >= minPts && lane == 0) {
+            G.core[p] = 1;
+            G.parent[p] = p;
+            atomicMin(&G.hrep[G.pslot[p]], p);
+        }
+    }
 }
 
 __global__ void link_cell_kernel(const GridDev G, long long n)
@@ -241,26 +470,140 @@ __global__ void link_cell_kernel(const GridDev G, long long n)
     G.parent[p] = G.hrep[G.pslot[p]];  // the cell's smallest core index (all-pairs within eps)
 }
 
-__global__ void link_cross_kernel(const GridDev G, long long n)
+// Cells holding at least one core point, compacted.
+__global__ void cell_list_kernel(const GridDev G, int* cells, int* ncells)
 {
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n || !G.core[p]) return;
-    const unsigned long long own = G.hkey[G.pslot[p]];
-    for_neighbour_cells(G, p, [&](int slot, const long long*) {
-        if (G.hkey[slot] <= own) return true;          // each cell pair from one side
-        const int r2 = G.hrep[slot];
-        if (r2 == INT_MAX) return true;                // no core point there
-        if (uf_find(G, (int)p) == uf_find(G, r2)) return true;
-        const long long b = G.hstart[slot], e = b + (long long)G.hcount[slot];
-        for (long long k = b; k < e; ++k) {
-            const int q = G.cellpts[k];
-            if (G.core[q] && within(G, p, q)) {
-                uf_union(G, (int)p, q);
-                break;
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool has = s < G.H && G.hkey[s] != kEmpty && G.hrep[s] != INT_MAX;
+    const int lane = __lane_id();
+    const unsigned long long m = __ballot(has);
+    int base = 0;
+    if (lane == 0 && m) base = atomicAdd(ncells, __popcll(m));
+    base = __shfl(base, 0);
+    if (has) cells[base + __popcll(m & lanes_below())] = (int)s;
+}
+
+// May some point of cell a lie within eps of some point of cell b?  (Exact
+// boxes; same monotone-rounding argument as box_relation.)
+__device__ __forceinline__ bool boxes_may_touch(const GridDev& G, int a, int b)
+{
+    double g2 = 0.0;
+    for (int d = 0; d < G.dim; ++d) {
+        const double loA = ord_dec(G.bbox[d * G.H + a]), hiA = ord_dec(G.bbox[(3 + d) * G.H + a]);
+        const double loB = ord_dec(G.bbox[d * G.H + b]), hiB = ord_dec(G.bbox[(3 + d) * G.H + b]);
+        const double g = loB > hiA ? loB - hiA : (loA > hiB ? loA - hiB : 0.0);
+        g2 = g2 + g * g;
+    }
+    return sqrt(g2) <= G.eps;
+}
+
+// One witness pair (core p in cell [a0, a1), core q in cell b, within eps)
+// found by the whole wavefront -- 64 points of a against 64 of b per tile --
+// and united.  Returns at the first witness.
+__device__ void link_cell_pair(const GridDev& G, int a, long long a0, long long a1, int b)
+{
+    const int lane = __lane_id();
+    const long long b0 = G.hstart[b], b1 = b0 + (long long)G.hcount[b];
+    const int rb = G.hrep[b];
+    for (long long ka = a0; ka < a1; ka += 64) {
+        int p = -1, rel = 0;
+        double px[3] = {0.0, 0.0, 0.0};
+        if (ka + lane < a1) {
+            p = G.cellpts[ka + lane];
+            if (G.core[p]) rel = box_relation(G, p, b);
+        }
+        const unsigned long long m2 = __ballot(rel == 2);
+        if (m2) {                                  // all of b within eps of p, rb among them
+            if (lane == __builtin_ctzll(m2)) uf_union(G, p, rb);
+            return;
+        }
+        const bool cand = rel == 1;
+        if (!__ballot(cand)) continue;
+        if (cand)
+            for (int d = 0; d < G.dim; ++d) px[d] = G.x[d][p];
+        for (long long kb = b0; kb < b1; kb += 64) {
+            int q = -1;
+            bool qok = false;
+            double qx[3] = {0.0, 0.0, 0.0};
+            if (kb + lane < b1) {
+                q = G.cellpts[kb + lane];
+                qok = G.core[q] != 0 && box_relation(G, q, a) != 0;
+                if (qok)
+                    for (int d = 0; d < G.dim; ++d) qx[d] = G.x[d][q];
+            }
+            unsigned long long mq = __ballot(qok);
+            while (mq) {
+                const int l = __builtin_ctzll(mq);
+                mq &= mq - 1;
+                double d2 = 0.0;
+                for (int d = 0; d < G.dim; ++d) {
+                    const double df = px[d] - __shfl(qx[d], l);
+                    d2 = d2 + df * df;
+                }
+                const unsigned long long mh = __ballot(cand && sqrt(d2) <= G.eps);
+                if (mh) {
+                    const int qq = __shfl(q, l);
+                    if (lane == __builtin_ctzll(mh)) uf_union(G, p, qq);
+                    return;
+                }
             }
         }
-        return true;
-    });
+    }
+}
+
+// Ring of neighbour offset o: 0 = shares a face, 1 = rest of the 3^dim
+// block, 2 = outer shell of the 5^dim block.
+__device__ __forceinline__ int offset_ring(const GridDev& G, int o)
+{
+    const int dx = abs(o % 5 - 2), dy = abs((o / 5) % 5 - 2), dz = G.dim == 3 ? abs(o / 25 - 2) : 0;
+    const int cheb = max(dx, max(dy, dz));
+    if (cheb == 2) return 2;
+    return dx + dy + dz == 1 ? 0 : 1;
+}
+
+// Cross-cell links: one wavefront per core-holding cell a, over its
+// neighbour cells b of one ring with a larger key (each pair from one side),
+// skipping pairs already connected or whose boxes are farther apart than eps.
+// The rings run as three launches, nearest first: in a dense region the face
+// links connect the cells, and the farther pairs then pass the root check.
+__global__ void link_pairs_kernel(const GridDev G, const int* cells, const int* ncells, int ring)
+{
+    const int lane = __lane_id();
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+    const int nc = *ncells;
+    const int noff = G.dim == 3 ? 125 : 25;
+    for (long long i = wave; i < nc; i += nw) {
+        const int a = cells[i];
+        const unsigned long long ka = G.hkey[a];
+        const int ra = G.hrep[a];
+        long long c0[3];
+        unpack_cell(G.dim, ka, c0);
+        int sl[2] = {-1, -1};
+        for (int j = 0; j < 2; ++j) {
+            const int o = lane + 64 * j;
+            long long c[3];
+            if (o >= noff || offset_ring(G, o) != ring || !neighbour_cell(G, c0, o, c)) continue;
+            const unsigned long long kb = pack_cell(G.dim, c);
+            if (kb <= ka) continue;
+            const int slot = lookup(G, kb);
+            if (slot < 0 || G.hrep[slot] == INT_MAX || !boxes_may_touch(G, a, slot)) continue;
+            sl[j] = slot;
+        }
+        const long long a0 = G.hstart[a], a1 = a0 + (long long)G.hcount[a];
+        for (int j = 0; j < 2; ++j) {
+            unsigned long long m = __ballot(sl[j] >= 0);
+            while (m) {
+                const int ld = __builtin_ctzll(m);
+                m &= m - 1;
+                const int b = __shfl(sl[j], ld);
+                int same = 0;
+                if (lane == 0) same = uf_find(G, ra) == uf_find(G, G.hrep[b]);
+                if (__shfl(same, 0)) continue;
+                link_cell_pair(G, a, a0, a1, b);
+            }
+        }
+    }
 }
 
 __global__ void flatten_kernel(const GridDev G, long long n, int* root)
@@ -284,6 +627,12 @@ __global__ void border_kernel(const GridDev G, long long n, const int* root, int
     }
     int best = INT_MAX;
     for_neighbour_cells(G, p, [&](int slot, const long long*) {
+        const int rel = box_relation(G, p, slot);
+        if (rel == 0) return true;
+        if (rel == 2) {                                // smallest core index of the cell
+            best = min(best, G.hrep[slot]);
+            return true;
+        }
         const long long b = G.hstart[slot], e = b + (long long)G.hcount[slot];
         for (long long k = b; k < e; ++k) {
             const int q = G.cellpts[k];
@@ -392,6 +741,9 @@ int dbscan_device(CallCtx& ctx, const double* const* pts, const int* include, lo
     DevBuf<unsigned long long> hkey(H), hcount(H);
     DevBuf<long long> hstart(H);
     DevBuf<int> hcur(H), hrep(H), pslot(n), cellpts(n), core(n), parent(n), root(n), labroot(n);
+    DevBuf<unsigned long long> bbox(6 * H);
+    G.bbox = bbox.get();
+    G.H = (long long)H;
     G.hkey = hkey.get();
     G.hcount = hcount.get();
     G.hstart = hstart.get();
@@ -406,13 +758,22 @@ int dbscan_device(CallCtx& ctx, const double* const* pts, const int* include, lo
     hipLaunchKernelGGL(fill_u64_kernel, dim3(gh), dim3(T), 0, st, G.hcount, (long long)H, 0ull);
     hipLaunchKernelGGL(fill_int_kernel, dim3(gh), dim3(T), 0, st, G.hcur, (long long)H, 0);
     hipLaunchKernelGGL(fill_int_kernel, dim3(gh), dim3(T), 0, st, G.hrep, (long long)H, INT_MAX);
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(3 * gh), dim3(T), 0, st, G.bbox, 3 * (long long)H, ~0ull);
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(3 * gh), dim3(T), 0, st, G.bbox + 3 * H, 3 * (long long)H, 0ull);
     hipLaunchKernelGGL(insert_kernel, dim3(gn), dim3(T), 0, st, G, n);
     CHAOS_HIP(hipGetLastError());
     exclusive_scan(st, reinterpret_cast<const long long*>(G.hcount), G.hstart, (long long)H);
     hipLaunchKernelGGL(fill_kernel, dim3(gn), dim3(T), 0, st, G, n);
-    hipLaunchKernelGGL(core_kernel, dim3(gn), dim3(T), 0, st, G, n, prm.minPts);
+    DevBuf<int> cand(n), cells(n), counters(2);  // counters: candidates, cells
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, st, counters.get(), 2LL, 0);
+    hipLaunchKernelGGL(core_dense_kernel, dim3(gn), dim3(T), 0, st, G, n, prm.minPts, cand.get(), counters.get());
+    hipLaunchKernelGGL(core_sparse_kernel, dim3(kWaveBlocks), dim3(T), 0, st, G, cand.get(), counters.get(),
+                       prm.minPts);
     hipLaunchKernelGGL(link_cell_kernel, dim3(gn), dim3(T), 0, st, G, n);
-    hipLaunchKernelGGL(link_cross_kernel, dim3(gn), dim3(T), 0, st, G, n);
+    hipLaunchKernelGGL(cell_list_kernel, dim3(gh), dim3(T), 0, st, G, cells.get(), counters.get() + 1);
+    for (int ring = 0; ring < 3; ++ring)
+        hipLaunchKernelGGL(link_pairs_kernel, dim3(kWaveBlocks), dim3(T), 0, st, G, cells.get(), counters.get() + 1,
+                           ring);
     hipLaunchKernelGGL(flatten_kernel, dim3(gn), dim3(T), 0, st, G, n, root.get());
     hipLaunchKernelGGL(border_kernel, dim3(gn), dim3(T), 0, st, G, n, root.get(), labroot.get());
     CHAOS_HIP(hipGetLastError());
@@ -435,7 +796,7 @@ int dbscan_device(CallCtx& ctx, const double* const* pts, const int* include, lo
                        scanP.get(), scanN.get(), labels);
     CHAOS_HIP(hipGetLastError());
     CHAOS_HIP(hipStreamSynchronize(st));
-    ctx.add_launch(16);
+    ctx.add_launch(21);
     return (int)K;
 }
 
@@ -519,3 +880,53 @@ void zscore_device(CallCtx& ctx, double* const* f, int k, const int* mask, long 
 }
 
 }  // namespace chaos
+
+using namespace chaos;
+
+extern "C" int chaos_dbscan(const chaos_options* opt, const double* x, const double* y, const double* z, long long n,
+                            int dim, double eps, int min_pts, int sign_rule, const int* include, int* labels,
+                            int* clusters)
+{
+    return guarded("chaos_dbscan", [&] {
+        chaos_options o = resolve_options(opt);
+        CHAOS_REQUIRE(n >= 0 && n < INT_MAX, "n out of range");
+        CHAOS_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+        CHAOS_REQUIRE(sign_rule == 0 || dim == 2, "sign_rule needs dim 2");
+        CHAOS_REQUIRE(x && y && (dim == 2 || z), "coordinate array is NULL");
+        CHAOS_REQUIRE(labels, "labels is NULL");
+        CHAOS_REQUIRE(sign_rule || min_pts >= 1, "min_pts must be >= 1");
+        if (clusters) *clusters = 0;
+        if (n == 0) return;
+        CallCtx ctx(o);
+        hipStream_t st = ctx.stream();
+        const bool dev = o.device_buffers != 0;
+        DevBuf<double> d_pts;
+        DevBuf<int> d_inc, d_lab;
+        const double* pts[3] = {x, y, z};
+        const int* inc = include;
+        int* lab = labels;
+        if (!dev) {
+            d_pts.alloc((size_t)dim * n);
+            for (int d = 0; d < dim; ++d) {
+                CHAOS_HIP(hipMemcpyAsync(d_pts.get() + d * n, pts[d], n * 8, hipMemcpyHostToDevice, st));
+                pts[d] = d_pts.get() + d * n;
+            }
+            d_lab.alloc(n);
+            lab = d_lab.get();
+        }
+        if (!inc || !dev) {
+            d_inc.alloc(n);
+            if (inc) CHAOS_HIP(hipMemcpyAsync(d_inc.get(), inc, n * sizeof(int), hipMemcpyHostToDevice, st));
+            else hipLaunchKernelGGL(fill_int_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, d_inc.get(), n, 1);
+            inc = d_inc.get();
+        }
+        ctx.begin();
+        DbscanParams prm{dim, eps, sign_rule ? 1 : min_pts, sign_rule != 0};
+        const int K = dbscan_device(ctx, pts, inc, n, prm, lab);
+        ctx.add_analysis_ms(ctx.end());
+        if (!dev) CHAOS_HIP(hipMemcpyAsync(labels, lab, n * sizeof(int), hipMemcpyDeviceToHost, st));
+        ctx.sync();
+        if (clusters) *clusters = K;
+        ctx.finish();
+    });
+}

@@ -161,8 +161,7 @@ Engine::Engine(int device, const chaos_engine_params& prm) : prm_(prm), device_(
     const hipDeviceProp_t& props = device_props(device);
     const int waves = props.multiProcessorCount * 4 * prm_.waves_per_simd;
     grid_ = std::max(1, waves / (prm_.block / 64));
-    size_t free_b = 0, total_b = 0;
-    CHAOS_HIP(hipMemGetInfo(&free_b, &total_b));
+    const size_t free_b = dev_mem_available();
     hbm_budget_ = (long long)(prm_.hbm_fraction * (double)free_b);
     t0_ = steady_ms();
 }

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -38,6 +39,110 @@ void init_defaults_locked()
 }  // namespace
 
 void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+// ---------------------------------------------------------------------------
+// caching device allocator
+// ---------------------------------------------------------------------------
+
+namespace {
+struct DevCache {
+    std::map<size_t, std::vector<void*>> free_blocks;  // size class -> blocks
+    size_t cached_bytes = 0;
+};
+std::mutex g_alloc_mu;
+std::map<int, DevCache> g_cache;                         // device -> cache
+std::map<void*, std::pair<int, size_t>> g_live;          // block -> (device, class)
+constexpr size_t kCacheCap = 16ull << 30;                // bytes kept per device
+
+size_t size_class(size_t bytes)
+{
+    if (bytes <= 512) return 512;
+    if (bytes <= (64ull << 20)) {
+        size_t c = 512;
+        while (c < bytes) c <<= 1;
+        return c;
+    }
+    const size_t g = 64ull << 20;
+    return (bytes + g - 1) / g * g;
+}
+
+void trim_locked(int device)
+{
+    DevCache& c = g_cache[device];
+    for (auto& kv : c.free_blocks)
+        for (void* p : kv.second) (void)hipFree(p);
+    c.free_blocks.clear();
+    c.cached_bytes = 0;
+}
+}  // namespace
+
+void* dev_alloc(size_t bytes)
+{
+    int dev = 0;
+    CHAOS_HIP(hipGetDevice(&dev));
+    const size_t cls = size_class(bytes);
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    DevCache& c = g_cache[dev];
+    auto it = c.free_blocks.find(cls);
+    if (it != c.free_blocks.end() && !it->second.empty()) {
+        void* p = it->second.back();
+        it->second.pop_back();
+        c.cached_bytes -= cls;
+        g_live[p] = {dev, cls};
+        return p;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, cls);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        trim_locked(dev);
+        e = hipMalloc(&p, cls);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail(CHAOS_E_NOMEM, "hipMalloc(%zu bytes) failed: %s", cls, hipGetErrorString(e));
+    }
+    g_live[p] = {dev, cls};
+    return p;
+}
+
+void dev_free(void* p)
+{
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    auto it = g_live.find(p);
+    if (it == g_live.end()) {
+        (void)hipFree(p);
+        return;
+    }
+    const int dev = it->second.first;
+    const size_t cls = it->second.second;
+    g_live.erase(it);
+    DevCache& c = g_cache[dev];
+    if (c.cached_bytes + cls > kCacheCap) {
+        (void)hipFree(p);
+        return;
+    }
+    c.free_blocks[cls].push_back(p);
+    c.cached_bytes += cls;
+}
+
+size_t dev_mem_available()
+{
+    int dev = 0;
+    CHAOS_HIP(hipGetDevice(&dev));
+    size_t free_b = 0, total_b = 0;
+    CHAOS_HIP(hipMemGetInfo(&free_b, &total_b));
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    auto it = g_cache.find(dev);
+    return free_b + (it == g_cache.end() ? 0 : it->second.cached_bytes);
+}
+
+void dev_cache_trim()
+{
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    for (auto& kv : g_cache) trim_locked(kv.first);
+}
 
 [[noreturn]] void fail(int code, const char* fmt, ...)
 {

@@ -57,7 +57,20 @@ int guarded(const char* fn, F&& body) noexcept
 // Options with defaults filled in (opt == NULL -> process defaults).
 chaos_options resolve_options(const chaos_options* opt);
 
-// Device buffer owned by one call (hipMalloc / hipFree on the call device).
+// Caching device allocator on the current device: freed blocks are kept in
+// size classes and handed out again instead of going back to hipFree (which
+// synchronises the device) -- per-call and per-job scratch is allocated and
+// freed thousands of times per second by the job server.  A block must only
+// be freed once the work using it has completed (every caller synchronises
+// its stream first).  On allocation failure the cache is emptied and the
+// allocation retried.
+void* dev_alloc(size_t bytes);
+void dev_free(void* p);
+void dev_cache_trim();
+// Free HBM on the current device plus the bytes parked in the cache.
+size_t dev_mem_available();
+
+// Device buffer owned by one call or job (cached hipMalloc on the current device).
 template <class T>
 class DevBuf {
 public:
@@ -80,16 +93,12 @@ public:
     {
         release();
         if (n == 0) return;
-        void* p = nullptr;
-        hipError_t e = hipMalloc(&p, n * sizeof(T));
-        if (e != hipSuccess)
-            fail(CHAOS_E_NOMEM, "hipMalloc(%zu bytes) failed: %s", n * sizeof(T), hipGetErrorString(e));
-        p_ = static_cast<T*>(p);
+        p_ = static_cast<T*>(dev_alloc(n * sizeof(T)));
         n_ = n;
     }
     void release()
     {
-        if (p_) (void)hipFree(p_);
+        if (p_) dev_free(p_);
         p_ = nullptr;
         n_ = 0;
     }

@@ -256,6 +256,7 @@ void stream_run(int device, const chaos_stream_config& c, const chaos_engine_par
         const chaos_server_stats st = eng.stats();
         rep.busy_ms = st.busy_ms;
         rep.launches = st.launches;
+        rep.finalize_ms = st.finalize_ms;
     } else {
         t_start = clock_ms();
         for (size_t i = 0; i < N; ++i) {

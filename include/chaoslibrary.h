@@ -211,6 +211,23 @@ int chaos_basinsOfAttraction_2(const chaos_options* opt, double tMax, int nPts,
                                int preScaller, double eps,
                                const char* OUT_FILE_PATH);
 
+/* The clustering stage of Basins on its own, over caller points.
+ * Replaces CUDA_dbscan (Basins/src/basinsCUDA.cu:866-923) when sign_rule is
+ * set: dim 2, min_pts 1, include[i] == -1 marks a fixed point, and clusters
+ * holding one are numbered -1..-K by their smallest fixed-point index, the
+ * others 1..K by their smallest index.  With sign_rule clear it is the
+ * z-scored 3-D DBSCAN of CONFIG basins: dim 2 or 3, core = at least min_pts
+ * points (itself included) within sqrt(sum d^2) <= eps; clusters numbered
+ * 1..K by smallest core index; a border point takes the cluster of its
+ * smallest-index core neighbour; noise and points with include[i] == 0 get 0.
+ * coords: dim arrays of n doubles (x, y[, z]); include may be NULL (all in).
+ * With opt->device_buffers the coordinates, include and labels are device
+ * pointers.  *clusters (may be NULL) receives K of the positive clusters. */
+int chaos_dbscan(const chaos_options* opt, const double* x, const double* y,
+                 const double* z, long long n, int dim, double eps,
+                 int min_pts, int sign_rule, const int* include, int* labels,
+                 int* clusters);
+
 /* ---------------- Batched engine and job server ------------------------- */
 /* No counterpart in the reference (one synchronous call per sweep, one
  * system per build).  Jobs of any of the four kinds run together in one
@@ -316,8 +333,9 @@ typedef struct chaos_stream_report {
     double makespan_s, rk4_steps, steps_per_s, jobs_per_s;
     double p50_ms, p99_ms, mean_ms, max_ms;
     double p99_kind_ms[3];
-    double busy_ms;
+    double busy_ms;         /* engine mode: device time of engine launches    */
     long long launches;
+    double finalize_ms;     /* engine mode: host wall time finishing jobs     */
 } chaos_stream_report;
 
 int chaos_stream_config_default(chaos_stream_config* cfg);

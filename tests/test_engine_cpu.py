@@ -38,5 +38,5 @@ def test_job_struct_layout(chaos):
     j = chaos.make_job(chaos.JOB_LLE2D, "rossler_rk4", 1, 8, [0.1] * 3, [0.2, 0.2, 5.7], [0, 1, 2, 3], [0, 2])
     # sizes and offsets of include/chaoslibrary.h structs on x86-64 (gcc)
     assert C.sizeof(j) == 232 and chaos.Job.seed.offset == 80 and chaos.Job.ranges.offset == 200
-    assert C.sizeof(chaos.StreamConfig) == 64 and C.sizeof(chaos.StreamReport) == 120
+    assert C.sizeof(chaos.StreamConfig) == 64 and C.sizeof(chaos.StreamReport) == 128
     assert j.amountOfInitialConditions == 3 and j.amountOfValues == 3 and list(j.indicesOfMutVars) == [0, 2]
