@@ -27,6 +27,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -61,10 +62,29 @@ def oracle():
     return _oracle
 
 
-def roofline(system, steps, kernel_s, name, kernel_ms):
+def profile_figure(summary, key):
+    """A figure recorded in a rocprofv3 summary under profiles/ (a line
+    `key: value`), or None when the summary or the line is absent."""
+    path = os.path.join(ROOT, "profiles", summary + ".md")
+    try:
+        with open(path) as f:
+            for line in f:
+                m = re.match(r"\s*" + re.escape(key) + r"\s*:\s*([-+0-9.eE]+)\s*$", line)
+                if m:
+                    return float(m.group(1))
+    except OSError:
+        pass
+    return None
+
+
+def roofline(system, steps, kernel_s, name, kernel_ms, summary=None):
+    """FP64 VALU roofline of the dominant kernel.  `traffic` is the kernel's
+    HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE passes recorded in
+    profiles/<summary>.md (scaled by the copy-kernel factors noted there)."""
     achieved = steps * FLOP_PER_STEP[system] / kernel_s / 1e12
+    traffic = profile_figure(summary, "traffic_bytes_per_launch") if summary else None
     return {"bound": "valu_fp64", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None, "kernel": name,
+            "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic, "kernel": name,
             "kernel_ms": round(kernel_ms, 3)}
 
 
@@ -125,7 +145,8 @@ def run_lle(args):
     res["config"] = {"workload": f"lle2d_rossler_rk4_{n}x{n}", "grid": n, "h": w["h"],
                      "transient_time": w["transient"], "tMax": w["tMax"], "NT": w["NT"], "eps": w["eps"],
                      "seed": w["seed"], "rk4_steps_per_map": per_map, "flop_per_step": 63}
-    res["roofline"] = roofline(w["system"], per_map, kms / 1e3, "lle_map_kernel<5>", kms)
+    res["roofline"] = roofline(w["system"], per_map, kms / 1e3, "lle_map_kernel<5>", kms,
+                               f"lle_map_kernel_rossler_{n}" if n == 2048 else None)
     if not args.no_cpu:
         log("cpu baseline ...")
         O = oracle()
@@ -210,10 +231,12 @@ def run_bif2d(args):
     res["data"] = "synthetic: rho = linspace(20,200), beta = linspace(0.5,4), sigma = 10"
     res["config"] = {"workload": f"bif2d_lorenz_rk4_{n}x{n}", "grid": n, "transient_time": 500.0, "tMax": 500.0,
                      "h": 0.01, "dbscan_eps": 1e-2, "peaks": stats.peaks}
-    res["roofline"] = roofline("lorenz_rk4", steps / args.steps, kms / 1e3, "sweep_kernel<6,0,0>", kms)
+    summary = f"sweep_kernel_bif2d_lorenz_{n}" if n == 1024 else None
+    res["roofline"] = roofline("lorenz_rk4", steps / args.steps, kms / 1e3, "sweep_kernel<6,0,0>", kms, summary)
     res["analysis_ms"] = ams
-    peak_bytes = stats.peaks * (16 + 16 + 12)  # store + scatter read + segment write
-    res["analysis_hbm_gbs"] = peak_bytes / (ams / 1e3) / 1e9 if ams > 0 else None
+    # peak output and 1-D DBSCAN stages: measured HBM bytes over their live time
+    ab = profile_figure(summary, "analysis_bytes_per_run") if summary else None
+    res["analysis_hbm_gbs"] = ab / (ams / 1e3) / 1e9 if ab and ams > 0 else None
     return res
 
 
@@ -239,10 +262,12 @@ def run_basins(args):
     res["data"] = "synthetic: IC grid x0, y0 in [-20,20], z0 = 25; rho = 24.5"
     res["config"] = {"workload": f"basins_lorenz_rk4_{n}x{n}", "grid": n, "transient_time": 300.0, "tMax": 300.0,
                      "h": 0.01, "eps": 0.5, "min_pts": 20}
-    res["roofline"] = roofline("lorenz_rk4", steps / args.steps, kms / 1e3, "sweep_kernel<6,0,2>", kms)
+    summary = f"sweep_kernel_basins_lorenz_{n}" if n == 2048 else None
+    res["roofline"] = roofline("lorenz_rk4", steps / args.steps, kms / 1e3, "sweep_kernel<6,0,2>", kms, summary)
     res["analysis_ms"] = ams
-    feat_bytes = n * n * (24 + 4) * 4  # features read by ~4 passes + labels
-    res["dbscan_hbm_gbs"] = feat_bytes / (ams / 1e3) / 1e9 if ams > 0 else None
+    # z-score + 3-D DBSCAN stages: measured HBM bytes over their live time
+    ab = profile_figure(summary, "analysis_bytes_per_run") if summary else None
+    res["dbscan_hbm_gbs"] = ab / (ams / 1e3) / 1e9 if ab and ams > 0 else None
     return res
 
 

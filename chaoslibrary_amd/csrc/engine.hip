@@ -65,7 +65,11 @@ __device__ __forceinline__ void sweep_tile(const JobDev& J, const Tile& T, int l
     } else {
         L.params(A, qq);
         L.load(J.st, q);
-        if (KIND == KIND_BIF) cur = J.cursors[T.group];
+        if (KIND == KIND_BIF) {
+            const WaveCursor c = J.cursors[T.group];
+            cur.chunk = __builtin_amdgcn_readfirstlane(c.chunk);
+            cur.fill = __builtin_amdgcn_readfirstlane(c.fill);
+        }
     }
     L.template ticks<-1>(A, J.wv, cur, T.group, T.g0, T.g1);
     if (T.g1 == J.total) {

@@ -227,12 +227,72 @@ struct SweepProg {
         s0 = s1 = sx = sz = first = last = 0.0;
     }
 
+    // Divergence test of the recorded variable.  The band test |v| < L is
+    // exact-safe (no lane inside it can fail the exact test); the exact test
+    // runs only when some lane of the wave is outside it, behind a uniform
+    // branch.  Callers keep the wave's control flow uniform.
     template <int WVT>
     __device__ __forceinline__ void check(const SweepArgs& A, int wv)
     {
         const double v = pick<NX, WVT>(X, wv);
-        if (alive && !(fabs(v) < A.L)) {
-            if (diverged_exact(v, A.maxValue, A.check)) alive = false;
+        const bool out = alive && !(fabs(v) < A.L);
+        if (__ballot(out)) {
+            if (out && diverged_exact(v, A.maxValue, A.check)) alive = false;
+        }
+    }
+
+    // One recorded sample t: peak detection on the sample, the kind's
+    // per-peak work, then preScaller steps and the divergence check.  BODY:
+    // 2 <= t < B - 1 (detector without the warm-up test, no first/last
+    // sample); PRE1: preScaller == 1.
+    template <int WVT, bool BODY, bool PRE1>
+    __device__ __forceinline__ void sample(const SweepArgs& A, int wv, WaveCursor& cur, int wave, int t)
+    {
+        const double v = pick<NX, WVT>(X, wv);
+        if (!BODY && t == 0) first = v;
+        if (KIND == KIND_BASINS_CFG) {
+            sx = sx + X[0];
+            sz = sz + X[NX - 1];
+        }
+        double pv = 0.0;
+        int pt = 0;
+        const bool e = (BODY ? det.feed_body(t - 1, v, pv, pt) : det.feed(t, v, pv, pt)) && alive;
+        if (KIND == KIND_BIF) {
+            append_peaks(A.pool, cur, wave, e, pv, pt, cnt);
+            if (e) ++cnt;
+        } else if (KIND == KIND_BASINS_REF) {
+            // running sums of the kept peaks and intervals
+            // (avgPeakFinderCUDA, basinsCUDA.cu:841-861)
+            if (e) {
+                if (cnt >= 1) {
+                    s0 = s0 + pv;
+                    s1 = s1 + (double)(pt - prev_t) * A.hp;
+                }
+                prev_t = pt;
+                ++cnt;
+            }
+        } else {
+            if (e) {
+                if (cnt >= 1) {
+                    s0 = s0 + pv;
+                    s1 = fma(pv, pv, s1);
+                }
+                ++cnt;
+            }
+        }
+        if (!BODY && t == A.B - 1) {
+            last = v;
+            double xp[NX];
+#pragma unroll
+            for (int k = 0; k < NX; ++k) xp[k] = X[k];
+            for (int r = 0; r < A.pre; ++r) Sy::step(X, a, A.sc);
+            check<WVT>(A, wv);
+            fixed = alive && fixed_point(step_delta2<NX>(X, xp), A.check);
+        } else {
+            if (PRE1) Sy::step(X, a, A.sc);
+            else
+                for (int r = 0; r < A.pre; ++r) Sy::step(X, a, A.sc);
+            check<WVT>(A, wv);
         }
     }
 
@@ -269,54 +329,18 @@ struct SweepProg {
             g = e;
         }
         // ---- record ----
+        // head (t < 2: detector warm-up, first sample), body, tail (t = B-1)
         const int B = A.B;
-        const int tb = (int)(g - A.S), te = (int)(g1 - A.S);
-        for (int t = tb; t < te; ++t) {
-            const double v = pick<NX, WVT>(X, wv);
-            if (t == 0) first = v;
-            if (KIND == KIND_BASINS_CFG) {
-                sx = sx + X[0];
-                sz = sz + X[NX - 1];
-            }
-            double pv = 0.0;
-            int pt = 0;
-            const bool e = det.feed(t, v, pv, pt) && alive;
-            if (KIND == KIND_BIF) {
-                append_peaks(A.pool, cur, wave, e, pv, pt, cnt);
-                if (e) ++cnt;
-            } else if (KIND == KIND_BASINS_REF) {
-                // running sums of the kept peaks and intervals
-                // (avgPeakFinderCUDA, basinsCUDA.cu:841-861)
-                if (e) {
-                    if (cnt >= 1) {
-                        s0 = s0 + pv;
-                        s1 = s1 + (double)(pt - prev_t) * A.hp;
-                    }
-                    prev_t = pt;
-                    ++cnt;
-                }
-            } else {
-                if (e) {
-                    if (cnt >= 1) {
-                        s0 = s0 + pv;
-                        s1 = fma(pv, pv, s1);
-                    }
-                    ++cnt;
-                }
-            }
-            if (t == B - 1) {
-                last = v;
-                double xp[NX];
-#pragma unroll
-                for (int k = 0; k < NX; ++k) xp[k] = X[k];
-                for (int r = 0; r < A.pre; ++r) Sy::step(X, a, A.sc);
-                check<WVT>(A, wv);
-                fixed = alive && fixed_point(step_delta2<NX>(X, xp), A.check);
-            } else {
-                for (int r = 0; r < A.pre; ++r) Sy::step(X, a, A.sc);
-                check<WVT>(A, wv);
-            }
+        int t = (int)(g - A.S);
+        const int te = (int)(g1 - A.S);
+        const int body_end = te < B - 1 ? te : B - 1;
+        for (; t < te && t < 2; ++t) sample<WVT, false, false>(A, wv, cur, wave, t);
+        if (A.pre == 1) {
+            for (; t < body_end; ++t) sample<WVT, true, true>(A, wv, cur, wave, t);
+        } else {
+            for (; t < body_end; ++t) sample<WVT, true, false>(A, wv, cur, wave, t);
         }
+        for (; t < te; ++t) sample<WVT, false, false>(A, wv, cur, wave, t);
     }
 
     __device__ __forceinline__ void finish(const SweepArgs& A, long long q) const

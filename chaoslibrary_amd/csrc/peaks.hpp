@@ -75,6 +75,32 @@ struct PeakDetector {
         p = cur;
         return emit;
     }
+
+    // feed() for t >= 2 (c = t - 1) without branches: the state machine as
+    // lane predicates, so the hot record loop carries no divergent control
+    // flow.  Same transitions as feed():
+    //   plateau scan or fresh candidate ("act") emits iff d[c]-d[c+1] > 1e-13;
+    //   it stays / becomes a plateau iff the step is flat within 1e-13;
+    //   skip follows an emit or a rejected plateau.
+    __device__ __forceinline__ bool feed_body(int c, double cur, double& val, int& time)
+    {
+        const double dnx = cur - p;
+        const bool c1 = dn > 1e-13;
+        const bool c2 = dnx <= 0.0;
+        const bool c3 = dnx < -1e-13;
+        const bool cand = !plat && !skip && c1 && c2;
+        const bool act = plat || cand;
+        const bool emit = act && c3;
+        const bool flat = c2 && !c3;
+        val = p;
+        time = plat ? (pstart + c) >> 1 : c;
+        pstart = cand && flat ? c : pstart;
+        skip = (plat && !flat) || (cand && c3);
+        plat = act && flat;
+        dn = dnx;
+        p = cur;
+        return emit;
+    }
 };
 
 // One stored peak: value, sample index, and (sequence number << 6 | lane).
@@ -130,7 +156,8 @@ __device__ __forceinline__ void append_peaks(const PeakPool& pool, WaveCursor& c
         if (cur.chunk < pool.nchunks && leader) pool.fill[cur.chunk] = cur.fill;
         unsigned nc = 0;
         if (leader) nc = atomicAdd(pool.head, 1u);
-        nc = __shfl(nc, first_lane(act));
+        // the leader is the first active lane: the cursor stays wave-uniform (SGPRs)
+        nc = __builtin_amdgcn_readfirstlane(nc);
         if (nc < pool.nchunks) {
             if (leader) pool.owner[nc] = wave;
         } else {
