@@ -240,12 +240,17 @@ def bif1d(tMax, nPts, h, initialConditions, ranges, indicesOfMutVars, writableVa
     total = C.c_longlong(0)
     args = (C.byref(o), tMax, nPts, h, ic.size, _dp(ic), _dp(rg), _ip(ix), writableVar, maxValue,
             transientTime, _dp(vals), vals.size, preScaller, _ip(counts), _dp(first))
-    rc = lib.chaos_bif1d(*args, None, None, 0, C.byref(total))
-    if rc not in (0, -5):
-        _check(rc)
-    cap = max(1, total.value)
+    # one sweep when the peak buffers are large enough (one peak per 32
+    # samples); a second, exactly sized one only when they are not
+    B = max(1, int(round(tMax / h / max(1, preScaller))))
+    cap = max(1, nPts * min(B // 2 + 1, max(256, B // 32)))
     pk, iv = np.zeros(cap), np.zeros(cap)
-    _check(lib.chaos_bif1d(*args, _dp(pk), _dp(iv), cap, C.byref(total)))
+    rc = lib.chaos_bif1d(*args, _dp(pk), _dp(iv), cap, C.byref(total))
+    if rc == -5:
+        cap = max(1, total.value)
+        pk, iv = np.zeros(cap), np.zeros(cap)
+        rc = lib.chaos_bif1d(*args, _dp(pk), _dp(iv), cap, C.byref(total))
+    _check(rc)
     return counts, first, pk[:total.value], iv[:total.value]
 
 
@@ -359,14 +364,14 @@ class ServerStats(C.Structure):
     _fields_ = [("launches", C.c_longlong), ("graph_replays", C.c_longlong), ("tiles", C.c_longlong),
                 ("busy_ms", C.c_double), ("finalize_ms", C.c_double), ("rk4_steps", C.c_double),
                 ("active_jobs", C.c_int), ("pending_jobs", C.c_int), ("hbm_in_use", C.c_longlong),
-                ("hbm_budget", C.c_longlong)]
+                ("hbm_budget", C.c_longlong), ("finishing_jobs", C.c_int), ("reserved", C.c_int)]
 
 
 class StreamConfig(C.Structure):
     _fields_ = [("lambda_", C.c_double), ("duration_s", C.c_double), ("horizon_scale", C.c_double),
                 ("seed", C.c_uint64), ("bif_min", C.c_int), ("bif_max", C.c_int), ("lle_min", C.c_int),
                 ("lle_max", C.c_int), ("basins_min", C.c_int), ("basins_max", C.c_int), ("mode", C.c_int),
-                ("max_jobs", C.c_int)]
+                ("max_jobs", C.c_int), ("progress_s", C.c_double)]
 
 
 class StreamReport(C.Structure):

@@ -167,11 +167,76 @@ Engine::Engine(int device, const chaos_engine_params& prm) : prm_(prm), device_(
     grid_ = std::max(1, waves / (prm_.block / 64));
     const size_t free_b = dev_mem_available();
     hbm_budget_ = (long long)(prm_.hbm_fraction * (double)free_b);
+    CHAOS_HIP(hipStreamCreateWithFlags(&fstream_, hipStreamNonBlocking));  // default priority
     t0_ = steady_ms();
+    worker_ = std::thread([this] { finisher(); });
+}
+
+// Worker: finish jobs in completion order on fstream_.  A failure marks the
+// job failed with its message; the engine keeps serving the others.
+void Engine::finisher()
+{
+    (void)hipSetDevice(device_);
+    std::unique_ptr<CallCtx> fctx;
+    for (;;) {
+        JobRec* r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            cv_.wait(lk, [&] { return stop_ || !finq_.empty(); });
+            if (finq_.empty()) return;
+            r = finq_.front();
+            finq_.pop_front();
+        }
+        const double t0 = steady_ms();
+        int st = 2;
+        std::string err;
+        try {
+            if (!fctx) {
+                chaos_options o = resolve_options(nullptr);
+                o.device = device_;
+                o.stream = fstream_;
+                fctx.reset(new CallCtx(o));
+            }
+            finalize(*r, *fctx);
+        } catch (const std::exception& e) {
+            st = -1;
+            err = e.what();
+            (void)hipStreamSynchronize(fstream_);
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            finalize_ms_ += steady_ms() - t0;
+            hbm_used_ -= r->footprint;
+            if (st == 2) steps_done_ += r->rk4_steps;
+            r->error = err;
+            r->t_done = now_ms();
+            r->status = st;
+            --finishing_;
+            ++completed_;
+        }
+        cv_.notify_all();
+    }
+}
+
+int Engine::take_completed() { return completed_.exchange(0); }
+
+void Engine::wait_finisher_progress()
+{
+    std::unique_lock<std::mutex> lk(mu_);
+    const int before = finishing_.load();
+    if (before == 0) return;
+    cv_.wait(lk, [&] { return finishing_.load() < before; });
 }
 
 Engine::~Engine()
 {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
+    }
+    cv_.notify_all();
+    if (worker_.joinable()) worker_.join();
+    if (fstream_) (void)hipStreamDestroy(fstream_);
     jobs_.clear();
     if (exec_) (void)hipGraphExecDestroy(exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
@@ -332,11 +397,18 @@ void Engine::admit()
 {
     while (!pending_.empty() && !free_slots_.empty()) {
         JobRec& r = *jobs_[pending_.front()];
-        if (hbm_used_ + r.footprint > hbm_budget_) break;
-        if (active_groups_ + r.groups > prm_.max_tiles) break;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            if (hbm_used_ + r.footprint > hbm_budget_) break;
+            hbm_used_ += r.footprint;
+        }
+        if (active_groups_ + r.groups > prm_.max_tiles) {
+            std::lock_guard<std::mutex> lk(mu_);
+            hbm_used_ -= r.footprint;
+            break;
+        }
         r.slot = free_slots_.back();
         free_slots_.pop_back();
-        hbm_used_ += r.footprint;
         active_groups_ += r.groups;
         r.status = 1;
         r.t_admit = now_ms();
@@ -367,20 +439,20 @@ void Engine::enqueue_launch()
     CHAOS_HIP(hipGetLastError());
 }
 
-void Engine::finalize(JobRec& r)
+void Engine::finalize(JobRec& r, CallCtx& ctx)
 {
-    const double t0 = steady_ms();
+    hipStream_t st = ctx.stream();
     const chaos_job& j = r.spec;
     if (r.prog == PROG_LLE) {
         r.r_lle.resize(r.n);
-        CHAOS_HIP(hipMemcpyAsync(r.r_lle.data(), r.out.get(), r.n * 8, hipMemcpyDeviceToHost, stream_));
-        CHAOS_HIP(hipStreamSynchronize(stream_));
+        CHAOS_HIP(hipMemcpyAsync(r.r_lle.data(), r.out.get(), r.n * 8, hipMemcpyDeviceToHost, st));
+        CHAOS_HIP(hipStreamSynchronize(st));
     } else if (r.prog == PROG_BIF) {
-        bool ok = bif_post(*ctx_, r.bif, r.pool);
+        bool ok = bif_post(ctx, r.bif, r.pool);
         if (!ok) {
             // peak store overflow: rerun this job alone with the exact pool
             r.reran = 1;
-            chaos_options o = job_options(j, device_, stream_);
+            chaos_options o = job_options(j, device_, st);
             if (j.kind == CHAOS_JOB_BIF1D) {
                 r.r_counts.resize(r.n);
                 r.r_first.resize(r.n);
@@ -412,30 +484,30 @@ void Engine::finalize(JobRec& r)
             r.r_peaks.resize(std::max(1LL, r.bif.total));
             r.r_intervals.resize(std::max(1LL, r.bif.total));
             long long total = 0;
-            bif1d_collect(*ctx_, r.bif, r.r_counts.data(), r.r_first.data(), r.r_peaks.data(), r.r_intervals.data(),
+            bif1d_collect(ctx, r.bif, r.r_counts.data(), r.r_first.data(), r.r_peaks.data(), r.r_intervals.data(),
                           r.bif.total, &total);
             r.r_peaks.resize(total);
             r.r_intervals.resize(total);
         } else {
             DevBuf<int> map((size_t)r.n);
-            bif2d_analysis(*ctx_, j.semantics, r.bif, j.eps, map.get());
+            bif2d_analysis(ctx, j.semantics, r.bif, j.eps, map.get());
             r.r_map.resize(r.n);
-            CHAOS_HIP(hipMemcpyAsync(r.r_map.data(), map.get(), r.n * 4, hipMemcpyDeviceToHost, stream_));
-            CHAOS_HIP(hipStreamSynchronize(stream_));
+            CHAOS_HIP(hipMemcpyAsync(r.r_map.data(), map.get(), r.n * 4, hipMemcpyDeviceToHost, st));
+            CHAOS_HIP(hipStreamSynchronize(st));
         }
     } else {
         const bool ref = r.prog == PROG_BASINS_REF;
         DevBuf<int> labels((size_t)r.n);
-        basins_cluster(*ctx_, ref, r.n, r.feats.get(), r.feats.get() + r.n, r.feats.get() + 2 * r.n, r.flags.get(),
+        basins_cluster(ctx, ref, r.n, r.feats.get(), r.feats.get() + r.n, r.feats.get() + 2 * r.n, r.flags.get(),
                        j.eps, j.min_pts > 0 ? j.min_pts : 20, labels.get());
         const int nf = ref ? 2 : 3;
         r.r_labels.resize(r.n);
         r.r_flags.resize(r.n);
         r.r_feats.resize(nf * r.n);
-        CHAOS_HIP(hipMemcpyAsync(r.r_labels.data(), labels.get(), r.n * 4, hipMemcpyDeviceToHost, stream_));
-        CHAOS_HIP(hipMemcpyAsync(r.r_flags.data(), r.flags.get(), r.n * 4, hipMemcpyDeviceToHost, stream_));
-        CHAOS_HIP(hipMemcpyAsync(r.r_feats.data(), r.feats.get(), nf * r.n * 8, hipMemcpyDeviceToHost, stream_));
-        CHAOS_HIP(hipStreamSynchronize(stream_));
+        CHAOS_HIP(hipMemcpyAsync(r.r_labels.data(), labels.get(), r.n * 4, hipMemcpyDeviceToHost, st));
+        CHAOS_HIP(hipMemcpyAsync(r.r_flags.data(), r.flags.get(), r.n * 4, hipMemcpyDeviceToHost, st));
+        CHAOS_HIP(hipMemcpyAsync(r.r_feats.data(), r.feats.get(), nf * r.n * 8, hipMemcpyDeviceToHost, st));
+        CHAOS_HIP(hipStreamSynchronize(st));
     }
     // release the device working set
     r.dstate.release();
@@ -446,7 +518,6 @@ void Engine::finalize(JobRec& r)
     r.feats.release();
     r.bif = BifDevice();
     r.pool = PeakPoolBuf();
-    finalize_ms_ += steady_ms() - t0;
 }
 
 int Engine::step()
@@ -458,7 +529,12 @@ int Engine::step()
         const long long g0 = r.done, g1 = std::min(r.total, r.done + r.ticks_per_launch);
         for (long long g = 0; g < r.groups; ++g) h_tiles_[nt++] = Tile{r.slot, (int)g, g0, g1};
     }
-    if (nt == 0) return 0;
+    if (nt == 0) {
+        // nothing to integrate: wait for a job being finished (it may free
+        // the HBM a pending job needs)
+        wait_finisher_progress();
+        return take_completed();
+    }
     *h_ntiles_ = nt;
     CHAOS_HIP(hipEventRecord(ev0_, stream_));
     if (prm_.use_graph) {
@@ -475,9 +551,9 @@ int Engine::step()
     busy_ms_ += ms;
     ++launches_;
     tiles_ += nt;
-    int completed = 0;
     std::vector<long long> still;
     still.reserve(active_.size());
+    std::vector<JobRec*> done;
     for (long long id : active_) {
         JobRec& r = *jobs_[id];
         r.done = std::min(r.total, r.done + r.ticks_per_launch);
@@ -485,18 +561,24 @@ int Engine::step()
             still.push_back(id);
             continue;
         }
-        finalize(r);
-        r.status = 2;
-        r.t_done = now_ms();
-        steps_done_ += r.rk4_steps;
-        hbm_used_ -= r.footprint;
+        // integration finished: the slot and tile budget are free now, the
+        // HBM footprint stays charged until the worker has finished the job
+        r.status = 3;
         active_groups_ -= r.groups;
         free_slots_.push_back(r.slot);
         r.slot = -1;
-        ++completed;
+        done.push_back(&r);
     }
     active_.swap(still);
-    return completed;
+    if (!done.empty()) {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            for (JobRec* r : done) finq_.push_back(r);
+            finishing_ += (int)done.size();
+        }
+        cv_.notify_all();
+    }
+    return take_completed();
 }
 
 void Engine::drain()
@@ -504,9 +586,12 @@ void Engine::drain()
     while (!pending_.empty() || !active_.empty()) {
         const size_t before = pending_.size() + active_.size();
         step();
-        if (active_.empty() && !pending_.empty() && pending_.size() + active_.size() == before)
+        if (active_.empty() && !pending_.empty() && pending_.size() + active_.size() == before &&
+            finishing_.load() == 0)
             fail(CHAOS_E_BUSY, "pending job cannot be admitted");
     }
+    std::unique_lock<std::mutex> lk(mu_);
+    cv_.wait(lk, [&] { return finishing_.load() == 0; });
 }
 
 JobRec& Engine::job(long long id)
@@ -525,6 +610,7 @@ void Engine::release(long long id)
 
 chaos_server_stats Engine::stats() const
 {
+    std::lock_guard<std::mutex> lk(mu_);
     chaos_server_stats s{};
     s.launches = launches_;
     s.graph_replays = graph_replays_;
@@ -536,6 +622,7 @@ chaos_server_stats Engine::stats() const
     s.pending_jobs = (int)pending_.size();
     s.hbm_in_use = hbm_used_;
     s.hbm_budget = hbm_budget_;
+    s.finishing_jobs = finishing_.load();
     return s;
 }
 
@@ -628,6 +715,7 @@ static const JobRec& done_job(void* server, long long id, int kind)
 {
     CHAOS_REQUIRE(server, "server is NULL");
     const JobRec& r = static_cast<Engine*>(server)->job(id);
+    CHAOS_REQUIRE(r.status != -1, "job %lld failed: %s", id, r.error.c_str());
     CHAOS_REQUIRE(r.status == 2, "job %lld is not finished", id);
     CHAOS_REQUIRE(r.spec.kind == kind, "job %lld is of kind %d", id, r.spec.kind);
     return r;

@@ -1,9 +1,14 @@
 // engine.hpp -- batched engine: device job table, tile queue, host scheduler.
 #pragma once
 
+#include <atomic>
+#include <condition_variable>
 #include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "bifurcation.hpp"
@@ -42,7 +47,8 @@ struct Tile {
 struct JobRec {
     long long id = 0;
     chaos_job spec{};
-    int status = 0;  // 0 pending, 1 active, 2 done, -1 failed
+    std::atomic<int> status{0};  // 0 pending, 1 active, 3 finishing, 2 done, -1 failed
+    std::string error;           // failure message (status -1)
     int slot = -1;
     int prog = 0, lorenz = 0, reran = 0;
     double t_submit = 0, t_admit = 0, t_done = 0;
@@ -76,15 +82,19 @@ public:
     void release(long long id);
     chaos_server_stats stats() const;
     double now_ms() const;
-    bool idle() const { return pending_.empty() && active_.empty(); }
+    bool idle() const { return pending_.empty() && active_.empty() && finishing_.load() == 0; }
 
 private:
     void plan(JobRec& r);
     void admit();
     void allocate(JobRec& r);
-    void finalize(JobRec& r);
+    void finalize(JobRec& r, CallCtx& ctx);
     void capture();
     void enqueue_launch();
+    void finisher();                 // worker thread body
+    int take_completed();            // jobs finished since the last call
+    void wait_finisher_progress();   // block until a finishing job completes
+
 
     chaos_engine_params prm_;
     int device_ = 0;
@@ -108,6 +118,16 @@ private:
     long long next_id_ = 1, hbm_budget_ = 0, hbm_used_ = 0, active_groups_ = 0;
     long long launches_ = 0, graph_replays_ = 0, tiles_ = 0;
     double busy_ms_ = 0, finalize_ms_ = 0, steps_done_ = 0, t0_ = 0;
+    // Finishing stages (peak post-processing, clustering, result copies) run
+    // on a worker thread with its own stream, overlapping the next rounds.
+    // mu_ guards finq_, hbm_used_, finalize_ms_, steps_done_ and stop_.
+    mutable std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<JobRec*> finq_;
+    std::atomic<int> finishing_{0}, completed_{0};
+    bool stop_ = false;
+    hipStream_t fstream_ = nullptr;
+    std::thread worker_;
 };
 
 }  // namespace chaos

@@ -234,8 +234,14 @@ void stream_run(int device, const chaos_stream_config& c, const chaos_engine_par
         std::vector<long long> ids(N, -1);
         size_t next = 0, finished = 0;
         t_start = clock_ms();
+        double next_report = c.progress_s > 0 ? c.progress_s * 1000.0 : INFINITY;
         while (finished < N) {
             const double now = clock_ms() - t_start;
+            if (now >= next_report) {
+                std::fprintf(stderr, "stream engine: %.0f s, %zu of %zu jobs arrived, %zu done\n", now / 1000.0,
+                             next, N, finished);
+                next_report += c.progress_s * 1000.0;
+            }
             while (next < N && jobs[next].arrival_ms <= now) ids[next] = eng.submit(jobs[next].job), ++next;
             if (eng.idle()) {
                 if (next < N) sleep_until_ms(t_start + jobs[next].arrival_ms);
@@ -244,13 +250,17 @@ void stream_run(int device, const chaos_stream_config& c, const chaos_engine_par
             const int k = eng.step();
             if (k > 0) {
                 const double t = clock_ms() - t_start;
-                for (size_t i = 0; i < next; ++i)
-                    if (ids[i] >= 0 && done_ms[i] == 0.0 && eng.job(ids[i]).status == 2) {
+                for (size_t i = 0; i < next; ++i) {
+                    if (ids[i] < 0 || done_ms[i] != 0.0) continue;
+                    const JobRec& r = eng.job(ids[i]);
+                    if (r.status == -1) fail(CHAOS_E_HIP, "stream job %zu failed: %s", i, r.error.c_str());
+                    if (r.status == 2) {
                         done_ms[i] = t;
                         eng.release(ids[i]);
                         ids[i] = -1;
                         ++finished;
                     }
+                }
             }
         }
         const chaos_server_stats st = eng.stats();
@@ -259,7 +269,13 @@ void stream_run(int device, const chaos_stream_config& c, const chaos_engine_par
         rep.finalize_ms = st.finalize_ms;
     } else {
         t_start = clock_ms();
+        double next_report = c.progress_s > 0 ? c.progress_s * 1000.0 : INFINITY;
         for (size_t i = 0; i < N; ++i) {
+            if (clock_ms() - t_start >= next_report) {
+                std::fprintf(stderr, "stream per-job: %.0f s, %zu of %zu jobs done\n",
+                             (clock_ms() - t_start) / 1000.0, i, N);
+                next_report += c.progress_s * 1000.0;
+            }
             sleep_until_ms(t_start + jobs[i].arrival_ms);
             run_one_shot(jobs[i].job, device);
             done_ms[i] = clock_ms() - t_start;

@@ -271,7 +271,9 @@ typedef struct chaos_engine_params {
 } chaos_engine_params;
 
 typedef struct chaos_job_info {
-    int status;             /* 0 pending, 1 active, 2 done, -1 failed         */
+    int status;             /* 0 pending, 1 integrating, 3 finishing (post
+                               stages on the server's worker thread),
+                               2 done, -1 failed                             */
     int kind;
     double t_submit_ms, t_admit_ms, t_done_ms;   /* server clock              */
     double rk4_steps;
@@ -291,14 +293,19 @@ typedef struct chaos_server_stats {
     double rk4_steps;       /* steps completed                                */
     int active_jobs, pending_jobs;
     long long hbm_in_use, hbm_budget;
+    int finishing_jobs;     /* handed to the worker thread, not yet done      */
+    int reserved;
 } chaos_server_stats;
 
 int chaos_engine_params_default(chaos_engine_params* p);
 int chaos_server_create(int device, const chaos_engine_params* p, void** server);
 int chaos_server_destroy(void* server);
 int chaos_server_submit(void* server, const chaos_job* job, long long* job_id);
-/* One scheduling round: admit, launch every active job's next tiles, finish
- * completed jobs.  *completed receives the number finished this round. */
+/* One scheduling round: admit, launch every active job's next tiles and hand
+ * jobs whose integration ended to the server's worker thread, which runs
+ * their post stages on its own stream while later rounds integrate.
+ * *completed receives the number of jobs that became done (or failed) since
+ * the previous call.  drain() returns once every job is done or failed. */
 int chaos_server_step(void* server, int* completed);
 int chaos_server_drain(void* server);
 int chaos_server_job_info(void* server, long long job_id, chaos_job_info* info);
@@ -325,6 +332,8 @@ typedef struct chaos_stream_config {
     int bif_min, bif_max, lle_min, lle_max, basins_min, basins_max;
     int mode;
     int max_jobs;           /* cap on generated jobs (0 = none)               */
+    double progress_s;      /* progress line on stderr every progress_s of
+                               wall time (0 = none)                          */
 } chaos_stream_config;
 
 typedef struct chaos_stream_report {

@@ -34,9 +34,41 @@ def test_server_without_device_fails_cleanly(chaos):
 
 
 def test_job_struct_layout(chaos):
-    import ctypes as C
     j = chaos.make_job(chaos.JOB_LLE2D, "rossler_rk4", 1, 8, [0.1] * 3, [0.2, 0.2, 5.7], [0, 1, 2, 3], [0, 2])
-    # sizes and offsets of include/chaoslibrary.h structs on x86-64 (gcc)
-    assert C.sizeof(j) == 232 and chaos.Job.seed.offset == 80 and chaos.Job.ranges.offset == 200
-    assert C.sizeof(chaos.StreamConfig) == 64 and C.sizeof(chaos.StreamReport) == 128
     assert j.amountOfInitialConditions == 3 and j.amountOfValues == 3 and list(j.indicesOfMutVars) == [0, 2]
+
+
+# ctypes mirror -> C struct name in include/chaoslibrary.h
+MIRRORS = {"Options": "chaos_options", "Stats": "chaos_stats", "Job": "chaos_job",
+           "EngineParams": "chaos_engine_params", "JobInfo": "chaos_job_info",
+           "ServerStats": "chaos_server_stats", "StreamConfig": "chaos_stream_config",
+           "StreamReport": "chaos_stream_report"}
+
+
+def test_ctypes_mirrors_match_the_header(chaos, tmp_path):
+    """Every ctypes structure has the size and field offsets gcc gives the
+    header's struct (the header is the ABI)."""
+    import ctypes as C
+    import os
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "chaoslibrary.h"', "int main(void) {"]
+    for py, cs in MIRRORS.items():
+        lines.append(f'printf("{py} size %zu\\n", sizeof({cs}));')
+        for f, _ in getattr(chaos, py)._fields_:
+            cf = "lambda" if f == "lambda_" else f
+            lines.append(f'printf("{py} {f} %zu\\n", offsetof({cs}, {cf}));')
+    lines.append("return 0; }")
+    src = tmp_path / "abi.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "abi"
+    subprocess.run(["gcc", "-I", os.path.join(root, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    for line in filter(None, out):
+        py, f, v = line.split()
+        cls = getattr(chaos, py)
+        got = C.sizeof(cls) if f == "size" else getattr(cls, f).offset
+        assert got == int(v), (py, f, got, v)

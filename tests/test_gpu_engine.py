@@ -108,10 +108,11 @@ def test_admission_by_hbm_budget(chaos):
         srv.step()
         st = srv.stats()
         n_active_max = max(n_active_max, st.active_jobs)
-        if st.active_jobs == 0 and st.pending_jobs == 0:
+        if st.active_jobs == 0 and st.pending_jobs == 0 and st.finishing_jobs == 0:
             break
     assert n_active_max <= 2
     for jid, j in zip(ids, jobs):
+        assert srv.info(jid).status == 2
         assert same(srv.result(jid, j), one_shot(chaos, j))
     srv.close()
 
