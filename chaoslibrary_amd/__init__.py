@@ -331,6 +331,118 @@ def basinsOfAttraction_2(tMax, nPts, h, amountOfInitialConditions, initialCondit
 
 
 # ---------------------------------------------------------------------------
+# legacy entry points (namespace old_library)
+# ---------------------------------------------------------------------------
+
+_LLE_STYLE = [C.POINTER(Options), C.c_double, C.c_double, C.c_int, C.c_double, C.c_double, _D, C.c_int, _D, _I,
+              C.c_int, C.c_double, C.c_double, _D, C.c_int]
+_SIGS_LEGACY = {
+    "chaos_legacy_lle": (C.c_int, _LLE_STYLE + [C.c_int, C.c_int, C.c_void_p]),
+    "chaos_legacy_ls": (C.c_int, _LLE_STYLE + [C.c_int, C.c_void_p]),
+    "chaos_time_domain": (C.c_int, [C.POINTER(Options), C.c_double, C.c_int, C.c_double, C.c_int, _D, _D, _I,
+                                    C.c_int, C.c_double, C.c_double, _D, C.c_int, C.c_int, C.c_void_p,
+                                    C.c_void_p, C.c_longlong, C.POINTER(C.c_longlong)]),
+    "chaos_TimeDomainCalculation": (C.c_int, [C.POINTER(Options), C.c_double, C.c_int, C.c_double, C.c_int, _D,
+                                              _D, _I, C.c_int, C.c_double, C.c_double, _D, C.c_int, C.c_int,
+                                              C.c_char_p]),
+}
+for _n in ("chaos_LLE1D", "chaos_LLE1DIC", "chaos_LLE2DIC", "chaos_LS1D", "chaos_LS2D"):
+    _SIGS_LEGACY[_n] = (C.c_int, _LLE_STYLE + [C.c_char_p])
+for _name, (_res, _args) in _SIGS_LEGACY.items():
+    getattr(lib, _name).restype = _res
+    getattr(lib, _name).argtypes = _args
+
+
+def _lle_style_args(o, tMax, NT, nPts, h, eps, ic, ranges, idx, writableVar, maxValue, transientTime, vals):
+    return (C.byref(o), tMax, NT, nPts, h, eps, _dp(ic), ic.size, _dp(ranges), _ip(idx), writableVar, maxValue,
+            transientTime, _dp(vals), vals.size)
+
+
+def legacy_lle(tMax, NT, nPts, h, eps, initialConditions, ranges, indicesOfMutVars, transientTime, values,
+               dims=1, sweep_ic=False, maxValue=0.0, writableVar=0, system="chameleon", semantics=SEM_REFERENCE,
+               **kw):
+    """chaos_legacy_lle: LLE over nPts**dims points (p = k0 + nPts*k1)."""
+    o = make_options(system, semantics, **kw)
+    ic, rg, ix, vals = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
+    out = np.zeros(nPts ** dims)
+    _check(lib.chaos_legacy_lle(*_lle_style_args(o, tMax, NT, nPts, h, eps, ic, rg, ix, writableVar, maxValue,
+                                                 transientTime, vals), dims, 1 if sweep_ic else 0,
+                                C.c_void_p(out.ctypes.data)))
+    return out
+
+
+def legacy_ls(tMax, NT, nPts, h, eps, initialConditions, ranges, indicesOfMutVars, transientTime, values,
+              dims=1, maxValue=0.0, writableVar=0, system="chameleon", semantics=SEM_REFERENCE, **kw):
+    """chaos_legacy_ls: Lyapunov spectrum, shape (nPts**dims, NX)."""
+    o = make_options(system, semantics, **kw)
+    ic, rg, ix, vals = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
+    out = np.zeros((nPts ** dims, ic.size))
+    _check(lib.chaos_legacy_ls(*_lle_style_args(o, tMax, NT, nPts, h, eps, ic, rg, ix, writableVar, maxValue,
+                                                transientTime, vals), dims, C.c_void_p(out.ctypes.data)))
+    return out
+
+
+def time_domain(tMax, nPts, h, initialConditions, ranges, indicesOfMutVars, writableVar, maxValue, transientTime,
+                values, preScaller=1, system="chameleon", semantics=SEM_REFERENCE, **kw):
+    """chaos_time_domain: (samples[nPts, B], flags[nPts])."""
+    o = make_options(system, semantics, **kw)
+    ic, rg, ix, vals = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
+    B = C.c_longlong(0)
+    args = (C.byref(o), tMax, nPts, h, ic.size, _dp(ic), _dp(rg), _ip(ix), writableVar, maxValue, transientTime,
+            _dp(vals), vals.size, preScaller)
+    _check(lib.chaos_time_domain(*args, None, None, 0, C.byref(B)))
+    out = np.zeros((nPts, B.value))
+    flags = np.zeros(nPts, dtype=np.int32)
+    _check(lib.chaos_time_domain(*args, C.c_void_p(out.ctypes.data), C.c_void_p(flags.ctypes.data), out.size,
+                                 C.byref(B)))
+    return out, flags
+
+
+def _legacy_file(fn, tMax, NT, nPts, h, eps, initialConditions, amountOfInitialConditions, ranges,
+                 indicesOfMutVars, writableVar, maxValue, transientTime, values, amountOfValues, OUT_FILE_PATH,
+                 options):
+    ic, rg, ix, vals = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
+    _check(getattr(lib, fn)(_opt_ref(options), tMax, NT, nPts, h, eps, _dp(ic), amountOfInitialConditions, _dp(rg),
+                            _ip(ix), writableVar, maxValue, transientTime, _dp(vals), amountOfValues,
+                            str(OUT_FILE_PATH).encode()))
+
+
+def LLE1D(*args, options=None):
+    """old_library::LLE1D (debri/hostLibrary.cuh:241-256)."""
+    _legacy_file("chaos_LLE1D", *args, options=options)
+
+
+def LLE1DIC(*args, options=None):
+    """old_library::LLE1DIC (debri/hostLibrary.cuh:263-278)."""
+    _legacy_file("chaos_LLE1DIC", *args, options=options)
+
+
+def LLE2DIC(*args, options=None):
+    """old_library::LLE2DIC (debri/hostLibrary.cuh:339-354)."""
+    _legacy_file("chaos_LLE2DIC", *args, options=options)
+
+
+def LS1D(*args, options=None):
+    """old_library::LS1D (debri/hostLibrary.cuh:377-392)."""
+    _legacy_file("chaos_LS1D", *args, options=options)
+
+
+def LS2D(*args, options=None):
+    """old_library::LS2D (debri/hostLibrary.cuh:416-431)."""
+    _legacy_file("chaos_LS2D", *args, options=options)
+
+
+def TimeDomainCalculation(tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges, indicesOfMutVars,
+                          writableVar, maxValue, transientTime, values, amountOfValues, preScaller, OUT_FILE_PATH,
+                          options=None):
+    """old_library::TimeDomainCalculation (debri/hostLibrary.cuh:69-83)."""
+    ic, rg, ix, vals = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
+    _check(lib.chaos_TimeDomainCalculation(_opt_ref(options), tMax, nPts, h, amountOfInitialConditions, _dp(ic),
+                                           _dp(rg), _ip(ix), writableVar, maxValue, transientTime, _dp(vals),
+                                           amountOfValues, preScaller, str(OUT_FILE_PATH).encode()))
+
+
+# ---------------------------------------------------------------------------
 # batched engine / job server / mixed-job stream
 # ---------------------------------------------------------------------------
 

@@ -7,6 +7,7 @@
 #include "../../include/LLEHost.h"
 #include "../../include/basinsHOST.h"
 #include "../../include/bifurcationHOST.h"
+#include "../../include/legacyHOST.h"
 #include "../../include/chaoslibrary.h"
 
 namespace {
@@ -68,3 +69,38 @@ void basinsOfAttraction_2(const double tMax, const int nPts, const double h, con
 }
 
 }  // namespace Basins
+
+namespace old_library {
+
+void TimeDomainCalculation(const double tMax, const int nPts, const double h, const int amountOfInitialConditions,
+                           const double* initialConditions, const double* ranges, const int* indicesOfMutVars,
+                           const int writableVar, const double maxValue, const double transientTime,
+                           const double* values, const int amountOfValues, const int preScaller,
+                           std::string OUT_FILE_PATH)
+{
+    check(chaos_TimeDomainCalculation(nullptr, tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges,
+                                      indicesOfMutVars, writableVar, maxValue, transientTime, values, amountOfValues,
+                                      preScaller, OUT_FILE_PATH.c_str()));
+}
+
+#define CHAOS_LEGACY_LLE_STYLE(NAME)                                                                              \
+    void NAME(const double tMax, const double NT, const int nPts, const double h, const double eps,                \
+              const double* initialConditions, const int amountOfInitialConditions, const double* ranges,          \
+              const int* indicesOfMutVars, const int writableVar, const double maxValue,                           \
+              const double transientTime, const double* values, const int amountOfValues,                          \
+              std::string OUT_FILE_PATH)                                                                           \
+    {                                                                                                              \
+        check(chaos_##NAME(nullptr, tMax, NT, nPts, h, eps, initialConditions, amountOfInitialConditions, ranges,  \
+                           indicesOfMutVars, writableVar, maxValue, transientTime, values, amountOfValues,         \
+                           OUT_FILE_PATH.c_str()));                                                                \
+    }
+
+CHAOS_LEGACY_LLE_STYLE(LLE1D)
+CHAOS_LEGACY_LLE_STYLE(LLE1DIC)
+CHAOS_LEGACY_LLE_STYLE(LLE2DIC)
+CHAOS_LEGACY_LLE_STYLE(LS1D)
+CHAOS_LEGACY_LLE_STYLE(LS2D)
+
+#undef CHAOS_LEGACY_LLE_STYLE
+
+}  // namespace old_library

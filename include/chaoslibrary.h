@@ -228,6 +228,92 @@ int chaos_dbscan(const chaos_options* opt, const double* x, const double* y,
                  int min_pts, int sign_rule, const int* include, int* labels,
                  int* clusters);
 
+/* ---------------- Legacy entry points (debri/, namespace old_library) ---- */
+/* The legacy library sweeps the same integrator with its own LLE kernels
+ * (debri/cudaLibrary.cu:1897-2137), the Lyapunov spectrum
+ * (debri/cudaLibrary.cu:2189-2331) and a trajectory recorder
+ * (debri/hostLibrary.cu:5094-5331).  Windows int(tMax/NT) of int(NT/h)
+ * steps; every trajectory stops for the rest of a window after a step that
+ * leaves X[0] NaN/inf or fabsf(X[0]) > maxValue (maxValue != 0), as the
+ * legacy loop does (writableVar is not used by these kernels).  Directions:
+ * seeded unit vectors (chaos_options.seed) instead of curand / sinf.
+ *
+ *   chaos_LLE1D   <- old_library::LLE1D   (debri/hostLibrary.cuh:241-256)
+ *   chaos_LLE1DIC <- old_library::LLE1DIC (debri/hostLibrary.cuh:263-278)
+ *   chaos_LLE2DIC <- old_library::LLE2DIC (debri/hostLibrary.cuh:339-354)
+ *   chaos_LS1D    <- old_library::LS1D    (debri/hostLibrary.cuh:377-392)
+ *   chaos_LS2D    <- old_library::LS2D    (debri/hostLibrary.cuh:416-431)
+ *   chaos_TimeDomainCalculation <- old_library::TimeDomainCalculation
+ *                                          (debri/hostLibrary.cuh:69-83)
+ *
+ * chaos_legacy_lle: out[p], p = k0 + nPts*k1 over dims 1 or 2; sweep_ic 0
+ * sweeps values[indicesOfMutVars[d]] (LLEKernelCUDA form), 1 sweeps
+ * initialConditions[indicesOfMutVars[d]] (LLEKernelICCUDA form).
+ * chaos_legacy_ls: out[p*NX + k], exponents of the parameter sweep. */
+int chaos_legacy_lle(const chaos_options* opt, double tMax, double NT, int nPts,
+                     double h, double eps, const double* initialConditions,
+                     int amountOfInitialConditions, const double* ranges,
+                     const int* indicesOfMutVars, int writableVar,
+                     double maxValue, double transientTime, const double* values,
+                     int amountOfValues, int dims, int sweep_ic, double* out);
+int chaos_legacy_ls(const chaos_options* opt, double tMax, double NT, int nPts,
+                    double h, double eps, const double* initialConditions,
+                    int amountOfInitialConditions, const double* ranges,
+                    const int* indicesOfMutVars, int writableVar,
+                    double maxValue, double transientTime, const double* values,
+                    int amountOfValues, int dims, double* out);
+int chaos_LLE1D(const chaos_options* opt, double tMax, double NT, int nPts,
+                double h, double eps, const double* initialConditions,
+                int amountOfInitialConditions, const double* ranges,
+                const int* indicesOfMutVars, int writableVar, double maxValue,
+                double transientTime, const double* values, int amountOfValues,
+                const char* OUT_FILE_PATH);
+int chaos_LLE1DIC(const chaos_options* opt, double tMax, double NT, int nPts,
+                  double h, double eps, const double* initialConditions,
+                  int amountOfInitialConditions, const double* ranges,
+                  const int* indicesOfMutVars, int writableVar, double maxValue,
+                  double transientTime, const double* values, int amountOfValues,
+                  const char* OUT_FILE_PATH);
+int chaos_LLE2DIC(const chaos_options* opt, double tMax, double NT, int nPts,
+                  double h, double eps, const double* initialConditions,
+                  int amountOfInitialConditions, const double* ranges,
+                  const int* indicesOfMutVars, int writableVar, double maxValue,
+                  double transientTime, const double* values, int amountOfValues,
+                  const char* OUT_FILE_PATH);
+int chaos_LS1D(const chaos_options* opt, double tMax, double NT, int nPts,
+               double h, double eps, const double* initialConditions,
+               int amountOfInitialConditions, const double* ranges,
+               const int* indicesOfMutVars, int writableVar, double maxValue,
+               double transientTime, const double* values, int amountOfValues,
+               const char* OUT_FILE_PATH);
+int chaos_LS2D(const chaos_options* opt, double tMax, double NT, int nPts,
+               double h, double eps, const double* initialConditions,
+               int amountOfInitialConditions, const double* ranges,
+               const int* indicesOfMutVars, int writableVar, double maxValue,
+               double transientTime, const double* values, int amountOfValues,
+               const char* OUT_FILE_PATH);
+/* Recorded samples X[writableVar] of a 1-D parameter sweep: out[p*B + t],
+ * B = int(tMax/h/preScaller) (*samples); the transient runs
+ * int(transientTime/h) iterations of preScaller steps; flags[p] in
+ * {1, 0, -1}; only flag-1 points are recorded (others, and samples after a
+ * divergence, are 0).  out == NULL only reports *samples. */
+int chaos_time_domain(const chaos_options* opt, double tMax, int nPts, double h,
+                      int amountOfInitialConditions,
+                      const double* initialConditions, const double* ranges,
+                      const int* indicesOfMutVars, int writableVar,
+                      double maxValue, double transientTime,
+                      const double* values, int amountOfValues, int preScaller,
+                      double* out, int* flags, long long capacity,
+                      long long* samples);
+int chaos_TimeDomainCalculation(const chaos_options* opt, double tMax, int nPts,
+                                double h, int amountOfInitialConditions,
+                                const double* initialConditions,
+                                const double* ranges,
+                                const int* indicesOfMutVars, int writableVar,
+                                double maxValue, double transientTime,
+                                const double* values, int amountOfValues,
+                                int preScaller, const char* OUT_FILE_PATH);
+
 /* ---------------- Batched engine and job server ------------------------- */
 /* No counterpart in the reference (one synchronous call per sweep, one
  * system per build).  Jobs of any of the four kinds run together in one

@@ -950,3 +950,238 @@ int orc_dbscan_bruteforce(const double* pts, int n, int dim, double eps,
     free(par);
     return K;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Legacy entry points (debri/, namespace old_library)                      */
+/* ------------------------------------------------------------------------ */
+
+/* Horizon of the legacy drivers: `int x = T / h` truncation
+ * (debri/hostLibrary.cu:3128-3136) or llround in CONFIG semantics. */
+static long long legacy_count(double q, int sem)
+{
+    return sem == ORC_SEM_REFERENCE ? (long long)q : llround(q);
+}
+
+/* debri/cudaLibrary.cu:472-524 (loopCalculateDiscreteModel) as the LLE and
+ * LS kernels call it (preScaller 1, writableVar 0): n steps, stopping early
+ * after a step that leaves X[0] NaN/inf or fabsf(X[0]) > maxValue
+ * (maxValue != 0).  Its return value is discarded by every caller.
+ * Pinned by: divergence cases in tests/test_oracle_legacy.py. */
+static void legacy_run(int sys, double* X, const double* a, double h, long long n, double maxValue)
+{
+    for (long long i = 0; i < n; ++i) {
+        orc_step(sys, X, a, h);
+        if (diverged(X[0], maxValue, ORC_CHK_BASINS_REF)) return;
+    }
+}
+
+/* Swept values of legacy point idx (debri/cudaLibrary.cu:1182-1186,
+ * getValueByIdx with valueNumber = dimension index). */
+static void legacy_point(int idx, int nPts, int dims, const double* ranges, const int* mut,
+                         double* target)
+{
+    for (int d = 0; d < dims; ++d)
+        target[mut[d]] = orc_grid_value_pow(idx, nPts, ranges[2 * d], ranges[2 * d + 1], d);
+}
+
+/*
+ * debri/cudaLibrary.cu:1897-2031 (LLEKernelCUDA, form 0: parameters swept)
+ * and :2034-2137 (LLEKernelICCUDA, form 1: initial conditions swept).
+ * Windows W = int(tMax/NT), NT/h steps each, on X then on the perturbed Y;
+ * form 0: d = sqrt(sum((1/eps)(x-y))^2), Y = X - (X-Y)/d;
+ * form 1: d = sqrt(sum (x-y)^2)/eps, Y = X - (X-Y)*(d != 0 ? 1/d : d);
+ * result sum(log d)/tMax.  The initial direction is the seeded unit vector
+ * (orc_unit_vector at the point index) instead of curand / sinf.
+ * Pinned by: the linear diagonal system closed form, a stable equilibrium,
+ * form 0 == form 1 where both are defined, and the LLE module on the same
+ * trajectory (tests/test_oracle_legacy.py).
+ */
+double orc_legacy_lle_point(int sys, int sem, int form, double tMax, double NT, double h,
+                            double eps, const double* X0, int nx, const double* a,
+                            double maxValue, double transientTime, uint64_t seed,
+                            long long point)
+{
+    double X[ORC_MAX_NX], Y[ORC_MAX_NX], v[ORC_MAX_NX];
+    long long S = legacy_count(transientTime / h, sem);
+    long long NTp = legacy_count(NT / h, sem);
+    long long W = legacy_count(tMax / NT, sem);
+    for (int i = 0; i < nx; ++i) X[i] = X0[i];
+    legacy_run(sys, X, a, h, S, maxValue);
+    orc_unit_vector(seed, point, nx, v);
+    for (int i = 0; i < nx; ++i) Y[i] = X[i] + v[i] * eps;
+    double result = 0;
+    for (long long w = 0; w < W; ++w) {
+        legacy_run(sys, X, a, h, NTp, maxValue);
+        legacy_run(sys, Y, a, h, NTp, maxValue);
+        double t = 0;
+        if (form == 0) {
+            for (int l = 0; l < nx; ++l) {
+                double t2 = 1 / eps;
+                t2 = t2 * (X[l] - Y[l]);
+                t += t2 * t2;
+            }
+            t = sqrt(t);
+            result += log(t);
+            for (int j = 0; j < nx; ++j) Y[j] = X[j] - ((X[j] - Y[j]) / t);
+        } else {
+            for (int l = 0; l < nx; ++l) t += (X[l] - Y[l]) * (X[l] - Y[l]);
+            t = sqrt(t) / eps;
+            result += log(t);
+            if (t != 0) t = 1 / t;
+            for (int j = 0; j < nx; ++j) Y[j] = X[j] - ((X[j] - Y[j]) * t);
+        }
+    }
+    return result / tMax;
+}
+
+/* Legacy LLE over a 1-D or 2-D sweep of parameters (form 0) or initial
+ * conditions (form 1): out[idx], idx = k0 + nPts*k1 (debri/hostLibrary.cu:
+ * 3111-3325 LLE1D, :3328-3543 LLE1DIC, :3692-3827 LLE2DIC). */
+int orc_legacy_lle(int sys, int sem, int form, int dims, double tMax, double NT, int nPts,
+                   double h, double eps, const double* ic, int nx, const double* ranges,
+                   const int* idx, double maxValue, double transientTime,
+                   const double* values, int na, uint64_t seed, double* out,
+                   int point_begin, int point_end)
+{
+    int snx, sna;
+    if (orc_dims(sys, &snx, &sna) || nx != snx || na != sna || nPts < 2) return -2;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int p = point_begin; p < point_end; ++p) {
+        double X[ORC_MAX_NX], a[ORC_MAX_NX];
+        for (int i = 0; i < nx; ++i) X[i] = ic[i];
+        for (int i = 0; i < na; ++i) a[i] = values[i];
+        legacy_point(p, nPts, dims, ranges, idx, form == 0 ? a : X);
+        out[p - point_begin] = orc_legacy_lle_point(sys, sem, form, tMax, NT, h, eps, X, nx, a,
+                                                    maxValue, transientTime, seed, p);
+    }
+    return 0;
+}
+
+/* debri/cudaLibrary.cu:2141-2187 (projectionOperator, gramSchmidtProcess):
+ * classical Gram-Schmidt of the n rows of a into b (each row minus its
+ * projections, numerator from the original row, onto the earlier
+ * unnormalised rows), then every row normalised; den[i] = row norm.
+ * Pinned by: orthonormality, span and the first row's direction
+ * (tests/test_oracle_legacy.py). */
+void orc_gram_schmidt(const double* a, double* b, int n, double* den)
+{
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < n; ++j) b[j + i * n] = a[j + i * n];
+        for (int j = 0; j < i; ++j) {
+            double num = 0, dn = 0;
+            for (int k = 0; k < n; ++k) {
+                num += a[i * n + k] * b[j * n + k];
+                dn += b[j * n + k] * b[j * n + k];
+            }
+            double frac = dn == 0 ? 0 : num / dn;
+            for (int k = 0; k < n; ++k) b[i * n + k] -= frac * b[j * n + k];
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        double d = 0;
+        for (int j = 0; j < n; ++j) d += b[i * n + j] * b[i * n + j];
+        d = sqrt(d);
+        for (int j = 0; j < n; ++j) b[i * n + j] = d == 0 ? 0 : b[i * n + j] / d;
+        if (den) den[i] = d;
+    }
+}
+
+/*
+ * debri/cudaLibrary.cu:2189-2331 (LSKernelCUDA): Lyapunov spectrum with nx
+ * perturbed trajectories.  Initial directions: nx seeded unit vectors
+ * (orc_unit_vector at point*4 + j, instead of the sinf rows), Gram-Schmidt,
+ * Y_j = Y_j*eps + X.  Per window: X and every Y_j advance NT/h steps (early
+ * stop as legacy_run), D_k = Y_k - X, Gram-Schmidt(D -> Z, den),
+ * res[k] += log(den[k]/eps), Y_k = X + Z_k*eps.  out[k] = res[k]/tMax.
+ * Pinned by: the linear diagonal system (spectrum = ln R(h lambda_k)/h in
+ * order), sum of exponents = trace for Lorenz (-(sigma+1+beta)), largest
+ * exponent vs the LLE oracle (tests/test_oracle_legacy.py).
+ */
+int orc_legacy_ls_point(int sys, int sem, double tMax, double NT, double h, double eps,
+                        const double* X0, int nx, const double* a, double maxValue,
+                        double transientTime, uint64_t seed, long long point, double* out)
+{
+    double X[ORC_MAX_NX], Y[16] = {0}, Z[16] = {0}, den[4] = {0}, res[4] = {0};
+    if (nx > 4) return -2;
+    long long S = legacy_count(transientTime / h, sem);
+    long long NTp = legacy_count(NT / h, sem);
+    long long W = legacy_count(tMax / NT, sem);
+    for (int i = 0; i < nx; ++i) {
+        X[i] = X0[i];
+        res[i] = 0;
+    }
+    for (int j = 0; j < nx; ++j) orc_unit_vector(seed, point * 4 + j, nx, Z + j * nx);
+    legacy_run(sys, X, a, h, S, maxValue);
+    orc_gram_schmidt(Z, Y, nx, NULL);
+    for (int j = 0; j < nx; ++j)
+        for (int i = 0; i < nx; ++i) Y[j * nx + i] = Y[j * nx + i] * eps + X[i];
+    for (long long w = 0; w < W; ++w) {
+        legacy_run(sys, X, a, h, NTp, maxValue);
+        for (int j = 0; j < nx; ++j) legacy_run(sys, Y + j * nx, a, h, NTp, maxValue);
+        for (int k = 0; k < nx; ++k)
+            for (int l = 0; l < nx; ++l) Y[k * nx + l] = Y[k * nx + l] - X[l];
+        orc_gram_schmidt(Y, Z, nx, den);
+        for (int k = 0; k < nx; ++k) {
+            res[k] += log(den[k] / eps);
+            for (int j = 0; j < nx; ++j) Y[k * nx + j] = X[j] + Z[k * nx + j] * eps;
+        }
+    }
+    for (int i = 0; i < nx; ++i) out[i] = res[i] / tMax;
+    return 0;
+}
+
+/* Legacy LS over a 1-D or 2-D parameter sweep: out[idx*nx + k]
+ * (debri/hostLibrary.cu:3829-3959 LS1D, :3961-4113 LS2D). */
+int orc_legacy_ls(int sys, int sem, int dims, double tMax, double NT, int nPts, double h,
+                  double eps, const double* ic, int nx, const double* ranges, const int* idx,
+                  double maxValue, double transientTime, const double* values, int na,
+                  uint64_t seed, double* out, int point_begin, int point_end)
+{
+    int snx, sna;
+    if (orc_dims(sys, &snx, &sna) || nx != snx || na != sna || nPts < 2 || nx > 4) return -2;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int p = point_begin; p < point_end; ++p) {
+        double a[ORC_MAX_NX];
+        for (int i = 0; i < na; ++i) a[i] = values[i];
+        legacy_point(p, nPts, dims, ranges, idx, a);
+        orc_legacy_ls_point(sys, sem, tMax, NT, h, eps, ic, nx, a, maxValue, transientTime, seed, p,
+                            out + (long long)(p - point_begin) * nx);
+    }
+    return 0;
+}
+
+/*
+ * debri/hostLibrary.cu:5094-5331 (TimeDomainCalculation) with
+ * debri/cudaLibrary.cu:879-953 (calculateDiscreteModelCUDA) and :604-680
+ * (loopCalculateDiscreteModel_int): per parameter point, the transient runs
+ * int(transientTime/h) iterations of preScaller steps with the checks on
+ * X[writableVar]; only a flag of 1 records B = int(tMax/h/preScaller)
+ * samples (X[writableVar] before each iteration).  Samples after a
+ * divergence, and all samples of an unrecorded point, are 0 here (the
+ * reference leaves that memory uninitialised).  out[p*B + t]; flags[p].
+ * Pinned by: orc_loop's pins and a hand-integrated short run.
+ */
+long long orc_time_domain(int sys, int sem, double tMax, int nPts, double h, const double* ic, int nx,
+                          const double* ranges, const int* idx, int writableVar, double maxValue,
+                          double transientTime, const double* values, int na, int preScaller,
+                          double* out, int* flags)
+{
+    int snx, sna;
+    if (orc_dims(sys, &snx, &sna) || nx != snx || na != sna || nPts < 2 || preScaller < 1) return -2;
+    long long B = legacy_count(tMax / h / preScaller, sem);
+    long long S = legacy_count(transientTime / h, sem);
+    for (long long i = 0; i < (long long)nPts * B; ++i) out[i] = 0.0;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int p = 0; p < nPts; ++p) {
+        double X[ORC_MAX_NX], a[ORC_MAX_NX];
+        for (int i = 0; i < nx; ++i) X[i] = ic[i];
+        for (int i = 0; i < na; ++i) a[i] = values[i];
+        legacy_point(p, nPts, 1, ranges, idx, a);
+        int flag = orc_loop(sys, X, a, h, (int)S, preScaller, writableVar, maxValue, ORC_CHK_BASINS_REF, NULL);
+        if (flag == 1)
+            flag = orc_loop(sys, X, a, h, (int)B, preScaller, writableVar, maxValue, ORC_CHK_BASINS_REF,
+                            out + (long long)p * B);
+        if (flags) flags[p] = flag;
+    }
+    return B;
+}

@@ -107,6 +107,29 @@ int orc_zscore(double* f0, double* f1, double* f2, const int* flags, long long n
 int orc_dbscan_bruteforce(const double* pts, int n, int dim, double eps,
                           int minPts, const int* include, int* labels);
 
+/* Legacy entry points (debri/, namespace old_library). */
+double orc_legacy_lle_point(int sys, int sem, int form, double tMax, double NT, double h,
+                            double eps, const double* X0, int nx, const double* a,
+                            double maxValue, double transientTime, uint64_t seed,
+                            long long point);
+int orc_legacy_lle(int sys, int sem, int form, int dims, double tMax, double NT, int nPts,
+                   double h, double eps, const double* ic, int nx, const double* ranges,
+                   const int* idx, double maxValue, double transientTime,
+                   const double* values, int na, uint64_t seed, double* out,
+                   int point_begin, int point_end);
+void orc_gram_schmidt(const double* a, double* b, int n, double* den);
+int orc_legacy_ls_point(int sys, int sem, double tMax, double NT, double h, double eps,
+                        const double* X0, int nx, const double* a, double maxValue,
+                        double transientTime, uint64_t seed, long long point, double* out);
+int orc_legacy_ls(int sys, int sem, int dims, double tMax, double NT, int nPts, double h,
+                  double eps, const double* ic, int nx, const double* ranges, const int* idx,
+                  double maxValue, double transientTime, const double* values, int na,
+                  uint64_t seed, double* out, int point_begin, int point_end);
+long long orc_time_domain(int sys, int sem, double tMax, int nPts, double h, const double* ic, int nx,
+                          const double* ranges, const int* idx, int writableVar, double maxValue,
+                          double transientTime, const double* values, int na, int preScaller,
+                          double* out, int* flags);
+
 /* CPU baseline: LLE rows over OpenMP threads; returns threads used. */
 int orc_lle2d_omp(int sys, int sem, double tMax, double NT, int nPts, double h,
                   double eps, const double* ic, int nx, const double* ranges,

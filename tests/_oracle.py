@@ -60,6 +60,22 @@ def _load():
         "orc_det_sum": (C.c_double, [_D, _I, C.c_longlong]),
         "orc_zscore": (C.c_int, [_D, _D, _D, _I, C.c_longlong, _D]),
         "orc_dbscan_bruteforce": (C.c_int, [_D, C.c_int, C.c_int, C.c_double, C.c_int, _I, _I]),
+        "orc_legacy_lle_point": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                              C.c_double, C.c_double, _D, C.c_int, _D, C.c_double,
+                                              C.c_double, C.c_uint64, C.c_longlong]),
+        "orc_legacy_lle": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                     C.c_double, C.c_double, _D, C.c_int, _D, _I, C.c_double, C.c_double,
+                                     _D, C.c_int, C.c_uint64, _D, C.c_int, C.c_int]),
+        "orc_gram_schmidt": (None, [_D, _D, C.c_int, _D]),
+        "orc_legacy_ls_point": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                          _D, C.c_int, _D, C.c_double, C.c_double, C.c_uint64,
+                                          C.c_longlong, _D]),
+        "orc_legacy_ls": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double,
+                                    C.c_double, _D, C.c_int, _D, _I, C.c_double, C.c_double, _D, C.c_int,
+                                    C.c_uint64, _D, C.c_int, C.c_int]),
+        "orc_time_domain": (C.c_longlong, [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, _D, C.c_int,
+                                           _D, _I, C.c_int, C.c_double, C.c_double, _D, C.c_int, C.c_int,
+                                           _D, _I]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -234,3 +250,57 @@ def dbscan_bruteforce(pts, eps, min_pts, include=None):
     k = lib.orc_dbscan_bruteforce(dptr(pts), n, dim, eps, min_pts, iptr(inc), iptr(labels))
     assert k >= 0
     return labels
+
+
+def _arr(x, dt=np.float64):
+    return np.ascontiguousarray(x, dtype=dt)
+
+
+def legacy_lle(sysid, sem, form, dims, tMax, NT, nPts, h, eps, ic, ranges, idx, maxv, tT, values, seed,
+               point_begin=0, point_end=None):
+    n = nPts ** dims
+    point_end = n if point_end is None else point_end
+    ic, ranges, values = _arr(ic), _arr(ranges), _arr(values)
+    idx = _arr(idx, np.int32)
+    out = np.zeros(point_end - point_begin)
+    rc = lib.orc_legacy_lle(sysid, sem, form, dims, tMax, NT, nPts, h, eps, dptr(ic), ic.size, dptr(ranges),
+                            iptr(idx), maxv, tT, dptr(values), values.size, seed, dptr(out), point_begin,
+                            point_end)
+    assert rc == 0, rc
+    return out
+
+
+def gram_schmidt(a):
+    a = _arr(a)
+    n = a.shape[0]
+    b = np.zeros_like(a)
+    den = np.zeros(n)
+    lib.orc_gram_schmidt(dptr(a), dptr(b), n, dptr(den))
+    return b, den
+
+
+def legacy_ls(sysid, sem, dims, tMax, NT, nPts, h, eps, ic, ranges, idx, maxv, tT, values, seed,
+              point_begin=0, point_end=None):
+    n = nPts ** dims
+    point_end = n if point_end is None else point_end
+    ic, ranges, values = _arr(ic), _arr(ranges), _arr(values)
+    idx = _arr(idx, np.int32)
+    out = np.zeros((point_end - point_begin) * ic.size)
+    rc = lib.orc_legacy_ls(sysid, sem, dims, tMax, NT, nPts, h, eps, dptr(ic), ic.size, dptr(ranges),
+                           iptr(idx), maxv, tT, dptr(values), values.size, seed, dptr(out), point_begin,
+                           point_end)
+    assert rc == 0, rc
+    return out.reshape(point_end - point_begin, ic.size)
+
+
+def time_domain(sysid, sem, tMax, nPts, h, ic, ranges, idx, wv, maxv, tT, values, pre):
+    ic, ranges, values = _arr(ic), _arr(ranges), _arr(values)
+    idx = _arr(idx, np.int32)
+    q = tMax / h / pre
+    B = int(q) if sem == SEM_REFERENCE else int(np.floor(q + 0.5))
+    out = np.zeros(nPts * max(B, 0))
+    flags = np.zeros(nPts, dtype=np.int32)
+    got = lib.orc_time_domain(sysid, sem, tMax, nPts, h, dptr(ic), ic.size, dptr(ranges), iptr(idx), wv, maxv,
+                              tT, dptr(values), values.size, pre, dptr(out), iptr(flags))
+    assert got == B, (got, B)
+    return out.reshape(nPts, B), flags

@@ -57,3 +57,28 @@ def test_invalid_arguments_rejected_without_device(chaos):
 
 def test_device_count_does_not_crash(chaos):
     assert chaos.device_count() >= 0
+
+
+def test_cpp_api_links(chaos, tmp_path):
+    """Every C++ entry point of include/*HOST.h / LLEHost.h resolves in
+    libchaos.so (a program taking their addresses links)."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    fns = {"bifurcationHOST.h": ["Bifurcation::bifurcation1D", "Bifurcation::bifurcation2D"],
+           "LLEHost.h": ["LLE::LLE2D"], "basinsHOST.h": ["Basins::basinsOfAttraction_2"],
+           "legacyHOST.h": ["old_library::TimeDomainCalculation", "old_library::LLE1D", "old_library::LLE1DIC",
+                            "old_library::LLE2DIC", "old_library::LS1D", "old_library::LS2D"]}
+    src = "".join(f'#include "{h}"\n' for h in fns) + "#include <cstdio>\nint main() {\n"
+    for names in fns.values():
+        for n in names:
+            src += f'    std::printf("%p\\n", (void*)&{n});\n'
+    src += "    return 0;\n}\n"
+    f = tmp_path / "api.cpp"
+    f.write_text(src)
+    lib = os.path.join(ROOT, "chaoslibrary_amd")
+    r = subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(f), "-o", str(tmp_path / "api"),
+                        "-L", lib, "-lchaos", "-Wl,--unresolved-symbols=ignore-in-shared-libs"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
