@@ -346,6 +346,13 @@ _SIGS_LEGACY = {
                                               _D, _I, C.c_int, C.c_double, C.c_double, _D, C.c_int, C.c_int,
                                               C.c_char_p]),
 }
+_BIF_STYLE = [C.POINTER(Options), C.c_double, C.c_int, C.c_double, C.c_int, _D, _D, _I, C.c_int, C.c_double,
+              C.c_double, _D, C.c_int, C.c_int]
+_SIGS_LEGACY["chaos_legacy_bif1d_ic"] = (C.c_int, _BIF_STYLE + [C.c_void_p, C.c_void_p, C.c_longlong,
+                                                               C.POINTER(C.c_longlong)])
+_SIGS_LEGACY["chaos_legacy_bif2d_ic"] = (C.c_int, _BIF_STYLE + [C.c_double, C.c_void_p])
+_SIGS_LEGACY["chaos_bifurcation1DIC"] = (C.c_int, _BIF_STYLE + [C.c_char_p])
+_SIGS_LEGACY["chaos_bifurcation2DIC"] = (C.c_int, _BIF_STYLE + [C.c_double, C.c_char_p])
 for _n in ("chaos_LLE1D", "chaos_LLE1DIC", "chaos_LLE2DIC", "chaos_LS1D", "chaos_LS2D"):
     _SIGS_LEGACY[_n] = (C.c_int, _LLE_STYLE + [C.c_char_p])
 for _name, (_res, _args) in _SIGS_LEGACY.items():
@@ -396,6 +403,59 @@ def time_domain(tMax, nPts, h, initialConditions, ranges, indicesOfMutVars, writ
     _check(lib.chaos_time_domain(*args, C.c_void_p(out.ctypes.data), C.c_void_p(flags.ctypes.data), out.size,
                                  C.byref(B)))
     return out, flags
+
+
+def legacy_bif1d_ic(tMax, nPts, h, initialConditions, ranges, indicesOfMutVars, writableVar, maxValue,
+                    transientTime, values, preScaller=1, system="chameleon", semantics=SEM_REFERENCE, **kw):
+    """chaos_legacy_bif1d_ic: (counts[nPts], kept peak values concatenated)."""
+    o = make_options(system, semantics, **kw)
+    ic, rg, ix, vals = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
+    counts = np.zeros(nPts, dtype=np.int32)
+    total = C.c_longlong(0)
+    args = (C.byref(o), tMax, nPts, h, ic.size, _dp(ic), _dp(rg), _ip(ix), writableVar, maxValue, transientTime,
+            _dp(vals), vals.size, preScaller, C.c_void_p(counts.ctypes.data))
+    B = max(1, int(round(tMax / h / max(1, preScaller))))
+    cap = max(1, nPts * min(B // 2 + 1, max(256, B // 32)))
+    pk = np.zeros(cap)
+    rc = lib.chaos_legacy_bif1d_ic(*args, C.c_void_p(pk.ctypes.data), cap, C.byref(total))
+    if rc == -5:
+        cap = max(1, total.value)
+        pk = np.zeros(cap)
+        rc = lib.chaos_legacy_bif1d_ic(*args, C.c_void_p(pk.ctypes.data), cap, C.byref(total))
+    _check(rc)
+    return counts, pk[:total.value]
+
+
+def legacy_bif2d_ic(tMax, nPts, h, initialConditions, ranges, indicesOfMutVars, writableVar, maxValue,
+                    transientTime, values, preScaller=1, eps=0.1, system="chameleon", semantics=SEM_REFERENCE, **kw):
+    """chaos_legacy_bif2d_ic: int map over nPts^2 initial conditions (p = k0 + nPts*k1)."""
+    o = make_options(system, semantics, **kw)
+    ic, rg, ix, vals = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
+    out = np.zeros(nPts * nPts, dtype=np.int32)
+    _check(lib.chaos_legacy_bif2d_ic(C.byref(o), tMax, nPts, h, ic.size, _dp(ic), _dp(rg), _ip(ix), writableVar,
+                                     maxValue, transientTime, _dp(vals), vals.size, preScaller, eps,
+                                     C.c_void_p(out.ctypes.data)))
+    return out
+
+
+def bifurcation1DIC(tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges, indicesOfMutVars,
+                    writableVar, maxValue, transientTime, values, amountOfValues, preScaller, OUT_FILE_PATH,
+                    options=None):
+    """old_library::bifurcation1DIC (debri/hostLibrary.cuh:109-123)."""
+    ic, rg, ix, vals = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
+    _check(lib.chaos_bifurcation1DIC(_opt_ref(options), tMax, nPts, h, amountOfInitialConditions, _dp(ic), _dp(rg),
+                                     _ip(ix), writableVar, maxValue, transientTime, _dp(vals), amountOfValues,
+                                     preScaller, str(OUT_FILE_PATH).encode()))
+
+
+def bifurcation2DIC(tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges, indicesOfMutVars,
+                    writableVar, maxValue, transientTime, values, amountOfValues, preScaller, eps, OUT_FILE_PATH,
+                    options=None):
+    """old_library::bifurcation2DIC (debri/hostLibrary.cuh:219-234)."""
+    ic, rg, ix, vals = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
+    _check(lib.chaos_bifurcation2DIC(_opt_ref(options), tMax, nPts, h, amountOfInitialConditions, _dp(ic), _dp(rg),
+                                     _ip(ix), writableVar, maxValue, transientTime, _dp(vals), amountOfValues,
+                                     preScaller, eps, str(OUT_FILE_PATH).encode()))
 
 
 def _legacy_file(fn, tMax, NT, nPts, h, eps, initialConditions, amountOfInitialConditions, ranges,

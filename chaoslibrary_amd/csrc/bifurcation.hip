@@ -75,9 +75,10 @@ SweepArgs make_bif_args(const chaos_options& o, const BifInputs& in)
     CHAOS_REQUIRE(in.h > 0 && std::isfinite(in.h), "h must be positive");
     CHAOS_REQUIRE(in.preScaller >= 1, "preScaller must be >= 1");
     CHAOS_REQUIRE(in.ic && in.ranges && in.values && in.idx, "NULL input array");
+    const int lim = in.legacy_ic ? in.nx : in.na;
     for (int d = 0; d < in.dims; ++d)
-        CHAOS_REQUIRE(in.idx[d] >= 0 && in.idx[d] < in.na, "indicesOfMutVars[%d]=%d out of range [0,%d)", d,
-                      in.idx[d], in.na);
+        CHAOS_REQUIRE(in.idx[d] >= 0 && in.idx[d] < lim, "indicesOfMutVars[%d]=%d out of range [0,%d)", d,
+                      in.idx[d], lim);
     CHAOS_REQUIRE(in.writableVar >= 0 && in.writableVar < in.nx, "writableVar out of range");
     const long long n = in.dims == 2 ? (long long)in.nPts * in.nPts : in.nPts;
     CHAOS_REQUIRE(n <= (1LL << 31) - 64, "grid too large");
@@ -95,6 +96,14 @@ SweepArgs make_bif_args(const chaos_options& o, const BifInputs& in)
     CHAOS_REQUIRE(A.B >= 1, "tMax/h/preScaller must give at least one recorded sample");
     A.pre = 1;  // Bifurcation steps once per sample (bifurcationCUDA.cu:435-443)
     A.check = o.semantics == CHAOS_SEM_REFERENCE ? CHK_BIF_REF : CHK_CONFIG;
+    if (in.legacy_ic) {
+        // the legacy loop (debri/cudaLibrary.cu:604-680) steps preScaller
+        // times per sample and tests isnan/isinf and fabsf(x) > maxValue
+        A.pre = in.preScaller;
+        A.check = o.semantics == CHAOS_SEM_REFERENCE ? CHK_BASINS_REF : CHK_CONFIG;
+        A.ic_sweep = 1;
+        A.legacy_transient = 1;
+    }
     A.maxValue = in.maxValue;
     A.L = divergence_fast_bound(A.check, in.maxValue);
     A.hp = in.h * (double)in.preScaller;  // bifurcationCUDA.cu:196,726
@@ -442,9 +451,11 @@ void bif1d_collect(CallCtx& ctx, BifDevice& R, int* counts, double* first, doubl
 
 static BifInputs bif_inputs(double tMax, int nPts, double h, int nx, const double* ic, const double* ranges,
                             const int* idx, int writableVar, double maxValue, double transientTime,
-                            const double* values, int na, int preScaller, double eps, int dims)
+                            const double* values, int na, int preScaller, double eps, int dims,
+                            int legacy_ic = 0)
 {
     BifInputs in{};
+    in.legacy_ic = legacy_ic;
     in.tMax = tMax;
     in.nPts = nPts;
     in.h = h;
@@ -467,6 +478,84 @@ static BifInputs bif_inputs(double tMax, int nPts, double h, int nx, const doubl
 
 using namespace chaos;
 
+namespace chaos {
+
+static void bif1d_impl(const chaos_options* opt, double tMax, int nPts, double h, int nIC, const double* ic,
+                       const double* ranges, const int* idx, int writableVar, double maxValue, double transientTime,
+                       const double* values, int nV, int preScaller, int* counts, double* first, double* peaks,
+                       double* intervals, long long capacity, long long* total, int legacy_ic)
+{
+    chaos_options o = resolve_options(opt);
+    CHAOS_REQUIRE(counts && first && total, "NULL output");
+    CHAOS_REQUIRE(capacity == 0 || (peaks && intervals), "NULL peaks/intervals with capacity > 0");
+    BifInputs in = bif_inputs(tMax, nPts, h, nIC, ic, ranges, idx, writableVar, maxValue, transientTime, values, nV,
+                              preScaller, 0.0, 1, legacy_ic);
+    CallCtx ctx(o);
+    BifDevice R;
+    run_bif(ctx, o, in, R);
+    bif1d_collect(ctx, R, counts, first, peaks, intervals, capacity, total);
+    const long long k_out = *total;
+    ctx.finish();
+    if (k_out > capacity) fail(CHAOS_E_CAPACITY, "need capacity %lld, got %lld", k_out, capacity);
+}
+
+static void bif2d_impl(const chaos_options* opt, double tMax, int nPts, double h, int nIC, const double* ic,
+                       const double* ranges, const int* idx, int writableVar, double maxValue, double transientTime,
+                       const double* values, int nV, int preScaller, double eps, int* out, int legacy_ic)
+{
+    chaos_options o = resolve_options(opt);
+    CHAOS_REQUIRE(out, "out is NULL");
+    BifInputs in = bif_inputs(tMax, nPts, h, nIC, ic, ranges, idx, writableVar, maxValue, transientTime, values, nV,
+                              preScaller, eps, 2, legacy_ic);
+    CallCtx ctx(o);
+    BifDevice R;
+    run_bif(ctx, o, in, R);
+    DevBuf<int> tmp;
+    int* d_out = out;
+    if (!o.device_buffers) {
+        tmp.alloc((size_t)R.n);
+        d_out = tmp.get();
+    }
+    bif2d_analysis(ctx, o.semantics, R, eps, d_out);
+    if (!o.device_buffers)
+        CHAOS_HIP(hipMemcpyAsync(out, d_out, R.n * sizeof(int), hipMemcpyDeviceToHost, ctx.stream()));
+    ctx.sync();
+    ctx.finish();
+}
+
+// Host results of a 1-D sweep in one pass when the peak buffers are large
+// enough (one peak per 32 samples), else a second, exactly sized pass.
+static long long bif1d_host(const chaos_options* opt, double tMax, int nPts, double h, int nIC, const double* ic,
+                            const double* ranges, const int* idx, int writableVar, double maxValue,
+                            double transientTime, const double* values, int nV, int preScaller,
+                            std::vector<int>& counts, std::vector<double>& first, std::vector<double>& pk,
+                            std::vector<double>& iv, int legacy_ic)
+{
+    const chaos_options o = resolve_options(opt);
+    counts.assign(nPts, 0);
+    first.assign(nPts, 0.0);
+    const long long B = std::max(1, record_samples(tMax, h, std::max(1, preScaller), o.semantics));
+    long long cap = (long long)nPts * std::min(B / 2 + 1, std::max(256LL, B / 32));
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        pk.assign((size_t)std::max(1LL, cap), 0.0);
+        iv.assign((size_t)std::max(1LL, cap), 0.0);
+        long long total = 0;
+        try {
+            bif1d_impl(opt, tMax, nPts, h, nIC, ic, ranges, idx, writableVar, maxValue, transientTime, values, nV,
+                       preScaller, counts.data(), first.data(), pk.data(), iv.data(), cap, &total, legacy_ic);
+            pk.resize(total);
+            iv.resize(total);
+            return total;
+        } catch (const Error& e) {
+            if (e.code != CHAOS_E_CAPACITY || attempt == 1) throw;
+            cap = total;
+        }
+    }
+    return 0;
+}
+
+}  // namespace chaos
+
 extern "C" int chaos_bif1d(const chaos_options* opt, double tMax, int nPts, double h, int amountOfInitialConditions,
                            const double* initialConditions, const double* ranges, const int* indicesOfMutVars,
                            int writableVar, double maxValue, double transientTime, const double* values,
@@ -474,19 +563,56 @@ extern "C" int chaos_bif1d(const chaos_options* opt, double tMax, int nPts, doub
                            double* intervals, long long capacity, long long* total)
 {
     return guarded("chaos_bif1d", [&] {
-        chaos_options o = resolve_options(opt);
-        CHAOS_REQUIRE(counts && first && total, "NULL output");
-        CHAOS_REQUIRE(capacity == 0 || (peaks && intervals), "NULL peaks/intervals with capacity > 0");
-        BifInputs in = bif_inputs(tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges,
-                                  indicesOfMutVars, writableVar, maxValue, transientTime, values, amountOfValues,
-                                  preScaller, 0.0, 1);
-        CallCtx ctx(o);
-        BifDevice R;
-        run_bif(ctx, o, in, R);
-        bif1d_collect(ctx, R, counts, first, peaks, intervals, capacity, total);
-        const long long k_out = *total;
-        ctx.finish();
-        if (k_out > capacity) fail(CHAOS_E_CAPACITY, "need capacity %lld, got %lld", k_out, capacity);
+        bif1d_impl(opt, tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges, indicesOfMutVars,
+                   writableVar, maxValue, transientTime, values, amountOfValues, preScaller, counts, first, peaks,
+                   intervals, capacity, total, 0);
+    });
+}
+
+extern "C" int chaos_legacy_bif1d_ic(const chaos_options* opt, double tMax, int nPts, double h,
+                                     int amountOfInitialConditions, const double* initialConditions,
+                                     const double* ranges, const int* indicesOfMutVars, int writableVar,
+                                     double maxValue, double transientTime, const double* values,
+                                     int amountOfValues, int preScaller, int* counts, double* peaks,
+                                     long long capacity, long long* total)
+{
+    return guarded("chaos_legacy_bif1d_ic", [&] {
+        CHAOS_REQUIRE(nPts >= 2, "nPts must be >= 2");
+        std::vector<double> first(nPts), iv((size_t)std::max(1LL, capacity));
+        bif1d_impl(opt, tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges, indicesOfMutVars,
+                   writableVar, maxValue, transientTime, values, amountOfValues, preScaller, counts, first.data(),
+                   peaks, capacity > 0 ? iv.data() : nullptr, capacity, total, 1);
+    });
+}
+
+extern "C" int chaos_bifurcation1DIC(const chaos_options* opt, double tMax, int nPts, double h,
+                                     int amountOfInitialConditions, const double* initialConditions,
+                                     const double* ranges, const int* indicesOfMutVars, int writableVar,
+                                     double maxValue, double transientTime, const double* values,
+                                     int amountOfValues, int preScaller, const char* OUT_FILE_PATH)
+{
+    return guarded("chaos_bifurcation1DIC", [&] {
+        CHAOS_REQUIRE(OUT_FILE_PATH, "OUT_FILE_PATH is NULL");
+        CHAOS_REQUIRE(nPts >= 2, "nPts must be >= 2");
+        TextFile f(OUT_FILE_PATH);
+        std::vector<int> counts;
+        std::vector<double> first, pk, iv;
+        bif1d_host(opt, tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges, indicesOfMutVars,
+                   writableVar, maxValue, transientTime, values, amountOfValues, preScaller, counts, first, pk, iv,
+                   1);
+        // debri/hostLibrary.cu:1447-1456: "<ic>, <peak>\n" per kept peak at
+        // precision 12, nothing for fixed points or diverged points, no header
+        long long k = 0;
+        for (int p = 0; p < nPts; ++p) {
+            const double x = grid_value(p, nPts, ranges[0], ranges[1]);
+            for (int j = 0; j < counts[p]; ++j, ++k) {
+                f.put_g(x, 12);
+                f.put(", ", 2);
+                f.put_g(pk[k], 12);
+                f.put("\n", 1);
+            }
+        }
+        f.close();
     });
 }
 
@@ -500,20 +626,11 @@ extern "C" int chaos_bifurcation1D(const chaos_options* opt, double tMax, int nP
         CHAOS_REQUIRE(OUT_FILE_PATH, "OUT_FILE_PATH is NULL");
         CHAOS_REQUIRE(nPts >= 1, "nPts must be >= 1");
         TextFile f(OUT_FILE_PATH);
-        std::vector<int> counts(nPts);
-        std::vector<double> first(nPts);
-        long long total = 0;
-        int rc = chaos_bif1d(opt, tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges,
-                             indicesOfMutVars, writableVar, maxValue, transientTime, values, amountOfValues,
-                             preScaller, counts.data(), first.data(), nullptr, nullptr, 0, &total);
-        if (rc != CHAOS_OK && rc != CHAOS_E_CAPACITY) throw Error(rc, chaos_last_error());
-        std::vector<double> pk((size_t)std::max(1LL, total)), iv((size_t)std::max(1LL, total));
-        if (total > 0) {
-            rc = chaos_bif1d(opt, tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges,
-                             indicesOfMutVars, writableVar, maxValue, transientTime, values, amountOfValues,
-                             preScaller, counts.data(), first.data(), pk.data(), iv.data(), total, &total);
-            if (rc != CHAOS_OK) throw Error(rc, chaos_last_error());
-        }
+        std::vector<int> counts;
+        std::vector<double> first, pk, iv;
+        bif1d_host(opt, tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges, indicesOfMutVars,
+                   writableVar, maxValue, transientTime, values, amountOfValues, preScaller, counts, first, pk, iv,
+                   0);
         // header at default precision, body at 12 digits (:645-647,746-763)
         write_range_header(f, ranges, 1, 6);
         long long k = 0;
@@ -545,25 +662,44 @@ extern "C" int chaos_bif2d(const chaos_options* opt, double tMax, int nPts, doub
                            int amountOfValues, int preScaller, double eps, int* out)
 {
     return guarded("chaos_bif2d", [&] {
+        bif2d_impl(opt, tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges, indicesOfMutVars,
+                   writableVar, maxValue, transientTime, values, amountOfValues, preScaller, eps, out, 0);
+    });
+}
+
+extern "C" int chaos_legacy_bif2d_ic(const chaos_options* opt, double tMax, int nPts, double h,
+                                     int amountOfInitialConditions, const double* initialConditions,
+                                     const double* ranges, const int* indicesOfMutVars, int writableVar,
+                                     double maxValue, double transientTime, const double* values,
+                                     int amountOfValues, int preScaller, double eps, int* out)
+{
+    return guarded("chaos_legacy_bif2d_ic", [&] {
+        CHAOS_REQUIRE(nPts >= 2, "nPts must be >= 2");
+        bif2d_impl(opt, tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges, indicesOfMutVars,
+                   writableVar, maxValue, transientTime, values, amountOfValues, preScaller, eps, out, 1);
+    });
+}
+
+extern "C" int chaos_bifurcation2DIC(const chaos_options* opt, double tMax, int nPts, double h,
+                                     int amountOfInitialConditions, const double* initialConditions,
+                                     const double* ranges, const int* indicesOfMutVars, int writableVar,
+                                     double maxValue, double transientTime, const double* values,
+                                     int amountOfValues, int preScaller, double eps, const char* OUT_FILE_PATH)
+{
+    return guarded("chaos_bifurcation2DIC", [&] {
+        CHAOS_REQUIRE(OUT_FILE_PATH, "OUT_FILE_PATH is NULL");
+        CHAOS_REQUIRE(nPts >= 2, "nPts must be >= 2");
         chaos_options o = resolve_options(opt);
-        CHAOS_REQUIRE(out, "out is NULL");
-        BifInputs in = bif_inputs(tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges,
-                                  indicesOfMutVars, writableVar, maxValue, transientTime, values, amountOfValues,
-                                  preScaller, eps, 2);
-        CallCtx ctx(o);
-        BifDevice R;
-        run_bif(ctx, o, in, R);
-        DevBuf<int> tmp;
-        int* d_out = out;
-        if (!o.device_buffers) {
-            tmp.alloc((size_t)R.n);
-            d_out = tmp.get();
-        }
-        bif2d_analysis(ctx, o.semantics, R, eps, d_out);
-        if (!o.device_buffers)
-            CHAOS_HIP(hipMemcpyAsync(out, d_out, R.n * sizeof(int), hipMemcpyDeviceToHost, ctx.stream()));
-        ctx.sync();
-        ctx.finish();
+        o.device_buffers = 0;
+        TextFile f(OUT_FILE_PATH);
+        std::vector<int> res((size_t)nPts * nPts);
+        bif2d_impl(&o, tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges, indicesOfMutVars,
+                   writableVar, maxValue, transientTime, values, amountOfValues, preScaller, eps, res.data(), 1);
+        // debri/hostLibrary.cu:2934-2935, 3060-3073: two range lines at
+        // default precision, then the map
+        write_range_header(f, ranges, 2, 6);
+        write_map_int(f, res.data(), (long long)nPts * nPts, nPts);
+        f.close();
     });
 }
 

@@ -13,6 +13,7 @@ namespace chaos {
 struct BifInputs {
     double tMax, h, maxValue, transientTime, eps;
     int nPts, nx, na, writableVar, preScaller, dims;
+    int legacy_ic;  // legacy bifurcation1DIC / 2DIC: initial conditions swept, legacy loop semantics
     const double *ic, *ranges, *values;
     const int* idx;
 };

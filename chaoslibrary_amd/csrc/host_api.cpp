@@ -83,6 +83,26 @@ void TimeDomainCalculation(const double tMax, const int nPts, const double h, co
                                       preScaller, OUT_FILE_PATH.c_str()));
 }
 
+void bifurcation1DIC(const double tMax, const int nPts, const double h, const int amountOfInitialConditions,
+                     const double* initialConditions, const double* ranges, const int* indicesOfMutVars,
+                     const int writableVar, const double maxValue, const double transientTime, const double* values,
+                     const int amountOfValues, const int preScaller, std::string OUT_FILE_PATH)
+{
+    check(chaos_bifurcation1DIC(nullptr, tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges,
+                                indicesOfMutVars, writableVar, maxValue, transientTime, values, amountOfValues,
+                                preScaller, OUT_FILE_PATH.c_str()));
+}
+
+void bifurcation2DIC(const double tMax, const int nPts, const double h, const int amountOfInitialConditions,
+                     const double* initialConditions, const double* ranges, const int* indicesOfMutVars,
+                     const int writableVar, const double maxValue, const double transientTime, const double* values,
+                     const int amountOfValues, const int preScaller, const double eps, std::string OUT_FILE_PATH)
+{
+    check(chaos_bifurcation2DIC(nullptr, tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges,
+                                indicesOfMutVars, writableVar, maxValue, transientTime, values, amountOfValues,
+                                preScaller, eps, OUT_FILE_PATH.c_str()));
+}
+
 #define CHAOS_LEGACY_LLE_STYLE(NAME)                                                                              \
     void NAME(const double tMax, const double NT, const int nPts, const double h, const double eps,                \
               const double* initialConditions, const int amountOfInitialConditions, const double* ranges,          \

@@ -195,7 +195,7 @@ struct SweepProg {
     // Parameter vector of point pq.
     __device__ __forceinline__ void params(const SweepArgs& A, long long pq)
     {
-        if (KIND == KIND_BIF) {
+        if (KIND == KIND_BIF && !A.ic_sweep) {
             // swept entries of the parameter vector (bifurcationCUDA.cu:282-284)
             double g0, g1;
             grid(A, pq, g0, g1);
@@ -209,11 +209,12 @@ struct SweepProg {
     __device__ __forceinline__ void init(const SweepArgs& A, long long pq, bool in_range)
     {
         params(A, pq);
-        if (KIND == KIND_BIF) {
+        if (KIND == KIND_BIF && !A.ic_sweep) {
 #pragma unroll
             for (int k = 0; k < NX; ++k) X[k] = A.ic[k];
         } else {
-            // swept entries of the initial condition (basinsCUDA.cu:253-262)
+            // swept entries of the initial condition (basinsCUDA.cu:253-262,
+            // debri/cudaLibrary.cu:1022-1062)
             double g0, g1;
             grid(A, pq, g0, g1);
 #pragma unroll
@@ -302,7 +303,20 @@ struct SweepProg {
     {
         long long g = g0;
         // ---- transient ----
-        if (KIND == KIND_BIF) {
+        if (KIND == KIND_BIF && A.legacy_transient) {
+            // legacy IC sweeps (debri/cudaLibrary.cu:1063-1070): the transient
+            // tests X[0] for NaN/inf only (maxValue 0) and a fixed point at its
+            // end is still recorded
+            if (g < A.S) {
+                const long long e = g1 < A.S ? g1 : A.S;
+                for (long long s = g; s < e; ++s) {
+                    Sy::step(X, a, A.sc);
+                    const bool bad = alive && !isfinite(X[0]);
+                    if (__ballot(bad) && bad) alive = false;
+                }
+                g = e;
+            }
+        } else if (KIND == KIND_BIF) {
             // checked every step; a transient that ends diverged or at a
             // fixed point is not recorded and reports 0 (bifurcationCUDA.cu:334-342)
             if (g < A.S - 1) {

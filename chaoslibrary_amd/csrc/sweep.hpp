@@ -35,6 +35,8 @@ struct SweepArgs {
     double maxValue;      // divergence bound (0 = off)
     double L;             // |v| < L guarantees "not diverged" (fast path)
     double hp;            // interval unit: h * preScaller
+    int ic_sweep;         // Bifurcation: swept entries are initial conditions (legacy *IC sweeps)
+    int legacy_transient; // Bifurcation: transient checked on X[0] for NaN/inf only, a fixed point still records
     StepConst sc;
     // outputs (per point)
     int* flags;

@@ -292,6 +292,45 @@ int chaos_LS2D(const chaos_options* opt, double tMax, double NT, int nPts,
                const int* indicesOfMutVars, int writableVar, double maxValue,
                double transientTime, const double* values, int amountOfValues,
                const char* OUT_FILE_PATH);
+/* Bifurcation over initial conditions (old_library::bifurcation1DIC,
+ * debri/hostLibrary.cuh:109-123; bifurcation2DIC, :219-234).  The swept
+ * entries are initialConditions[indicesOfMutVars[d]]; the transient
+ * (int(transientTime/h) single steps) tests X[0] for NaN/inf only and a
+ * fixed point at its end is still recorded; the record runs preScaller
+ * steps per sample with the legacy checks (isnan/isinf, fabsf > maxValue).
+ * chaos_legacy_bif1d_ic: counts[p] = kept peaks of flag-1 points, else the
+ * flag (0 or -1); peaks concatenated (capacity / *total as chaos_bif1d).
+ * chaos_legacy_bif2d_ic: out[p] as chaos_bif2d (REFERENCE: the per-point
+ * dbscan, debri/cudaLibrary.cu:1593-1666). */
+int chaos_legacy_bif1d_ic(const chaos_options* opt, double tMax, int nPts,
+                          double h, int amountOfInitialConditions,
+                          const double* initialConditions, const double* ranges,
+                          const int* indicesOfMutVars, int writableVar,
+                          double maxValue, double transientTime,
+                          const double* values, int amountOfValues,
+                          int preScaller, int* counts, double* peaks,
+                          long long capacity, long long* total);
+int chaos_legacy_bif2d_ic(const chaos_options* opt, double tMax, int nPts,
+                          double h, int amountOfInitialConditions,
+                          const double* initialConditions, const double* ranges,
+                          const int* indicesOfMutVars, int writableVar,
+                          double maxValue, double transientTime,
+                          const double* values, int amountOfValues,
+                          int preScaller, double eps, int* out);
+int chaos_bifurcation1DIC(const chaos_options* opt, double tMax, int nPts,
+                          double h, int amountOfInitialConditions,
+                          const double* initialConditions, const double* ranges,
+                          const int* indicesOfMutVars, int writableVar,
+                          double maxValue, double transientTime,
+                          const double* values, int amountOfValues,
+                          int preScaller, const char* OUT_FILE_PATH);
+int chaos_bifurcation2DIC(const chaos_options* opt, double tMax, int nPts,
+                          double h, int amountOfInitialConditions,
+                          const double* initialConditions, const double* ranges,
+                          const int* indicesOfMutVars, int writableVar,
+                          double maxValue, double transientTime,
+                          const double* values, int amountOfValues,
+                          int preScaller, double eps, const char* OUT_FILE_PATH);
 /* Recorded samples X[writableVar] of a 1-D parameter sweep: out[p*B + t],
  * B = int(tMax/h/preScaller) (*samples); the transient runs
  * int(transientTime/h) iterations of preScaller steps; flags[p] in

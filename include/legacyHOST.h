@@ -16,6 +16,18 @@ void TimeDomainCalculation(const double tMax, const int nPts, const double h, co
                            const double* values, const int amountOfValues, const int preScaller,
                            std::string OUT_FILE_PATH);
 
+// debri/hostLibrary.cuh:109-123
+void bifurcation1DIC(const double tMax, const int nPts, const double h, const int amountOfInitialConditions,
+                     const double* initialConditions, const double* ranges, const int* indicesOfMutVars,
+                     const int writableVar, const double maxValue, const double transientTime, const double* values,
+                     const int amountOfValues, const int preScaller, std::string OUT_FILE_PATH);
+
+// debri/hostLibrary.cuh:219-234
+void bifurcation2DIC(const double tMax, const int nPts, const double h, const int amountOfInitialConditions,
+                     const double* initialConditions, const double* ranges, const int* indicesOfMutVars,
+                     const int writableVar, const double maxValue, const double transientTime, const double* values,
+                     const int amountOfValues, const int preScaller, const double eps, std::string OUT_FILE_PATH);
+
 // debri/hostLibrary.cuh:241-256
 void LLE1D(const double tMax, const double NT, const int nPts, const double h, const double eps,
            const double* initialConditions, const int amountOfInitialConditions, const double* ranges,

@@ -1185,3 +1185,100 @@ long long orc_time_domain(int sys, int sem, double tMax, int nPts, double h, con
     }
     return B;
 }
+
+/* One legacy IC-sweep bifurcation point: debri/cudaLibrary.cu:1022-1062
+ * (calculateDiscreteModelICCUDA) with the legacy loop (:604-680).  The
+ * transient, int(transientTime/h) single steps, tests X[0] for NaN/inf only
+ * (maxValue 0); a flag of 1 or -1 records B samples of X[writableVar] with
+ * preScaller steps each and the full checks.  Returns the flag. */
+static int legacy_ic_point(int sys, int sem, double* X, const double* a, double h, int S, int B,
+                           int preScaller, int writableVar, double maxValue, double* data)
+{
+    int chk = sem == ORC_SEM_REFERENCE ? ORC_CHK_BASINS_REF : ORC_CHK_CONFIG;
+    int flag = orc_loop(sys, X, a, h, S, 1, 0, 0.0, chk, NULL);
+    if (flag != 1 && flag != -1) return 0;
+    return orc_loop(sys, X, a, h, B, preScaller, writableVar, maxValue, chk, data);
+}
+
+/* debri/hostLibrary.cu:1243-1492 (bifurcation1DIC, values not the CSV):
+ * counts[p] = kept peaks (peakFinder, first dropped) of flag-1 points, else
+ * the flag (0 or -1, which the legacy writer skips); kept peak values
+ * concatenated.  Returns the total or -1 if capacity is too small.
+ * Pinned by: the IC sweep equals the parameter-sweep driver on a system
+ * where a parameter enters only through the state (tests). */
+long long orc_legacy_bif1d_ic(int sys, int sem, double tMax, int nPts, double h, const double* ic, int nx,
+                              const double* ranges, const int* idx, int writableVar, double maxValue,
+                              double transientTime, const double* values, int na, int preScaller,
+                              int* counts, double* peaks, long long capacity)
+{
+    int snx, sna;
+    if (orc_dims(sys, &snx, &sna) || nx != snx || na != sna || nPts < 2 || preScaller < 1) return -2;
+    int B = record_len(tMax, h, preScaller, sem);
+    int S = horizon(transientTime, h, sem);
+    if (B < 1) return -2;
+    double* data = (double*)malloc(sizeof(double) * (size_t)B * 3);
+    if (!data) return -3;
+    long long total = 0;
+    for (int p = 0; p < nPts; ++p) {
+        double X[ORC_MAX_NX];
+        for (int i = 0; i < nx; ++i) X[i] = ic[i];
+        X[idx[0]] = orc_grid_value_pow(p, nPts, ranges[0], ranges[1], 0);
+        int flag = legacy_ic_point(sys, sem, X, values, h, S, B, preScaller, writableVar, maxValue, data);
+        if (flag != 1) {
+            counts[p] = flag;
+            continue;
+        }
+        int P = orc_peak_finder(data, B, data + B, data + 2 * (size_t)B, h * (double)preScaller);
+        counts[p] = P;
+        for (int k = 0; k < P; ++k, ++total)
+            if (total < capacity) peaks[total] = data[B + k];
+    }
+    free(data);
+    return total > capacity ? -1 : total;
+}
+
+/* debri/hostLibrary.cu:2819-3109 (bifurcation2DIC, values not the CSV):
+ * out[p] = -1 / 0 for flags -1 / 0, else the quirky per-point dbscan
+ * (debri/cudaLibrary.cu:1593-1666, the same text as
+ * Bifurcation/src/bifurcationCUDA.cu:516-572) of the kept peaks and
+ * intervals (REFERENCE) or the 1-D DBSCAN count of the kept peaks (CONFIG). */
+int orc_legacy_bif2d_ic(int sys, int sem, double tMax, int nPts, double h, const double* ic, int nx,
+                        const double* ranges, const int* idx, int writableVar, double maxValue,
+                        double transientTime, const double* values, int na, int preScaller, double eps,
+                        int* out)
+{
+    int snx, sna;
+    if (orc_dims(sys, &snx, &sna) || nx != snx || na != sna || nPts < 2 || preScaller < 1) return -2;
+    int B = record_len(tMax, h, preScaller, sem);
+    int S = horizon(transientTime, h, sem);
+    if (B < 1) return -2;
+    int err = 0;
+#pragma omp parallel
+    {
+        double* data = (double*)malloc(sizeof(double) * (size_t)B * 4);
+        if (!data) {
+#pragma omp atomic write
+            err = -3;
+        }
+#pragma omp for schedule(dynamic, 4)
+        for (long long p = 0; p < (long long)nPts * nPts; ++p) {
+            if (!data) continue;
+            double X[ORC_MAX_NX];
+            for (int i = 0; i < nx; ++i) X[i] = ic[i];
+            X[idx[0]] = orc_grid_value_pow((int)p, nPts, ranges[0], ranges[1], 0);
+            X[idx[1]] = orc_grid_value_pow((int)p, nPts, ranges[2], ranges[3], 1);
+            int flag = legacy_ic_point(sys, sem, X, values, h, S, B, preScaller, writableVar, maxValue, data);
+            int r = flag;
+            if (flag == 1) {
+                double* pk = data + B;
+                double* iv = data + 2 * (size_t)B;
+                int P = orc_peak_finder(data, B, pk, iv, h * (double)preScaller);
+                r = sem == ORC_SEM_REFERENCE ? orc_dbscan_ref(pk, iv, data + 3 * (size_t)B, P, B, eps)
+                                             : orc_dbscan1d_count(pk, P, eps);
+            }
+            out[p] = r;
+        }
+        free(data);
+    }
+    return err;
+}

@@ -130,6 +130,15 @@ long long orc_time_domain(int sys, int sem, double tMax, int nPts, double h, con
                           double transientTime, const double* values, int na, int preScaller,
                           double* out, int* flags);
 
+long long orc_legacy_bif1d_ic(int sys, int sem, double tMax, int nPts, double h, const double* ic, int nx,
+                              const double* ranges, const int* idx, int writableVar, double maxValue,
+                              double transientTime, const double* values, int na, int preScaller,
+                              int* counts, double* peaks, long long capacity);
+int orc_legacy_bif2d_ic(int sys, int sem, double tMax, int nPts, double h, const double* ic, int nx,
+                        const double* ranges, const int* idx, int writableVar, double maxValue,
+                        double transientTime, const double* values, int na, int preScaller, double eps,
+                        int* out);
+
 /* CPU baseline: LLE rows over OpenMP threads; returns threads used. */
 int orc_lle2d_omp(int sys, int sem, double tMax, double NT, int nPts, double h,
                   double eps, const double* ic, int nx, const double* ranges,

@@ -76,6 +76,12 @@ def _load():
         "orc_time_domain": (C.c_longlong, [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, _D, C.c_int,
                                            _D, _I, C.c_int, C.c_double, C.c_double, _D, C.c_int, C.c_int,
                                            _D, _I]),
+        "orc_legacy_bif1d_ic": (C.c_longlong, [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, _D, C.c_int,
+                                               _D, _I, C.c_int, C.c_double, C.c_double, _D, C.c_int, C.c_int,
+                                               _I, _D, C.c_longlong]),
+        "orc_legacy_bif2d_ic": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, _D, C.c_int, _D,
+                                          _I, C.c_int, C.c_double, C.c_double, _D, C.c_int, C.c_int, C.c_double,
+                                          _I]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -304,3 +310,24 @@ def time_domain(sysid, sem, tMax, nPts, h, ic, ranges, idx, wv, maxv, tT, values
                               tT, dptr(values), values.size, pre, dptr(out), iptr(flags))
     assert got == B, (got, B)
     return out.reshape(nPts, B), flags
+
+
+def legacy_bif1d_ic(sysid, sem, tMax, nPts, h, ic, ranges, idx, wv, maxv, tT, values, pre, capacity=1 << 22):
+    ic, ranges, values = _arr(ic), _arr(ranges), _arr(values)
+    idx = _arr(idx, np.int32)
+    counts = np.zeros(nPts, dtype=np.int32)
+    peaks = np.zeros(capacity)
+    total = lib.orc_legacy_bif1d_ic(sysid, sem, tMax, nPts, h, dptr(ic), ic.size, dptr(ranges), iptr(idx), wv,
+                                    maxv, tT, dptr(values), values.size, pre, iptr(counts), dptr(peaks), capacity)
+    assert total >= 0, total
+    return counts, peaks[:total]
+
+
+def legacy_bif2d_ic(sysid, sem, tMax, nPts, h, ic, ranges, idx, wv, maxv, tT, values, pre, eps):
+    ic, ranges, values = _arr(ic), _arr(ranges), _arr(values)
+    idx = _arr(idx, np.int32)
+    out = np.zeros(nPts * nPts, dtype=np.int32)
+    rc = lib.orc_legacy_bif2d_ic(sysid, sem, tMax, nPts, h, dptr(ic), ic.size, dptr(ranges), iptr(idx), wv, maxv,
+                                 tT, dptr(values), values.size, pre, eps, iptr(out))
+    assert rc == 0, rc
+    return out

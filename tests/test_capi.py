@@ -68,7 +68,8 @@ def test_cpp_api_links(chaos, tmp_path):
         pytest.skip("g++ not available")
     fns = {"bifurcationHOST.h": ["Bifurcation::bifurcation1D", "Bifurcation::bifurcation2D"],
            "LLEHost.h": ["LLE::LLE2D"], "basinsHOST.h": ["Basins::basinsOfAttraction_2"],
-           "legacyHOST.h": ["old_library::TimeDomainCalculation", "old_library::LLE1D", "old_library::LLE1DIC",
+           "legacyHOST.h": ["old_library::TimeDomainCalculation", "old_library::bifurcation1DIC",
+                            "old_library::bifurcation2DIC", "old_library::LLE1D", "old_library::LLE1DIC",
                             "old_library::LLE2DIC", "old_library::LS1D", "old_library::LS2D"]}
     src = "".join(f'#include "{h}"\n' for h in fns) + "#include <cstdio>\nint main() {\n"
     for names in fns.values():

@@ -100,3 +100,55 @@ def test_legacy_files(chaos, tmp_path):
     chaos.TimeDomainCalculation(0.2, 3, 0.01, 3, ic, [4.0, 6.0], [2], 0, 0.0, 0.5, vals, 3, 1, str(f), options=opt)
     td, _ = chaos.time_domain(0.2, 3, 0.01, ic, [4.0, 6.0], [2], 0, 0.0, 0.5, vals, 1, system="rossler_rk4")
     assert f.read_text() == "".join("".join("%.16g, " % x for x in row) + " \n" for row in td)
+
+
+@pytest.mark.parametrize("pre,maxv", [(1, 0.0), (2, 0.0), (1, 18.0)])
+def test_legacy_bif1d_ic_matches_oracle(chaos, oracle, pre, maxv):
+    print(f"legacy bif1DIC pre={pre} maxValue={maxv}", flush=True)
+    args = (30.0, 40, 0.01, LOR["ic"], [-15.0, 15.0], [0], 0, maxv, 5.0, LOR["values"], pre)
+    counts, peaks = chaos.legacy_bif1d_ic(*args, system="lorenz_rk4")
+    oc, op = oracle.legacy_bif1d_ic(oracle.SYS["lorenz_rk4"], 0, *args)
+    assert np.array_equal(counts, oc)
+    assert np.array_equal(peaks, op)
+    assert peaks.size > 0
+
+
+def test_legacy_bif1d_ic_fixed_points(chaos, oracle):
+    # rho = 0.5: starts at the origin are recorded and end as fixed points
+    vals = [10.0, 0.5, 8 / 3]
+    counts, peaks = chaos.legacy_bif1d_ic(5.0, 4, 0.01, [0.0, 0.0, 0.0], [0.0, 0.0], [0], 0, 0.0, 1.0, vals,
+                                          system="lorenz_rk4")
+    assert list(counts) == [-1] * 4 and peaks.size == 0
+
+
+@pytest.mark.parametrize("sem", [0, 1])
+def test_legacy_bif2d_ic_matches_oracle(chaos, oracle, sem):
+    print(f"legacy bif2DIC sem={sem}", flush=True)
+    args = (20.0, 12, 0.01, [0.0, 0.0, 25.0], [-20.0, 20.0, -20.0, 20.0], [0, 1], 0, 0.0, 10.0,
+            [10.0, 24.5, 8 / 3], 1)
+    got = chaos.legacy_bif2d_ic(*args, eps=0.05, system="lorenz_rk4", semantics=sem)
+    want = oracle.legacy_bif2d_ic(oracle.SYS["lorenz_rk4"], sem, *args, 0.05)
+    assert np.array_equal(got, want)
+
+
+def test_legacy_bifurcation_ic_files(chaos, tmp_path):
+    opt = chaos.make_options("lorenz_rk4", 0)
+    ic, vals = LOR["ic"], LOR["values"]
+    f = tmp_path / "b1dic.csv"
+    chaos.bifurcation1DIC(10.0, 5, 0.01, 3, ic, [-5.0, 5.0], [0], 0, 0.0, 2.0, vals, 3, 1, str(f), options=opt)
+    counts, peaks = chaos.legacy_bif1d_ic(10.0, 5, 0.01, ic, [-5.0, 5.0], [0], 0, 0.0, 2.0, vals,
+                                          system="lorenz_rk4")
+    lines, k = [], 0
+    for p in range(5):
+        x = -5.0 + p * ((5.0 - -5.0) / 4)
+        for _ in range(max(0, counts[p])):
+            lines.append("%.12g, %.12g\n" % (x, peaks[k]))
+            k += 1
+    assert f.read_text() == "".join(lines) and k > 0
+    f = tmp_path / "b2dic.csv"
+    rg = [-10.0, 10.0, -10.0, 10.0]
+    chaos.bifurcation2DIC(10.0, 4, 0.01, 3, ic, rg, [0, 1], 0, 0.0, 2.0, vals, 3, 1, 0.05, str(f), options=opt)
+    m = chaos.legacy_bif2d_ic(10.0, 4, 0.01, ic, rg, [0, 1], 0, 0.0, 2.0, vals, 1, eps=0.05, system="lorenz_rk4")
+    text = "-10 10\n-10 10\n" + "".join((", " if i else "") + ("\n" if i and i % 4 == 0 else "") + "%d" % v
+                                        for i, v in enumerate(m))
+    assert f.read_text() == text

@@ -139,3 +139,28 @@ def test_time_domain_samples_are_the_trajectory():
     # a bound below |x| stops the transient: nothing recorded
     out, flags = O.time_domain(LOR, 0, tMax, 3, h, ic, [20.0, 30.0], [1], 0, 0.5, tT, vals, pre)
     assert np.all(flags == 0) and not out.any()
+
+
+def test_legacy_bif1d_ic_equals_parameter_sweep_on_one_point():
+    # a degenerate IC range puts every point at the same start: with
+    # preScaller 1 and no divergence or equilibrium, the legacy IC sweep and
+    # the Bifurcation module's parameter sweep record the same samples
+    ic, vals = [1.0, 1.0, 1.0], [10.0, 28.0, 8 / 3]
+    counts, peaks = O.legacy_bif1d_ic(LOR, 0, 20.0, 3, 0.01, ic, [1.0, 1.0], [0], 0, 0.0, 5.0, vals, 1)
+    c2, _, p2, _ = O.bif1d(LOR, 0, 20.0, 2, 0.01, ic, [28.0, 28.0], [1], 0, 0.0, 5.0, vals, 1)
+    assert np.all(counts == c2[0]) and c2[0] > 5
+    assert np.array_equal(peaks[:counts[0]], p2[:c2[0]])
+
+
+def test_legacy_ic_transient_fixed_point_still_records():
+    # rho = 0.5, start at the origin: an exact equilibrium.  The legacy IC
+    # sweep records it and reports -1; the Bifurcation module's parameter
+    # sweep drops a fixed-point transient to 0 (bifurcationCUDA.cu:334-342)
+    vals = [10.0, 0.5, 8 / 3]
+    counts, peaks = O.legacy_bif1d_ic(LOR, 0, 5.0, 2, 0.01, [0.0, 0.0, 0.0], [0.0, 0.0], [0], 0, 0.0, 1.0, vals, 1)
+    assert list(counts) == [-1, -1] and peaks.size == 0
+    c2, _, _, _ = O.bif1d(LOR, 0, 5.0, 2, 0.01, [0.0, 0.0, 0.0], [0.5, 0.5], [1], 0, 0.0, 1.0, vals, 1)
+    assert list(c2) == [0, 0]
+    m = O.legacy_bif2d_ic(LOR, 0, 5.0, 2, 0.01, [0.0, 0.0, 0.0], [0.0, 0.0, 0.0, 0.0], [0, 1], 0, 0.0, 1.0,
+                          vals, 1, 0.1)
+    assert list(m) == [-1] * 4
