@@ -16,6 +16,7 @@ crosses PCIe in the timed region.
   basins  Lorenz basins 2048^2 ICs, rho=24.5, 300 + 300 t.u., 3-D DBSCAN
   stream  the served mixed-job stream, engine vs one launch per job
   copy    streaming copy of --bytes (reference for HBM traffic counters)
+  ls      legacy LS2D Lyapunov spectrum, Lorenz 1024^2 (rho x beta), 50 + 100 t.u.
 
 The roofline block prices the dominant kernel's algorithmic FP64 work
 (63 FLOP per Rossler RK4 step, 71 per Lorenz step, FMA = 2) against MI355X's
@@ -271,6 +272,37 @@ def run_basins(args):
     return res
 
 
+def run_ls(args):
+    """Legacy LS2D: Lyapunov spectrum of Lorenz over (rho, beta), 4 trajectories per point."""
+    import numpy as np
+    import chaoslibrary_amd as ch
+    n = args.grid if args.grid != 2048 else 1024
+    stats = ch.Stats()
+    dptr = C.c_void_p()
+    ch._check(ch.lib.chaos_device_malloc(C.byref(dptr), n * n * 3 * 8, 0))
+    ic, rg, ix, vals = (ch._f64([1.0, 1.0, 1.0]), ch._f64([20.0, 200.0, 0.5, 4.0]), ch._i32([1, 2]),
+                        ch._f64([10.0, 28.0, 8 / 3]))
+
+    def one():
+        o = ch.make_options("lorenz_rk4", 0, stats=stats, device_buffers=True)
+        ch._check(ch.lib.chaos_legacy_ls(C.byref(o), 100.0, 0.5, n, 0.01, 1e-8, ch._dp(ic), 3, ch._dp(rg),
+                                         ch._ip(ix), 0, 0.0, 50.0, ch._dp(vals), 3, 2, dptr))
+        return stats.kernel_ms, 0.0, stats.rk4_steps
+
+    wall, kms, _, steps = timed(args, one)
+    host = np.empty(n * n * 3)
+    ch._check(ch.lib.chaos_memcpy_d2h(host.ctypes.data, dptr, host.nbytes))
+    ch._check(ch.lib.chaos_device_free(dptr))
+    res = base_result(args, steps / wall, wall, steps)
+    res["metric"] = f"FP64 RK4 steps/s, legacy LS2D Lyapunov spectrum, Lorenz {n}^2 (4 trajectories per point)"
+    res["data"] = "synthetic: rho = linspace(20,200), beta = linspace(0.5,4), sigma = 10"
+    res["config"] = {"workload": f"ls2d_lorenz_rk4_{n}x{n}", "grid": n, "transient_time": 50.0, "tMax": 100.0,
+                     "NT": 0.5, "h": 0.01, "eps": 1e-8}
+    res["roofline"] = roofline("lorenz_rk4", steps / args.steps, kms / 1e3, "legacy_ls_kernel<6>", kms)
+    res["spectrum_sum_mean"] = float(host.reshape(-1, 3).sum(axis=1).mean())
+    return res
+
+
 def run_stream(args):
     import numpy as np
     import chaoslibrary_amd as ch
@@ -331,7 +363,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3, help="timed runs")
     ap.add_argument("--warmup", type=int, default=1, help="untimed runs")
-    ap.add_argument("--workload", default="lle", choices=["lle", "bif1d", "bif2d", "basins", "stream", "copy"])
+    ap.add_argument("--workload", default="lle", choices=["lle", "bif1d", "bif2d", "basins", "stream", "copy", "ls"])
     ap.add_argument("--grid", type=int, default=2048)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
@@ -350,7 +382,7 @@ def main():
                           "exchange, but multi-GPU runs are not part of it; use --gpus 1"}))
         sys.exit(2)
     res = {"lle": run_lle, "bif1d": run_bif1d, "bif2d": run_bif2d, "basins": run_basins, "stream": run_stream,
-           "copy": run_copy}[args.workload](args)
+           "copy": run_copy, "ls": run_ls}[args.workload](args)
     print(json.dumps(res), flush=True)
 
 

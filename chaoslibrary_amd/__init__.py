@@ -50,7 +50,10 @@ class Stats(C.Structure):
 class Options(C.Structure):
     _fields_ = [("system", C.c_int), ("semantics", C.c_int), ("seed", C.c_uint64),
                 ("device", C.c_int), ("device_buffers", C.c_int), ("stream", C.c_void_p),
-                ("min_pts", C.c_int), ("block", C.c_int), ("stats", C.POINTER(Stats))]
+                ("min_pts", C.c_int), ("block", C.c_int), ("stats", C.POINTER(Stats)),
+                ("output", C.c_int), ("reserved", C.c_int)]
+
+OUT_CSV, OUT_BINARY = 1, 2
 
 
 def _load():
@@ -119,7 +122,7 @@ def _ip(a):
 
 
 def make_options(system="chameleon", semantics=SEM_REFERENCE, seed=12345, device=0, block=0,
-                 min_pts=20, stats=None, device_buffers=False):
+                 min_pts=20, stats=None, device_buffers=False, output=OUT_CSV):
     o = Options()
     lib.chaos_options_init(C.byref(o))
     o.system = SYSTEMS[system] if isinstance(system, str) else int(system)
@@ -129,6 +132,7 @@ def make_options(system="chameleon", semantics=SEM_REFERENCE, seed=12345, device
     o.block = int(block)
     o.min_pts = int(min_pts)
     o.device_buffers = 1 if device_buffers else 0
+    o.output = int(output)
     if stats is not None:
         o.stats = C.pointer(stats)
     return o
@@ -333,6 +337,15 @@ def basinsOfAttraction_2(tMax, nPts, h, amountOfInitialConditions, initialCondit
 # ---------------------------------------------------------------------------
 # legacy entry points (namespace old_library)
 # ---------------------------------------------------------------------------
+
+lib.chaos_map_bin_to_csv.restype = C.c_int
+lib.chaos_map_bin_to_csv.argtypes = [C.c_char_p, C.c_char_p]
+
+
+def map_bin_to_csv(bin_path, csv_path):
+    """chaos_map_bin_to_csv: the CSV text of a map sidecar (<path>.bin)."""
+    _check(lib.chaos_map_bin_to_csv(str(bin_path).encode(), str(csv_path).encode()))
+
 
 _LLE_STYLE = [C.POINTER(Options), C.c_double, C.c_double, C.c_int, C.c_double, C.c_double, _D, C.c_int, _D, _I,
               C.c_int, C.c_double, C.c_double, _D, C.c_int]

@@ -145,7 +145,8 @@ extern "C" int chaos_basinsOfAttraction_2(const chaos_options* opt, double tMax,
         // all four (five) files get the two range lines first, at the
         // precision 15 the reference sets before writing them
         // (basinsCUDA.cu:557-590)
-        {
+        const bool csv = out_csv(o.output);
+        if (csv) {
             TextFile f(base);
             write_range_header(f, ranges, 2, 15);
             f.close();
@@ -161,26 +162,28 @@ extern "C" int chaos_basinsOfAttraction_2(const chaos_options* opt, double tMax,
                                 indicesOfMutVars, writableVar, maxValue, transientTime, values, amountOfValues,
                                 preScaller, eps, labels.data(), feat.data(), flags.data());
         if (rc != CHAOS_OK) throw Error(rc, chaos_last_error());
-        {
-            TextFile f(base, true);
-            write_map_int(f, labels.data(), n, nPts);
-            f.close();
-        }
-        // _1: feature 0, _2: feature 1, _3: flags (:733-792); CONFIG adds _4: feature 2
-        for (int k = 0; k < 2; ++k) {
-            TextFile g(base + "_" + std::to_string(k + 1) + ".csv", true);
-            write_map_double(g, feat.data() + k * n, n, nPts, 15, false);
-            g.close();
-        }
-        {
-            TextFile g(base + "_3.csv", true);
-            write_map_int(g, flags.data(), n, nPts);
-            g.close();
-        }
-        if (!ref) {
-            TextFile g(base + "_4.csv", true);
-            write_map_double(g, feat.data() + 2 * n, n, nPts, 15, false);
-            g.close();
-        }
+        // OUT: labels; _1: feature 0, _2: feature 1, _3: flags (:733-792);
+        // CONFIG adds _4: feature 2
+        auto emit = [&](const std::string& path, const double* dv, const int* iv) {
+            MapSpec m;
+            m.ranges = ranges;
+            m.header_precision = 15;
+            m.nPts = nPts;
+            m.count = n;
+            m.precision = 15;
+            m.dv = dv;
+            m.iv = iv;
+            if (csv) {  // headers were written before the compute
+                TextFile g(path, true);
+                write_map_text(g, m);
+                g.close();
+            }
+            m.header_lines = 2;
+            if (out_bin(o.output)) write_map_bin(path + ".bin", m);
+        };
+        emit(base, nullptr, labels.data());
+        for (int k = 0; k < 2; ++k) emit(base + "_" + std::to_string(k + 1) + ".csv", feat.data() + k * n, nullptr);
+        emit(base + "_3.csv", nullptr, flags.data());
+        if (!ref) emit(base + "_4.csv", feat.data() + 2 * n, nullptr);
     });
 }

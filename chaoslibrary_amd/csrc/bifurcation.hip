@@ -11,6 +11,7 @@
 //        per-point dbscan on the kept intervals.
 #include <algorithm>
 #include <cmath>
+#include <memory>
 #include <vector>
 
 #include "csv.hpp"
@@ -480,6 +481,24 @@ using namespace chaos;
 
 namespace chaos {
 
+// Two range lines at default precision, then the int map; CSV into the
+// already opened file and/or the binary sidecar.
+static void write_map_file(const chaos_options& o, TextFile* f, const char* path, const double* ranges, int nPts,
+                           const int* res)
+{
+    MapSpec m;
+    m.ranges = ranges;
+    m.header_lines = 2;
+    m.nPts = nPts;
+    m.count = (long long)nPts * nPts;
+    m.iv = res;
+    if (f) {
+        write_map_text(*f, m);
+        f->close();
+    }
+    if (out_bin(o.output)) write_map_bin(std::string(path) + ".bin", m);
+}
+
 static void bif1d_impl(const chaos_options* opt, double tMax, int nPts, double h, int nIC, const double* ic,
                        const double* ranges, const int* idx, int writableVar, double maxValue, double transientTime,
                        const double* values, int nV, int preScaller, int* counts, double* first, double* peaks,
@@ -691,15 +710,14 @@ extern "C" int chaos_bifurcation2DIC(const chaos_options* opt, double tMax, int 
         CHAOS_REQUIRE(nPts >= 2, "nPts must be >= 2");
         chaos_options o = resolve_options(opt);
         o.device_buffers = 0;
-        TextFile f(OUT_FILE_PATH);
+        std::unique_ptr<TextFile> f;
+        if (out_csv(o.output)) f.reset(new TextFile(OUT_FILE_PATH));
         std::vector<int> res((size_t)nPts * nPts);
         bif2d_impl(&o, tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges, indicesOfMutVars,
                    writableVar, maxValue, transientTime, values, amountOfValues, preScaller, eps, res.data(), 1);
         // debri/hostLibrary.cu:2934-2935, 3060-3073: two range lines at
         // default precision, then the map
-        write_range_header(f, ranges, 2, 6);
-        write_map_int(f, res.data(), (long long)nPts * nPts, nPts);
-        f.close();
+        write_map_file(o, f.get(), OUT_FILE_PATH, ranges, nPts, res.data());
     });
 }
 
@@ -714,14 +732,13 @@ extern "C" int chaos_bifurcation2D(const chaos_options* opt, double tMax, int nP
         CHAOS_REQUIRE(nPts >= 1, "nPts must be >= 1");
         chaos_options o = resolve_options(opt);
         o.device_buffers = 0;
-        TextFile f(OUT_FILE_PATH);
+        std::unique_ptr<TextFile> f;
+        if (out_csv(o.output)) f.reset(new TextFile(OUT_FILE_PATH));
         std::vector<int> res((size_t)nPts * nPts);
         int rc = chaos_bif2d(&o, tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges,
                              indicesOfMutVars, writableVar, maxValue, transientTime, values, amountOfValues,
                              preScaller, eps, res.data());
         if (rc != CHAOS_OK) throw Error(rc, chaos_last_error());
-        write_range_header(f, ranges, 2, 6);  // :109-110
-        write_map_int(f, res.data(), (long long)nPts * nPts, nPts);
-        f.close();
+        write_map_file(o, f.get(), OUT_FILE_PATH, ranges, nPts, res.data());  // header :109-110
     });
 }

@@ -8,6 +8,8 @@
 #include <cstdio>
 #include <string>
 
+#include "../../include/chaoslibrary.h"
+
 namespace chaos {
 
 class TextFile {
@@ -41,5 +43,29 @@ void write_range_header(TextFile& f, const double* ranges, int lines, int precis
 void write_map_double(TextFile& f, const double* v, long long count, int nPts, int precision,
                       bool nan_to_zero);
 void write_map_int(TextFile& f, const int* v, long long count, int nPts);
+
+// chaos_options.output: which forms of a map file to write (0 = CSV).
+inline bool out_csv(int output) { return output == 0 || (output & CHAOS_OUT_CSV); }
+inline bool out_bin(int output) { return (output & CHAOS_OUT_BINARY) != 0; }
+
+// One map file: optional range header lines, then count values (doubles at
+// `precision`, NaN optionally written as 0, or ints) in the row layout above.
+struct MapSpec {
+    const double* ranges = nullptr;
+    int header_lines = 0;
+    int header_precision = 6;
+    int nPts = 0;
+    long long count = 0;
+    int precision = 6;
+    bool nan_to_zero = false;
+    const double* dv = nullptr;  // exactly one of dv / iv
+    const int* iv = nullptr;
+};
+
+void write_map_text(TextFile& f, const MapSpec& m);
+// Binary sidecar: an 80-byte header (magic "CHAOSMAP", version, value type,
+// count, nPts, precision, NaN rule, header lines and precision, ranges) and
+// the raw values, little-endian.
+void write_map_bin(const std::string& path, const MapSpec& m);
 
 }  // namespace chaos

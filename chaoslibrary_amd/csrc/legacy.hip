@@ -414,9 +414,16 @@ static int legacy_lle_file(const char* fn, const chaos_options* opt, double tMax
         } else {
             // debri/hostLibrary.cu:3795-3803 (LLE2DIC): the 2-D layout, no
             // header, default precision
-            TextFile f(path);
-            write_map_double(f, v.data(), A.n_points, nPts, 6, false);
-            f.close();
+            MapSpec m;
+            m.nPts = nPts;
+            m.count = A.n_points;
+            m.dv = v.data();
+            if (out_csv(o.output)) {
+                TextFile f(path);
+                write_map_text(f, m);
+                f.close();
+            }
+            if (out_bin(o.output)) write_map_bin(std::string(path) + ".bin", m);
         }
     });
 }
@@ -491,10 +498,19 @@ static int legacy_ls_file(const char* fn, const chaos_options* opt, double tMax,
         std::vector<double> col((size_t)A.n_points);
         for (int k = 0; k < nIC; ++k) {
             for (long long p = 0; p < A.n_points; ++p) col[p] = v[p * nIC + k];
-            TextFile f(std::string(path) + std::to_string(k + 1) + ".csv");
-            write_range_header(f, ranges, 2, 6);
-            write_map_double(f, col.data(), A.n_points, nPts, 6, false);
-            f.close();
+            const std::string file = std::string(path) + std::to_string(k + 1) + ".csv";
+            MapSpec m;
+            m.ranges = ranges;
+            m.header_lines = 2;
+            m.nPts = nPts;
+            m.count = A.n_points;
+            m.dv = col.data();
+            if (out_csv(o.output)) {
+                TextFile f(file);
+                write_map_text(f, m);
+                f.close();
+            }
+            if (out_bin(o.output)) write_map_bin(file + ".bin", m);
         }
     });
 }

@@ -7,6 +7,7 @@
 // nothing but the final FP64 exponent touches HBM (the reference keeps the
 // state in shared memory and hands it between two kernels via semi_result).
 #include <cmath>
+#include <memory>
 #include <vector>
 
 #include "csv.hpp"
@@ -138,14 +139,24 @@ extern "C" int chaos_LLE2D(const chaos_options* opt, double tMax, double NT, int
         const long long n = (long long)nPts * (nPts > 0 ? nPts : 0);
         // Open first: the reference exits before computing when the file
         // cannot be opened (LLE/src/LLECUDA.cu:245-251).
-        TextFile f(OUT_FILE_PATH);
+        std::unique_ptr<TextFile> f;
+        if (out_csv(o.output)) f.reset(new TextFile(OUT_FILE_PATH));
         std::vector<double> res((size_t)n);
         int rc = chaos_lle2d(&o, tMax, NT, nPts, h, eps, initialConditions, amountOfInitialConditions, ranges,
                              indicesOfMutVars, writableVar, maxValue, transientTime, values, amountOfValues,
                              res.data());
         if (rc != CHAOS_OK) throw Error(rc, chaos_last_error());
-        write_range_header(f, ranges, 2, 6);
-        write_map_double(f, res.data(), n, nPts, 6, true);
-        f.close();
+        MapSpec m;
+        m.ranges = ranges;
+        m.header_lines = 2;
+        m.nPts = nPts;
+        m.count = n;
+        m.nan_to_zero = true;
+        m.dv = res.data();
+        if (f) {
+            write_map_text(*f, m);
+            f->close();
+        }
+        if (out_bin(o.output)) write_map_bin(std::string(OUT_FILE_PATH) + ".bin", m);
     });
 }

@@ -34,6 +34,8 @@ void init_defaults_locked()
     g_defaults.min_pts = 20;
     g_defaults.block = 0;
     g_defaults.stats = nullptr;
+    g_defaults.output = CHAOS_OUT_CSV;
+    g_defaults.reserved = 0;
     g_defaults_init = true;
 }
 }  // namespace

@@ -93,7 +93,18 @@ typedef struct chaos_options {
     int min_pts;            /* CONFIG basins DBSCAN minPts; default 20       */
     int block;              /* threads per block (0 = tuned default)         */
     chaos_stats* stats;     /* optional timing/work report                   */
+    int output;             /* map files (2-D outputs): CHAOS_OUT_CSV (default,
+                               0 counts as CSV) and/or CHAOS_OUT_BINARY, a
+                               sidecar <path>.bin that chaos_map_bin_to_csv
+                               turns into the same text later                */
+    int reserved;
 } chaos_options;
+
+enum { CHAOS_OUT_CSV = 1, CHAOS_OUT_BINARY = 2 };
+
+/* Writes the CSV text of a map sidecar (<path>.bin from CHAOS_OUT_BINARY):
+ * byte-identical to the CSV the entry point writes itself. */
+int chaos_map_bin_to_csv(const char* bin_path, const char* csv_path);
 
 /* Fill opt with the process defaults (CHAMELEON, REFERENCE, seed 12345,
  * device 0, minPts 20).  chaos_set_default_options replaces the defaults used

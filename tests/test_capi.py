@@ -83,3 +83,27 @@ def test_cpp_api_links(chaos, tmp_path):
                         "-L", lib, "-lchaos", "-Wl,--unresolved-symbols=ignore-in-shared-libs"],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-2000:]
+
+
+def test_map_sidecar_converter_cpu(chaos, tmp_path):
+    """A hand-built map sidecar converts to the reference text layout."""
+    import struct
+    import numpy as np
+    vals = np.array([0.5, float("nan"), 1.25e-7, -3.0, 2.0, 7.0], dtype=np.float64)
+    hdr = struct.pack("<8sIIqiiiiii4d", b"CHAOSMAP", 1, 0, vals.size, 3, 6, 1, 2, 6, 0, 0.05, 0.35, 2.0, 10.0)
+    assert len(hdr) == 80
+    b = tmp_path / "m.csv.bin"
+    b.write_bytes(hdr + vals.tobytes())
+    chaos.map_bin_to_csv(b, tmp_path / "m.csv")
+    assert (tmp_path / "m.csv").read_text() == "0.05 0.35\n2 10\n0.5, 0, 1.25e-07, \n-3, 2, 7"
+    ints = np.array([1, -1, 0, 3], dtype=np.int32)
+    hdr = struct.pack("<8sIIqiiiiii4d", b"CHAOSMAP", 1, 1, ints.size, 2, 6, 0, 0, 6, 0, 0, 0, 0, 0)
+    b.write_bytes(hdr + ints.tobytes())
+    chaos.map_bin_to_csv(b, tmp_path / "i.csv")
+    assert (tmp_path / "i.csv").read_text() == "1, -1, \n0, 3"
+    b.write_bytes(b"NOTAMAP!" + bytes(72))
+    with pytest.raises(chaos.ChaosError):
+        chaos.map_bin_to_csv(b, tmp_path / "x.csv")
+    b.write_bytes(hdr + ints.tobytes()[:-4])            # truncated
+    with pytest.raises(chaos.ChaosError):
+        chaos.map_bin_to_csv(b, tmp_path / "x.csv")
