@@ -6,7 +6,8 @@
 //   kept_kernel + scan       per-point segment offsets (flag == 1 points)
 //   scatter_kernel           chunk entries -> per-point contiguous (value, time)
 //   1-D: host assembles the kept peaks / intervals (first peak dropped)
-//   2-D: one wave per point -- CONFIG: sort (LDS bitonic) + gap count =
+//   2-D: one wave per point -- CONFIG: sort (LDS bitonic, launched per
+//        power-of-two size class so LDS follows the segment) + gap count =
 //        1-D DBSCAN(eps, minPts=1) cluster count; REFERENCE: the reference's
 //        per-point dbscan on the kept intervals.
 #include <algorithm>
@@ -42,7 +43,7 @@ __global__ void scatter_kernel(const PeakPool pool, unsigned used, const int* __
     if (p >= n || flags[p] != 1) return;
     const long long pos = off[p] + (e.kl >> 6);
     vals[pos] = e.v;
-    times[pos] = e.t;
+    if (times) times[pos] = e.t;
 }
 
 void PeakPoolBuf::alloc(hipStream_t st, long long chunks)
@@ -159,7 +160,8 @@ bool bif_post(CallCtx& ctx, BifDevice& R, PeakPoolBuf& pool)
     CHAOS_HIP(hipGetLastError());
     R.total = exclusive_scan(ctx.stream(), kept.get(), R.off.get(), n);
     R.vals.alloc((size_t)std::max(1LL, R.total));
-    R.times.alloc((size_t)std::max(1LL, R.total));
+    if (R.need_times) R.times.alloc((size_t)std::max(1LL, R.total));
+    else R.times.release();
     if (used > (unsigned)pool.nchunks) used = (unsigned)pool.nchunks;
     if (used > 0) {
         hipLaunchKernelGGL(scatter_kernel, dim3(ceil_div((unsigned long long)used * kChunk, 256)), dim3(256), 0,
@@ -236,34 +238,30 @@ __device__ __forceinline__ int gap_count(const double* v, int P, double eps)
     return 1 + __shfl(gaps, 0);
 }
 
+// CONFIG 2-D, every point: -1 for a fixed point, else 0; the points with
+// kept peaks are overwritten by dbscan1d_list_kernel.
+__global__ void flag_out_kernel(const int* __restrict__ flags, int* __restrict__ out, long long n)
+{
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) out[p] = flags[p] == -1 ? -1 : 0;
+}
+
 // CONFIG 2-D: number of 1-D DBSCAN(eps, minPts=1) clusters of the kept peak
-// values (first peak dropped); points whose segment exceeds the LDS
-// capacity are left for dbscan1d_big_kernel (marked -2).
-__global__ __launch_bounds__(64) void dbscan1d_count_kernel(const int* __restrict__ flags,
-                                                            const int* __restrict__ counts,
-                                                            const long long* __restrict__ off,
-                                                            const double* __restrict__ vals, double eps, int cap,
-                                                            int* __restrict__ out, long long n)
+// values (first peak dropped) of the listed points, which all have
+// 1 <= P <= M kept peaks: one wave per point, LDS sized for this size class
+// (the launches go by powers of two, so a map's few long segments do not
+// set the LDS footprint -- and the occupancy -- of all the others).
+__global__ __launch_bounds__(64) void dbscan1d_list_kernel(const int* __restrict__ list, int nlist,
+                                                           const int* __restrict__ counts,
+                                                           const long long* __restrict__ off,
+                                                           const double* __restrict__ vals, double eps, int M,
+                                                           int* __restrict__ out)
 {
     extern __shared__ double lds[];
-    const long long p = blockIdx.x;
-    if (p >= n) return;
-    const int fl = flags[p];
-    if (fl != 1) {
-        if (threadIdx.x == 0) out[p] = fl == -1 ? -1 : 0;
-        return;
-    }
-    const int P = counts[p] > 1 ? counts[p] - 1 : 0;
-    if (P == 0) {
-        if (threadIdx.x == 0) out[p] = 0;
-        return;
-    }
-    if (P > cap) {
-        if (threadIdx.x == 0) out[p] = -2;
-        return;
-    }
-    int M = 1;
-    while (M < P) M <<= 1;
+    const int b = blockIdx.x;
+    if (b >= nlist) return;
+    const int p = list[b];
+    const int P = counts[p] - 1;
     const double* v = vals + off[p] + 1;
     for (int k = threadIdx.x; k < M; k += blockDim.x) lds[k] = k < P ? v[k] : INFINITY;
     __syncthreads();
@@ -374,27 +372,53 @@ void bif2d_analysis(CallCtx& ctx, int semantics, BifDevice& R, double eps, int* 
         CHAOS_HIP(hipGetLastError());
         ctx.sync();
     } else {
-        int maxP = 0;
-        for (long long p = 0; p < n; ++p)
-            if (R.h_flags[p] == 1) maxP = std::max(maxP, R.h_counts[p] - 1);
-        const int cap_limit = 4096;
-        int M = 1;
-        while (M < std::min(maxP, cap_limit)) M <<= 1;
-        const int cap = std::min(M, cap_limit);
-        hipLaunchKernelGGL(dbscan1d_count_kernel, dim3((unsigned)n), dim3(64), (size_t)M * sizeof(double),
-                           ctx.stream(), R.flags.get(), R.counts.get(), R.off.get(), R.vals.get(), eps, cap, d_out, n);
-        CHAOS_HIP(hipGetLastError());
-        if (maxP > cap) {
-            std::vector<long long> big, soff;
-            long long tot = 0;
-            for (long long p = 0; p < n; ++p)
-                if (R.h_flags[p] == 1 && R.h_counts[p] - 1 > cap) {
-                    big.push_back(p);
-                    soff.push_back(tot);
-                    long long m = 1;
-                    while (m < R.h_counts[p] - 1) m <<= 1;
-                    tot += m;
-                }
+        hipLaunchKernelGGL(flag_out_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx.stream(), R.flags.get(), d_out,
+                           n);
+        // size classes M = 64 << c (c < kClasses) hold the points with
+        // M/2 < P <= M kept peaks (the first class also the smaller ones);
+        // longer segments sort in global memory
+        constexpr int kClasses = 7;  // up to 4096 values = 32 KiB of LDS
+        std::vector<int> cls[kClasses];
+        std::vector<long long> big, soff;
+        long long tot = 0;
+        for (long long p = 0; p < n; ++p) {
+            if (R.h_flags[p] != 1) continue;
+            const int P = R.h_counts[p] - 1;
+            if (P < 1) continue;
+            int c = 0;
+            while (c < kClasses && (64 << c) < P) ++c;
+            if (c < kClasses) {
+                cls[c].push_back((int)p);
+            } else {
+                big.push_back(p);
+                soff.push_back(tot);
+                long long m = 1;
+                while (m < P) m <<= 1;
+                tot += m;
+            }
+        }
+        std::vector<int> all;
+        std::vector<size_t> start(kClasses + 1, 0);
+        for (int c = 0; c < kClasses; ++c) {
+            start[c] = all.size();
+            all.insert(all.end(), cls[c].begin(), cls[c].end());
+        }
+        start[kClasses] = all.size();
+        DevBuf<int> d_list(std::max<size_t>(1, all.size()));
+        if (!all.empty())
+            CHAOS_HIP(hipMemcpyAsync(d_list.get(), all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice,
+                                     ctx.stream()));
+        for (int c = 0; c < kClasses; ++c) {
+            const int cnt = (int)(start[c + 1] - start[c]);
+            if (cnt == 0) continue;
+            const int M = 64 << c;
+            hipLaunchKernelGGL(dbscan1d_list_kernel, dim3((unsigned)cnt), dim3(64), (size_t)M * sizeof(double),
+                               ctx.stream(), d_list.get() + start[c], cnt, R.counts.get(), R.off.get(), R.vals.get(),
+                               eps, M, d_out);
+            CHAOS_HIP(hipGetLastError());
+            ctx.add_launch();
+        }
+        if (!big.empty()) {
             DevBuf<long long> d_big(big.size()), d_soff(soff.size());
             DevBuf<double> scratch((size_t)tot);
             CHAOS_HIP(hipMemcpyAsync(d_big.get(), big.data(), big.size() * 8, hipMemcpyHostToDevice, ctx.stream()));
@@ -405,6 +429,7 @@ void bif2d_analysis(CallCtx& ctx, int semantics, BifDevice& R, double eps, int* 
             CHAOS_HIP(hipGetLastError());
             ctx.sync();
         }
+        ctx.sync();  // the list buffers go back to the allocator at the end of this scope
     }
     ctx.add_analysis_ms(ctx.end());
     ctx.add_launch();
@@ -528,6 +553,7 @@ static void bif2d_impl(const chaos_options* opt, double tMax, int nPts, double h
                               preScaller, eps, 2, legacy_ic);
     CallCtx ctx(o);
     BifDevice R;
+    R.need_times = o.semantics == CHAOS_SEM_REFERENCE;  // the CONFIG count reads peak values only
     run_bif(ctx, o, in, R);
     DevBuf<int> tmp;
     int* d_out = out;

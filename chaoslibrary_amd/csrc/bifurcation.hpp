@@ -23,6 +23,7 @@ struct BifDevice {
     long long n = 0;         // points
     long long total = 0;     // stored peaks of flag-1 points (first peak included)
     double hp = 0;           // interval unit
+    bool need_times = true;  // scatter the peak times too (intervals)
     DevBuf<int> flags, counts;
     DevBuf<double> first;
     DevBuf<long long> off;   // segment offsets
