@@ -17,6 +17,10 @@ crosses PCIe in the timed region.
   stream  the served mixed-job stream, engine vs one launch per job
   copy    streaming copy of --bytes (reference for HBM traffic counters)
   ls      legacy LS2D Lyapunov spectrum, Lorenz 1024^2 (rho x beta), 50 + 100 t.u.
+  regression  src/main.cu-style sweeps (resolution 100..600, modelling time
+          1000..6000) of bifurcation2D / LLE2D / basinsOfAttraction_2 through
+          chaoslibrary_amd/chaos_regression, rows Test,Parameter,Value,Library,
+          ExecutionTime_ms with Library cpu | hip-per-job | hip-batched
 
 The roofline block prices the dominant kernel's algorithmic FP64 work
 (63 FLOP per Rossler RK4 step, 71 per Lorenz step, FMA = 2) against MI355X's
@@ -337,6 +341,150 @@ def run_stream(args):
     return res
 
 
+def _parse_suite(line):
+    """A `SUITE <name> key=value ...` line of chaos_regression."""
+    parts = line.split()
+    d = {"name": parts[1]}
+    for kv in parts[2:]:
+        k, v = kv.split("=", 1)
+        if k in ("init", "values", "ranges"):
+            d[k] = [float(x) for x in v.split(",")]
+        elif k == "idx":
+            d[k] = [int(x) for x in v.split(",")]
+        elif k in ("system", "semantics", "preScaller", "writableVar", "seed"):
+            d[k] = int(v)
+        else:
+            d[k] = float(v)
+    return d
+
+
+def _cpu_run_ms(O, s, n, TT, threads):
+    """Oracle time of one harness run (same inputs), on whole grid rows
+    around the middle of the map (at least 4 points per thread), scaled to
+    the n x n grid.  Basins: the per-point features only (no clustering)."""
+    import numpy as np
+    rows = max(1, min(n, -(-4 * threads // n)))
+    r0 = (n - rows) // 2
+    ic, vals, rg = (np.asarray(s[k], dtype=np.float64) for k in ("init", "values", "ranges"))
+    ix = np.asarray(s["idx"], dtype=np.int32)
+    t0 = time.perf_counter()
+    if s["name"] == "LLE":
+        out = np.zeros(rows * n)
+        used = O.lib.orc_lle2d_omp(s["system"], s["semantics"], s["CT"], s["NT"], n, s["h"], s["eps"], O.dptr(ic),
+                                   ic.size, O.dptr(rg), O.iptr(ix), TT, O.dptr(vals), vals.size, s["seed"],
+                                   O.dptr(out), r0, r0 + rows, 1, threads)
+        assert used > 0, used
+    elif s["name"] == "Bifurcation":
+        out = np.zeros(rows * n, dtype=np.int32)
+        rc = O.lib.orc_bif2d(s["system"], s["semantics"], s["CT"], n, s["h"], O.dptr(ic), ic.size, O.dptr(rg),
+                             O.iptr(ix), s["writableVar"], s["maxValue"], TT, O.dptr(vals), vals.size,
+                             s["preScaller"], s["eps"], O.iptr(out), r0, r0 + rows)
+        assert rc == 0, rc
+    else:
+        O.basins_features(s["system"], s["semantics"], s["CT"], n, s["h"], ic, rg, ix, s["writableVar"],
+                          s["maxValue"], TT, vals, s["preScaller"], r0 * n, (r0 + rows) * n)
+    dt = time.perf_counter() - t0
+    return dt * 1e3 * n / rows, rows
+
+
+def run_regression(args):
+    """src/main.cu-style harness: chaoslibrary_amd/chaos_regression times the
+    C++ API (hip-per-job) and the batched engine (hip-batched) over the
+    resolution and modelling-time sweeps and writes
+    <results>/<module>_performance_results.csv; the cpu rows appended here
+    time the oracle on the same runs."""
+    import shutil
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "chaoslibrary_amd", "chaos_regression")
+    if not os.path.exists(exe):
+        raise SystemExit("chaoslibrary_amd/chaos_regression is not built; run __graft_entry__.build()")
+    results = os.path.abspath(args.results_dir)
+    maps = tempfile.mkdtemp(prefix="chaos_regression_maps_")
+    cmd = [exe, "--params", args.params, "--results", results, "--out", maps, "--semantics",
+           "config" if args.semantics == 1 else "reference"]
+    if args.resolutions:
+        cmd += ["--resolutions", args.resolutions]
+    if args.times:
+        cmd += ["--times", args.times]
+    if args.fixed_resolution:
+        cmd += ["--fixed-resolution", str(args.fixed_resolution)]
+    log("+ " + " ".join(cmd))
+    suites, runs, summary = {}, [], ""
+    t0 = time.perf_counter()
+    proc = subprocess.Popen(cmd, stdout=subprocess.PIPE, text=True)
+    for line in proc.stdout:
+        line = line.rstrip("\n")
+        log(line)
+        if line.startswith("SUITE "):
+            d = _parse_suite(line)
+            suites[d["name"]] = d
+        elif line.startswith("RUN "):
+            _, test, par, val, lib, ms, steps, ok = line.split()
+            runs.append({"test": test, "parameter": par, "value": int(val), "library": lib, "ms": float(ms),
+                         "rk4_steps": float(steps), "ok": ok == "1"})
+        elif line.startswith("rows "):
+            summary = line
+    rc = proc.wait()
+    harness_s = time.perf_counter() - t0
+    shutil.rmtree(maps, ignore_errors=True)
+    if rc != 0:
+        raise SystemExit(f"chaos_regression exited with {rc}: {summary}")
+
+    cpu = {}
+    if not args.no_cpu:
+        O = oracle()
+        threads = cpu_threads()
+        for r in runs:
+            if r["library"] != "hip-per-job" or r["parameter"] not in ("Resolution", "ModelingTime"):
+                continue
+            s = suites[r["test"]]
+            n = r["value"] if r["parameter"] == "Resolution" else (args.fixed_resolution or 300)
+            TT = s["TT"] if r["parameter"] == "Resolution" else float(r["value"])
+            ms, rows = _cpu_run_ms(O, s, n, TT, threads)
+            cpu[(r["test"], r["parameter"], r["value"])] = ms
+            log(f"cpu {r['test']} {r['parameter']}={r['value']}: {ms:.0f} ms (from {rows} of {n} rows)")
+        files = {"Bifurcation": "bifurcation_performance_results.csv", "LLE": "lle_performance_results.csv",
+                 "Basins": "basins_performance_results.csv"}
+        for (test, par, val), ms in cpu.items():
+            with open(os.path.join(results, files[test]), "a") as f:
+                f.write(f"{test},{par},{val},cpu,{ms:.3f}" + (",2D" if test == "Bifurcation" else "") + "\n")
+
+    per_job = [r for r in runs if r["library"] == "hip-per-job" and r["ok"]
+               and r["parameter"] in ("Resolution", "ModelingTime")]
+    steps = sum(r["rk4_steps"] for r in per_job)
+    ms = sum(r["ms"] for r in per_job)
+    by_suite = {}
+    for name in suites:
+        d = {}
+        for lib in ("hip-per-job", "hip-batched"):
+            rr = [r for r in runs if r["test"] == name and r["library"] == lib]
+            single = [r for r in rr if r["parameter"] in ("Resolution", "ModelingTime")]
+            sweep = [r for r in rr if r["parameter"].endswith("Sweep")]
+            if single:
+                d[lib + "_ms"] = round(sum(r["ms"] for r in single), 3)
+            if sweep:
+                d[lib + "_sweeps_ms"] = round(sum(r["ms"] for r in sweep), 3)
+        c_ms = [v for (t, _, _), v in cpu.items() if t == name]
+        if c_ms:
+            d["cpu_ms"] = round(sum(c_ms), 1)
+            d["speedup_vs_cpu"] = round(sum(c_ms) / d["hip-per-job_ms"], 1) if d.get("hip-per-job_ms") else None
+        if d.get("hip-batched_sweeps_ms") and d.get("hip-per-job_sweeps_ms"):
+            d["batched_sweep_speedup"] = round(d["hip-per-job_sweeps_ms"] / d["hip-batched_sweeps_ms"], 3)
+        by_suite[name] = d
+    return {"metric": "src/main.cu-style regression harness: RK4 steps/s over the hip-per-job runs "
+                      "(C++ API, map files included)",
+            "value": steps / (ms / 1e3) if ms else 0.0, "unit": "RK4 steps/s", "n_gpus": 1, "steps": len(per_job),
+            "warmup": 0, "ms_per_step": ms / max(1, len(per_job)), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic parameter / initial-condition grids",
+            "config": {"workload": f"regression_{args.params}", "resolutions": args.resolutions or "100..600",
+                       "modelling_times": args.times or "1000..6000", "semantics": args.semantics},
+            "suites": by_suite, "harness": summary, "harness_s": round(harness_s, 1), "results_dir": results,
+            "cpu_rows": (f"oracle on whole middle grid rows (>= 4 points per thread, {cpu_threads()} OpenMP "
+                         f"threads) scaled to the full grid; basins: per-point features only")
+            if cpu else None}
+
+
 def run_copy(args):
     import chaoslibrary_amd as ch
     nb = args.bytes
@@ -363,7 +511,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3, help="timed runs")
     ap.add_argument("--warmup", type=int, default=1, help="untimed runs")
-    ap.add_argument("--workload", default="lle", choices=["lle", "bif1d", "bif2d", "basins", "stream", "copy", "ls"])
+    ap.add_argument("--workload", default="lle", choices=["lle", "bif1d", "bif2d", "basins", "stream", "copy", "ls",
+                                                           "regression"])
     ap.add_argument("--grid", type=int, default=2048)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
@@ -375,6 +524,13 @@ def main():
     ap.add_argument("--tune", action="store_true", help="autotune the engine before the stream")
     ap.add_argument("--mode", default="both", choices=["both", "batched", "per_job"], help="stream modes to run")
     ap.add_argument("--bytes", type=int, default=1 << 30)
+    ap.add_argument("--params", default="rk4", choices=["rk4", "reference"], help="regression parameter sets")
+    ap.add_argument("--semantics", type=int, default=0, help="regression: 0 reference, 1 config")
+    ap.add_argument("--resolutions", default="", help="regression: comma list (default 100..600)")
+    ap.add_argument("--times", default="", help="regression: comma list of modelling times (default 1000..6000)")
+    ap.add_argument("--fixed-resolution", type=int, default=0, help="regression: resolution of the time sweep")
+    ap.add_argument("--results-dir", default=os.path.join("workspace", "regression"),
+                    help="regression: where the *_performance_results.csv files go")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -382,7 +538,7 @@ def main():
                           "exchange, but multi-GPU runs are not part of it; use --gpus 1"}))
         sys.exit(2)
     res = {"lle": run_lle, "bif1d": run_bif1d, "bif2d": run_bif2d, "basins": run_basins, "stream": run_stream,
-           "copy": run_copy, "ls": run_ls}[args.workload](args)
+           "copy": run_copy, "ls": run_ls, "regression": run_regression}[args.workload](args)
     print(json.dumps(res), flush=True)
 
 

@@ -587,6 +587,7 @@ _SIGS3 = {
     "chaos_server_result_bif1d": (C.c_int, [C.c_void_p, C.c_longlong, _I, _D, _D, _D, C.c_longlong,
                                             C.POINTER(C.c_longlong)]),
     "chaos_server_result_basins": (C.c_int, [C.c_void_p, C.c_longlong, _I, _D, _I]),
+    "chaos_server_write_result": (C.c_int, [C.c_void_p, C.c_longlong, C.c_char_p, C.c_int]),
     "chaos_server_release": (C.c_int, [C.c_void_p, C.c_longlong]),
     "chaos_stream_config_default": (C.c_int, [C.POINTER(StreamConfig)]),
     "chaos_stream_run": (C.c_int, [C.c_int, C.POINTER(StreamConfig), C.POINTER(EngineParams),
@@ -697,6 +698,11 @@ class Server:
         feats = np.zeros((nf, n, n))
         _check(lib.chaos_server_result_basins(self.h, jid, _ip(labels), _dp(feats), _ip(flags)))
         return labels, feats, flags
+
+    def write_result(self, jid, path, output=0):
+        """Writes the finished job's files as the one-shot file-writing entry
+        point would (output: OUT_CSV and/or OUT_BINARY, 0 = CSV)."""
+        _check(lib.chaos_server_write_result(self.h, jid, os.fsencode(path), output))
 
     def release(self, jid):
         _check(lib.chaos_server_release(self.h, jid))

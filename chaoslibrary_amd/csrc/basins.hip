@@ -126,6 +126,54 @@ extern "C" int chaos_basins2d(const chaos_options* opt, double tMax, int nPts, d
     });
 }
 
+namespace chaos {
+
+void write_basins_headers(const std::string& base, const double* ranges, int nfeat, int output)
+{
+    // all four (five) files get the two range lines first, at the precision
+    // 15 the reference sets before writing them (basinsCUDA.cu:557-590)
+    if (!out_csv(output)) return;
+    TextFile f(base);
+    write_range_header(f, ranges, 2, 15);
+    f.close();
+    for (int k = 1; k <= nfeat + 1; ++k) {
+        TextFile g(base + "_" + std::to_string(k) + ".csv");
+        write_range_header(g, ranges, 2, 15);
+        g.close();
+    }
+}
+
+void write_basins_bodies(const std::string& base, const double* ranges, int nPts, int nfeat, const int* labels,
+                         const double* feat, const int* flags, int output)
+{
+    const long long n = (long long)nPts * nPts;
+    auto emit = [&](const std::string& path, const double* dv, const int* iv) {
+        MapSpec m;
+        m.ranges = ranges;
+        m.header_precision = 15;
+        m.nPts = nPts;
+        m.count = n;
+        m.precision = 15;
+        m.dv = dv;
+        m.iv = iv;
+        if (out_csv(output)) {  // the headers were written before
+            TextFile g(path, true);
+            write_map_text(g, m);
+            g.close();
+        }
+        m.header_lines = 2;
+        if (out_bin(output)) write_map_bin(path + ".bin", m);
+    };
+    // OUT: labels; _1: feature 0, _2: feature 1, _3: flags (:733-792);
+    // CONFIG adds _4: feature 2
+    emit(base, nullptr, labels);
+    for (int k = 0; k < 2; ++k) emit(base + "_" + std::to_string(k + 1) + ".csv", feat + k * n, nullptr);
+    emit(base + "_3.csv", nullptr, flags);
+    if (nfeat > 2) emit(base + "_4.csv", feat + 2 * n, nullptr);
+}
+
+}  // namespace chaos
+
 extern "C" int chaos_basinsOfAttraction_2(const chaos_options* opt, double tMax, int nPts, double h,
                                           int amountOfInitialConditions, const double* initialConditions,
                                           const double* ranges, const int* indicesOfMutVars, int writableVar,
@@ -142,48 +190,13 @@ extern "C" int chaos_basinsOfAttraction_2(const chaos_options* opt, double tMax,
         const long long n = (long long)nPts * nPts;
         const std::string base(OUT_FILE_PATH);
         const int nfeat = ref ? 2 : 3;
-        // all four (five) files get the two range lines first, at the
-        // precision 15 the reference sets before writing them
-        // (basinsCUDA.cu:557-590)
-        const bool csv = out_csv(o.output);
-        if (csv) {
-            TextFile f(base);
-            write_range_header(f, ranges, 2, 15);
-            f.close();
-            for (int k = 1; k <= nfeat + 1; ++k) {
-                TextFile g(base + "_" + std::to_string(k) + ".csv");
-                write_range_header(g, ranges, 2, 15);
-                g.close();
-            }
-        }
+        write_basins_headers(base, ranges, nfeat, o.output);
         std::vector<int> labels(n), flags(n);
         std::vector<double> feat(nfeat * n);
         int rc = chaos_basins2d(&o, tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges,
                                 indicesOfMutVars, writableVar, maxValue, transientTime, values, amountOfValues,
                                 preScaller, eps, labels.data(), feat.data(), flags.data());
         if (rc != CHAOS_OK) throw Error(rc, chaos_last_error());
-        // OUT: labels; _1: feature 0, _2: feature 1, _3: flags (:733-792);
-        // CONFIG adds _4: feature 2
-        auto emit = [&](const std::string& path, const double* dv, const int* iv) {
-            MapSpec m;
-            m.ranges = ranges;
-            m.header_precision = 15;
-            m.nPts = nPts;
-            m.count = n;
-            m.precision = 15;
-            m.dv = dv;
-            m.iv = iv;
-            if (csv) {  // headers were written before the compute
-                TextFile g(path, true);
-                write_map_text(g, m);
-                g.close();
-            }
-            m.header_lines = 2;
-            if (out_bin(o.output)) write_map_bin(path + ".bin", m);
-        };
-        emit(base, nullptr, labels.data());
-        for (int k = 0; k < 2; ++k) emit(base + "_" + std::to_string(k + 1) + ".csv", feat.data() + k * n, nullptr);
-        emit(base + "_3.csv", nullptr, flags.data());
-        if (!ref) emit(base + "_4.csv", feat.data() + 2 * n, nullptr);
+        write_basins_bodies(base, ranges, nPts, nfeat, labels.data(), feat.data(), flags.data(), o.output);
     });
 }

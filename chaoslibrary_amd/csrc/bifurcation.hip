@@ -506,22 +506,43 @@ using namespace chaos;
 
 namespace chaos {
 
-// Two range lines at default precision, then the int map; CSV into the
-// already opened file and/or the binary sidecar.
-static void write_map_file(const chaos_options& o, TextFile* f, const char* path, const double* ranges, int nPts,
-                           const int* res)
+void write_int_map_file(TextFile* f, const char* path, const double* ranges, int nPts, const int* v, int output)
 {
     MapSpec m;
     m.ranges = ranges;
     m.header_lines = 2;
     m.nPts = nPts;
     m.count = (long long)nPts * nPts;
-    m.iv = res;
+    m.iv = v;
     if (f) {
         write_map_text(*f, m);
         f->close();
     }
-    if (out_bin(o.output)) write_map_bin(std::string(path) + ".bin", m);
+    if (out_bin(output)) write_map_bin(std::string(path) + ".bin", m);
+}
+
+void write_bif1d_body(TextFile& f, const double* ranges, int nPts, const int* counts, const double* first,
+                      const double* peaks, const double* intervals)
+{
+    long long k = 0;
+    for (int p = 0; p < nPts; ++p) {
+        const double x = grid_value(p, nPts, ranges[0], ranges[1]);
+        if (counts[p] == -1) {
+            f.put_g(x, 12);
+            f.put(", ", 2);
+            f.put_g(first[p], 12);
+            f.put(", 0\n");
+            continue;
+        }
+        for (int j = 0; j < counts[p]; ++j, ++k) {
+            f.put_g(x, 12);
+            f.put(", ", 2);
+            f.put_g(peaks[k], 12);
+            f.put(", ", 2);
+            f.put_g(intervals[k], 12);
+            f.put("\n", 1);
+        }
+    }
 }
 
 static void bif1d_impl(const chaos_options* opt, double tMax, int nPts, double h, int nIC, const double* ic,
@@ -678,25 +699,7 @@ extern "C" int chaos_bifurcation1D(const chaos_options* opt, double tMax, int nP
                    0);
         // header at default precision, body at 12 digits (:645-647,746-763)
         write_range_header(f, ranges, 1, 6);
-        long long k = 0;
-        for (int p = 0; p < nPts; ++p) {
-            const double x = grid_value(p, nPts, ranges[0], ranges[1]);
-            if (counts[p] == -1) {
-                f.put_g(x, 12);
-                f.put(", ", 2);
-                f.put_g(first[p], 12);
-                f.put(", 0\n");
-                continue;
-            }
-            for (int j = 0; j < counts[p]; ++j, ++k) {
-                f.put_g(x, 12);
-                f.put(", ", 2);
-                f.put_g(pk[k], 12);
-                f.put(", ", 2);
-                f.put_g(iv[k], 12);
-                f.put("\n", 1);
-            }
-        }
+        write_bif1d_body(f, ranges, nPts, counts.data(), first.data(), pk.data(), iv.data());
         f.close();
     });
 }
@@ -743,7 +746,7 @@ extern "C" int chaos_bifurcation2DIC(const chaos_options* opt, double tMax, int 
                    writableVar, maxValue, transientTime, values, amountOfValues, preScaller, eps, res.data(), 1);
         // debri/hostLibrary.cu:2934-2935, 3060-3073: two range lines at
         // default precision, then the map
-        write_map_file(o, f.get(), OUT_FILE_PATH, ranges, nPts, res.data());
+        write_int_map_file(f.get(), OUT_FILE_PATH, ranges, nPts, res.data(), o.output);
     });
 }
 
@@ -765,6 +768,6 @@ extern "C" int chaos_bifurcation2D(const chaos_options* opt, double tMax, int nP
                              indicesOfMutVars, writableVar, maxValue, transientTime, values, amountOfValues,
                              preScaller, eps, res.data());
         if (rc != CHAOS_OK) throw Error(rc, chaos_last_error());
-        write_map_file(o, f.get(), OUT_FILE_PATH, ranges, nPts, res.data());  // header :109-110
+        write_int_map_file(f.get(), OUT_FILE_PATH, ranges, nPts, res.data(), o.output);  // header :109-110
     });
 }

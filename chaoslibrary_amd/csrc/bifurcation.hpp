@@ -58,4 +58,13 @@ void bif2d_analysis(CallCtx& ctx, int semantics, BifDevice& R, double eps, int* 
 void bif1d_collect(CallCtx& ctx, BifDevice& R, int* counts, double* first, double* peaks, double* intervals,
                    long long capacity, long long* total);
 
+class TextFile;
+// The bifurcation2D map file (two range lines, then the int map): CSV into f
+// (already open, or NULL) and/or the binary sidecar <path>.bin.
+void write_int_map_file(TextFile* f, const char* path, const double* ranges, int nPts, const int* v, int output);
+// The bifurcation1D body after its range line: per point "p, x_first, 0" for
+// flag -1, else "p, peak, interval" per kept peak, at precision 12.
+void write_bif1d_body(TextFile& f, const double* ranges, int nPts, const int* counts, const double* first,
+                      const double* peaks, const double* intervals);
+
 }  // namespace chaos

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "basins.hpp"
+#include "csv.hpp"
 #include "bifurcation.hpp"
 #include "dispatch.hpp"
 #include "engine.hpp"
@@ -763,6 +764,49 @@ extern "C" int chaos_server_result_basins(void* server, long long job_id, int* l
         std::copy(r.r_labels.begin(), r.r_labels.end(), labels);
         if (features) std::copy(r.r_feats.begin(), r.r_feats.end(), features);
         if (flags) std::copy(r.r_flags.begin(), r.r_flags.end(), flags);
+    });
+}
+
+extern "C" int chaos_server_write_result(void* server, long long job_id, const char* OUT_FILE_PATH, int output)
+{
+    return guarded("chaos_server_write_result", [&] {
+        CHAOS_REQUIRE(server, "server is NULL");
+        CHAOS_REQUIRE(OUT_FILE_PATH, "OUT_FILE_PATH is NULL");
+        CHAOS_REQUIRE(output >= 0 && output <= (CHAOS_OUT_CSV | CHAOS_OUT_BINARY), "invalid output %d", output);
+        const JobRec& r = static_cast<Engine*>(server)->job(job_id);
+        CHAOS_REQUIRE(r.status == 2, "job %lld is not finished", job_id);
+        const chaos_job& j = r.spec;
+        switch (j.kind) {
+        case CHAOS_JOB_LLE2D: {
+            std::unique_ptr<TextFile> f;
+            if (out_csv(output)) f.reset(new TextFile(OUT_FILE_PATH));
+            write_lle_file(f.get(), OUT_FILE_PATH, j.ranges, j.nPts, r.r_lle.data(), output);
+            break;
+        }
+        case CHAOS_JOB_BIF2D: {
+            std::unique_ptr<TextFile> f;
+            if (out_csv(output)) f.reset(new TextFile(OUT_FILE_PATH));
+            write_int_map_file(f.get(), OUT_FILE_PATH, j.ranges, j.nPts, r.r_map.data(), output);
+            break;
+        }
+        case CHAOS_JOB_BIF1D: {
+            TextFile f(OUT_FILE_PATH);
+            write_range_header(f, j.ranges, 1, 6);
+            write_bif1d_body(f, j.ranges, j.nPts, r.r_counts.data(), r.r_first.data(), r.r_peaks.data(),
+                             r.r_intervals.data());
+            f.close();
+            break;
+        }
+        case CHAOS_JOB_BASINS2D: {
+            const int nfeat = j.semantics == CHAOS_SEM_REFERENCE ? 2 : 3;
+            write_basins_headers(OUT_FILE_PATH, j.ranges, nfeat, output);
+            write_basins_bodies(OUT_FILE_PATH, j.ranges, j.nPts, nfeat, r.r_labels.data(), r.r_feats.data(),
+                                r.r_flags.data(), output);
+            break;
+        }
+        default:
+            fail(CHAOS_E_INVALID, "job %lld has unknown kind %d", job_id, j.kind);
+        }
     });
 }
 

@@ -126,6 +126,28 @@ extern "C" int chaos_lle2d(const chaos_options* opt, double tMax, double NT, int
     });
 }
 
+namespace chaos {
+
+void write_lle_file(TextFile* f, const char* path, const double* ranges, int nPts, const double* v, int output)
+{
+    // two range lines, then the map at the default precision with NaN
+    // written as 0 (LLE/src/LLECUDA.cu:252-253, 383-398)
+    MapSpec m;
+    m.ranges = ranges;
+    m.header_lines = 2;
+    m.nPts = nPts;
+    m.count = (long long)nPts * nPts;
+    m.nan_to_zero = true;
+    m.dv = v;
+    if (f) {
+        write_map_text(*f, m);
+        f->close();
+    }
+    if (out_bin(output)) write_map_bin(std::string(path) + ".bin", m);
+}
+
+}  // namespace chaos
+
 extern "C" int chaos_LLE2D(const chaos_options* opt, double tMax, double NT, int nPts, double h,
                            double eps, const double* initialConditions, int amountOfInitialConditions,
                            const double* ranges, const int* indicesOfMutVars, int writableVar,
@@ -146,17 +168,6 @@ extern "C" int chaos_LLE2D(const chaos_options* opt, double tMax, double NT, int
                              indicesOfMutVars, writableVar, maxValue, transientTime, values, amountOfValues,
                              res.data());
         if (rc != CHAOS_OK) throw Error(rc, chaos_last_error());
-        MapSpec m;
-        m.ranges = ranges;
-        m.header_lines = 2;
-        m.nPts = nPts;
-        m.count = n;
-        m.nan_to_zero = true;
-        m.dv = res.data();
-        if (f) {
-            write_map_text(*f, m);
-            f->close();
-        }
-        if (out_bin(o.output)) write_map_bin(std::string(OUT_FILE_PATH) + ".bin", m);
+        write_lle_file(f.get(), OUT_FILE_PATH, ranges, nPts, res.data(), o.output);
     });
 }

@@ -28,4 +28,9 @@ LleArgs make_lle_args(const chaos_options& o, double tMax, double NT, int nPts, 
 double lle_steps_per_point(const LleArgs& p);
 void launch_lle_rows(CallCtx& ctx, int system, LleArgs p, long long row0, long long rows, double* d_out);
 
+class TextFile;
+// The file LLE2D writes for an nPts x nPts map v: CSV into f (already open,
+// or NULL) and/or the binary sidecar <path>.bin, as output (CHAOS_OUT_*) asks.
+void write_lle_file(TextFile* f, const char* path, const double* ranges, int nPts, const double* v, int output);
+
 }  // namespace chaos

@@ -165,11 +165,15 @@ void hip_check(hipError_t e, const char* what, const char* file, int line)
     fail(code, "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
 }
 
+// Stats sink of this thread's calls made without options (the C++ API).
+static thread_local chaos_stats* t_thread_stats = nullptr;
+
 chaos_options resolve_options(const chaos_options* opt)
 {
     std::lock_guard<std::mutex> lk(g_mu);
     init_defaults_locked();
     chaos_options o = opt ? *opt : g_defaults;
+    if (!opt) o.stats = t_thread_stats;
     if (o.min_pts <= 0) o.min_pts = 20;
     return o;
 }
@@ -314,6 +318,12 @@ int chaos_set_default_options(const chaos_options* opt)
     init_defaults_locked();
     g_defaults = *opt;
     g_defaults.stats = nullptr;
+    return CHAOS_OK;
+}
+
+int chaos_set_thread_stats(chaos_stats* stats)
+{
+    t_thread_stats = stats;
     return CHAOS_OK;
 }
 

@@ -113,6 +113,10 @@ int chaos_map_bin_to_csv(const char* bin_path, const char* csv_path);
  * (#define USE_*_MODEL, systems/systems.cuh:9). */
 void        chaos_options_init(chaos_options* opt);
 int         chaos_set_default_options(const chaos_options* opt);
+/* Where calls made on this thread with opt == NULL (the C++ API) report their
+ * chaos_stats; NULL (the default) turns the report off.  The process
+ * defaults never carry a stats pointer. */
+int         chaos_set_thread_stats(chaos_stats* stats);
 int         chaos_api_version(void);
 const char* chaos_last_error(void);
 int         chaos_system_dims(int system, int* nx, int* na);
@@ -454,6 +458,15 @@ int chaos_server_result_bif1d(void* server, long long job_id, int* counts,
                               long long capacity, long long* total);
 int chaos_server_result_basins(void* server, long long job_id, int* labels,
                                double* features, int* flags);
+/* Writes a finished job's results to OUT_FILE_PATH in the files and formats
+ * of the matching file-writing entry point (chaos_LLE2D, chaos_bifurcation1D,
+ * chaos_bifurcation2D, chaos_basinsOfAttraction_2 -- the reference's
+ * LLE/src/LLECUDA.cu:383-398, Bifurcation/src/bifurcationCUDA.cu:220-233,
+ * 746-763 and Basins/src/basinsCUDA.cu:557-590, 708-792 writers); output is
+ * CHAOS_OUT_* as in chaos_options.output (0 = CSV).  The bytes equal the
+ * one-shot entry point's for the same job. */
+int chaos_server_write_result(void* server, long long job_id,
+                              const char* OUT_FILE_PATH, int output);
 int chaos_server_release(void* server, long long job_id);
 
 /* Seeded mixed-job stream: Poisson arrivals at rate lambda for duration_s,
