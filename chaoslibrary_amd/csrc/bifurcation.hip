@@ -294,71 +294,270 @@ __global__ __launch_bounds__(64) void dbscan1d_big_kernel(const int* __restrict_
 }
 
 // REFERENCE 2-D: Bifurcation/src/bifurcationCUDA.cu:516-572 over the kept
-// intervals.  The outer loop (with its stack quirk) is wave-uniform; each
-// pass over k is split across the 64 lanes, which is exact because pass k
-// only reads and writes helpful[k] and helpful[P+k].  distance() there sees
-// zeroed peak values, so hypotf(0, dy) == |float(dy)| (:575-583).
-__global__ __launch_bounds__(64) void dbscan_ref_kernel(const int* __restrict__ flags,
-                                                        const int* __restrict__ counts,
-                                                        const long long* __restrict__ off,
-                                                        const unsigned* __restrict__ times, double hp, double eps,
-                                                        int* __restrict__ helpful_all, int* __restrict__ out,
-                                                        long long n)
+// intervals of one point, one wave per point.
+//
+// The reference walks every k per pass (O(P^2)).  Here the decisions are the
+// same, made in bulk:
+//  * distance() sees zeroed peak values, so it is hypotf(0, dy) =
+//    |float(dy)| (:575-583) and depends only on the two intervals; an
+//    interval is (t[k+1]-t[k]) * hp, a function of the integer sample gap
+//    dt.  Indices are grouped into buckets of equal dt, sorted by dt.
+//  * a pass centred on y_i assigns every unassigned candidate (k < P-1, the
+//    reference's `k < amountOfPeaks - 1`) whose interval is within eps: whole
+//    buckets, and since |float(y_b - y_i)| grows monotonically away from
+//    y_i, a contiguous run of them around y_i.  Buckets left without
+//    unassigned candidates are unlinked (union-find "nearest live bucket"
+//    links), so a pass costs the buckets it empties plus two.
+//  * helpful[k < P] only goes from 0 to a cluster number, so a second pass
+//    centred on the same interval value finds nothing; such passes (most
+//    stack pops) are skipped, and runs of them are taken 64 at a time: a pop
+//    reads slot P+NumNeibor-1, clears it and moves the loop index, exactly as
+//    the reference's quirky stack does.
+// T is the index type of the scratch arrays (16-bit in LDS, 32-bit in global
+// memory); keys holds C >= P 64-bit slots (C a power of two).
+template <typename T>
+struct RefScratch {
+    unsigned long long* keys;  // (dt << 32) | k, sorted
+    T* h;                      // [2P] the reference's helpfulArray
+    T* bucket_of;              // [P] bucket of index k
+    T* bstart;                 // [nb + 1] first sorted position of bucket b
+    T* lp;                     // [nb] nearest live bucket at or left of b (or b-1 chain)
+    T* rp;                     // [nb] same to the right
+    unsigned char* expd;       // [nb] a pass centred on bucket b has run
+};
+
+template <typename T>
+__device__ __forceinline__ int ref_find_left(T* lp, int b)
 {
-    const long long p = blockIdx.x;
-    if (p >= n) return;
-    const int fl = flags[p];
-    if (fl != 1) {
-        if (threadIdx.x == 0) out[p] = fl == -1 ? -1 : 0;  // dbscanCUDA, :387-395
-        return;
+    while (b >= 0) {
+        const int p = __builtin_amdgcn_readfirstlane((int)lp[b]);
+        if (p == b) return b;
+        if (p >= 0) {
+            const int g = __builtin_amdgcn_readfirstlane((int)lp[p]);
+            if (g != p && threadIdx.x == 0) lp[b] = (T)g;  // path halving
+        }
+        b = p;
     }
-    const int P = counts[p] > 1 ? counts[p] - 1 : 0;
-    if (P <= 0 || P == 1) {
-        if (threadIdx.x == 0) out[p] = P;  // :521-525
-        return;
+    return -1;
+}
+
+template <typename T>
+__device__ __forceinline__ int ref_find_right(T* rp, int b, int nb)
+{
+    while (b < nb) {
+        const int p = __builtin_amdgcn_readfirstlane((int)rp[b]);
+        if (p == b) return b;
+        if (p < nb) {
+            const int g = __builtin_amdgcn_readfirstlane((int)rp[p]);
+            if (g != p && threadIdx.x == 0) rp[b] = (T)g;
+        }
+        b = p;
     }
-    const unsigned* t = times + off[p];
-    int* helpful = helpful_all + 2 * off[p];
-    for (int k = threadIdx.x; k < 2 * P; k += blockDim.x) helpful[k] = 0;
+    return nb;
+}
+
+template <typename T>
+__device__ int dbscan_ref_wave(const unsigned* __restrict__ t, int P, double hp, double eps, RefScratch<T> S,
+                               int C)
+{
+    const int lane = threadIdx.x;
+    unsigned long long* keys = S.keys;
+    for (int s = lane; s < C; s += 64)
+        keys[s] = s < P ? ((unsigned long long)(t[s + 1] - t[s]) << 32) | (unsigned)s : ~0ull;
+    for (int k = lane; k < 2 * P; k += 64) S.h[k] = 0;
     __syncthreads();
-    int cluster = 0, NumNeibor = 0;
-    auto iv = [&](int k) { return (double)((long long)t[k + 1] - (long long)t[k]) * hp; };
-    auto pass = [&](int i) {
-        const double yi = iv(i);
+    bitonic_sort(keys, C);
+    // buckets of equal dt over the sorted positions [0, P)
+    int nb = 0;
+    for (int base = 0; base < P; base += 64) {
+        const int s = base + lane;
+        bool head = false;
+        if (s < P) head = s == 0 || (keys[s] >> 32) != (keys[s - 1] >> 32);
+        const unsigned long long m = __ballot(head);
+        const int bid = nb + (int)__popcll(m & ((2ull << lane) - 1ull)) - 1;
+        if (s < P) {
+            if (head) S.bstart[bid] = (T)s;
+            S.bucket_of[(int)(keys[s] & 0xffffffffull)] = (T)bid;
+        }
+        nb += (int)__popcll(m);
+    }
+    __syncthreads();
+    if (lane == 0) S.bstart[nb] = (T)P;
+    for (int b = lane; b < nb; b += 64) S.lp[b] = 0;
+    __syncthreads();
+    // live: the bucket holds a candidate (k < P-1)
+    for (int s = lane; s < P; s += 64)
+        if ((int)(keys[s] & 0xffffffffull) < P - 1) S.lp[(int)S.bucket_of[(int)(keys[s] & 0xffffffffull)]] = 1;
+    __syncthreads();
+    for (int b = lane; b < nb; b += 64) {
+        const bool live = S.lp[b] != 0;
+        S.lp[b] = (T)(live ? b : b - 1);
+        S.rp[b] = (T)(live ? b : b + 1);
+        S.expd[b] = 0;
+    }
+    __syncthreads();
+
+    int cluster = 0, num = 0;
+    // every unassigned candidate of bucket b joins the cluster; b is unlinked
+    auto assign = [&](int b, int i) -> int {
+        const int s0 = S.bstart[b], s1 = S.bstart[b + 1];
         int found = 0;
-        for (int k = threadIdx.x; k < P - 1; k += blockDim.x) {
-            if (i != k && helpful[k] == 0) {
-                const double yk = iv(k);
-                const double d = (yi == yk) ? 0.0 : (double)fabsf((float)(yk - yi));
-                if (d < eps) {
-                    helpful[k] = cluster;
-                    helpful[P + k] = k;
-                    ++found;
+        for (int s = s0 + lane; s - lane < s1; s += 64) {
+            bool hit = false;
+            if (s < s1) {
+                const int k = (int)(keys[s] & 0xffffffffull);
+                if (k < P - 1 && k != i && S.h[k] == 0) {
+                    S.h[k] = (T)cluster;
+                    S.h[P + k] = (T)k;
+                    hit = true;
                 }
             }
+            found += (int)__popcll(__ballot(hit));
         }
-        for (int o = 32; o > 0; o >>= 1) found += __shfl_down(found, o);
-        NumNeibor += __shfl(found, 0);
+        if (lane == 0 && (int)S.lp[b] == b) {
+            S.lp[b] = (T)(b - 1);
+            S.rp[b] = (T)(b + 1);
+        }
         __syncthreads();
+        return found;
     };
-    for (int i = 0; i < P; i++) {
-        if (NumNeibor >= 1) {
-            i = helpful[P + NumNeibor - 1];
+    auto pass = [&](int i) {
+        const int b0 = __builtin_amdgcn_readfirstlane((int)S.bucket_of[i]);
+        if (__builtin_amdgcn_readfirstlane((int)S.expd[b0])) return;  // found nothing
+        if (lane == 0) S.expd[b0] = 1;
+        const double yi = (double)((long long)t[i + 1] - (long long)t[i]) * hp;
+        auto qualifies = [&](int b) {
+            const double yb = (double)(long long)(keys[(int)S.bstart[b]] >> 32) * hp;
+            const double d = (yb == yi) ? 0.0 : (double)fabsf((float)(yb - yi));
+            return __builtin_amdgcn_readfirstlane((int)(d < eps)) != 0;
+        };
+        int found = 0;
+        if (__builtin_amdgcn_readfirstlane((int)S.lp[b0]) == b0 && qualifies(b0)) found += assign(b0, i);
+        for (int b = ref_find_left(S.lp, b0 - 1); b >= 0 && qualifies(b); b = ref_find_left(S.lp, b - 1))
+            found += assign(b, i);
+        for (int b = ref_find_right(S.rp, b0 + 1, nb); b < nb && qualifies(b); b = ref_find_right(S.rp, b + 1, nb))
+            found += assign(b, i);
+        num += found;
+    };
+
+    int i = 0;
+    while (i < P) {
+        if (num >= 1) {
+            // the next up-to-64 pops: slot P+num-1-q holds index num-1-q or 0
+            const int cnt = num < 64 ? num : 64;
+            int v = 0;
+            bool eff = false;
+            if (lane < cnt) {
+                v = (int)S.h[P + num - 1 - lane];
+                eff = S.expd[(int)S.bucket_of[v]] == 0;
+            }
+            const unsigned long long em = __ballot(eff);
+            const int q0 = em ? (int)__builtin_ctzll(em) : cnt;
+            if (lane < q0) S.h[P + num - 1 - lane] = 0;
+            if (q0 > 0) {
+                // q0 pops whose passes find nothing: the loop index ends at
+                // the last popped value, then steps on
+                i = __shfl(v, q0 - 1) + 1;
+                num -= q0;
+                __syncthreads();
+                continue;
+            }
+            i = __builtin_amdgcn_readfirstlane(v);
+            if (lane == 0) S.h[P + num - 1] = 0;
+            num -= 1;
             __syncthreads();
-            if (threadIdx.x == 0) helpful[P + NumNeibor - 1] = 0;
-            __syncthreads();
-            NumNeibor = NumNeibor - 1;
             pass(i);
-        } else if (helpful[i] == 0) {
-            __syncthreads();
-            NumNeibor = 0;
+            ++i;
+        } else {
+            // next index still unassigned: a new cluster starts there
+            int j = P;
+            for (int base = i; base < P; base += 64) {
+                const int k = base + lane;
+                const unsigned long long zm = __ballot(k < P && S.h[k] == 0);
+                if (zm) {
+                    j = base + (int)__builtin_ctzll(zm);
+                    break;
+                }
+            }
+            if (j >= P) break;
+            i = j;
             ++cluster;
-            if (threadIdx.x == 0) helpful[i] = cluster;
+            if (lane == 0) S.h[i] = (T)cluster;
             __syncthreads();
             pass(i);
+            ++i;
         }
     }
-    if (threadIdx.x == 0) out[p] = cluster - 1;
+    return cluster - 1;
+}
+
+// Outputs that need no clustering: flag -1 -> -1, 0 -> 0 (dbscanCUDA,
+// :387-395), and P <= 1 kept intervals -> P (:521-525).
+__global__ void dbscan_ref_trivial_kernel(const int* __restrict__ flags, const int* __restrict__ counts,
+                                          int* __restrict__ out, long long n)
+{
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int fl = flags[p];
+    const int P = counts[p] > 1 ? counts[p] - 1 : 0;
+    if (fl != 1) out[p] = fl == -1 ? -1 : 0;
+    else if (P <= 1) out[p] = P;
+}
+
+// LDS bytes of dbscan_ref_lds_kernel for capacity C: 64-bit keys, 16-bit
+// helpful[2C], bucket_of, bstart (C + 1), lp, rp and the expanded flags.
+constexpr size_t ref_lds_bytes(int C) { return (size_t)C * (8 + 2 * 2 + 2 + 2 + 2 + 2 + 1) + 2 + 8; }
+
+template <typename T>
+__device__ __forceinline__ RefScratch<T> ref_scratch(void* base, int C)
+{
+    RefScratch<T> S;
+    S.keys = static_cast<unsigned long long*>(base);
+    S.h = reinterpret_cast<T*>(S.keys + C);
+    S.bucket_of = S.h + 2 * C;
+    S.bstart = S.bucket_of + C;
+    S.lp = S.bstart + C + 1;
+    S.rp = S.lp + C;
+    S.expd = reinterpret_cast<unsigned char*>(S.rp + C);
+    return S;
+}
+
+// The listed points (2 <= P <= C kept intervals each), scratch in LDS.
+__global__ __launch_bounds__(64) void dbscan_ref_lds_kernel(const int* __restrict__ list, int nlist,
+                                                            const int* __restrict__ counts,
+                                                            const long long* __restrict__ off,
+                                                            const unsigned* __restrict__ times, double hp,
+                                                            double eps, int C, int* __restrict__ out)
+{
+    extern __shared__ unsigned long long lds_keys[];
+    const int b = blockIdx.x;
+    if (b >= nlist) return;
+    const int p = list[b];
+    const int P = counts[p] - 1;
+    const int c = dbscan_ref_wave<short>(times + off[p], P, hp, eps, ref_scratch<short>(lds_keys, C), C);
+    if (threadIdx.x == 0) out[p] = c;
+}
+
+// Points with more kept intervals than the largest LDS class: the same with
+// 32-bit scratch in global memory, ref_global_bytes(C) at the given offsets.
+constexpr long long ref_global_bytes(long long C) { return C * (8 + 2 * 4 + 4 + 4 + 4 + 4 + 1) + 4 + 15; }
+
+__global__ __launch_bounds__(64) void dbscan_ref_big_kernel(const long long* __restrict__ big, int nbig,
+                                                            const int* __restrict__ counts,
+                                                            const long long* __restrict__ off,
+                                                            const unsigned* __restrict__ times, double hp,
+                                                            double eps, unsigned char* __restrict__ scratch,
+                                                            const long long* __restrict__ soff,
+                                                            int* __restrict__ out)
+{
+    const int b = blockIdx.x;
+    if (b >= nbig) return;
+    const long long p = big[b];
+    const int P = counts[p] - 1;
+    int C = 1;
+    while (C < P) C <<= 1;
+    const int c = dbscan_ref_wave<int>(times + off[p], P, hp, eps, ref_scratch<int>(scratch + soff[b], C), C);
+    if (threadIdx.x == 0) out[p] = c;
 }
 
 void bif2d_analysis(CallCtx& ctx, int semantics, BifDevice& R, double eps, int* d_out)
@@ -366,11 +565,64 @@ void bif2d_analysis(CallCtx& ctx, int semantics, BifDevice& R, double eps, int* 
     const long long n = R.n;
     ctx.begin();
     if (semantics == CHAOS_SEM_REFERENCE) {
-        DevBuf<int> helpful((size_t)std::max(1LL, 2 * R.total));
-        hipLaunchKernelGGL(dbscan_ref_kernel, dim3((unsigned)n), dim3(64), 0, ctx.stream(), R.flags.get(),
-                           R.counts.get(), R.off.get(), R.times.get(), R.hp, eps, helpful.get(), d_out, n);
+        hipLaunchKernelGGL(dbscan_ref_trivial_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx.stream(),
+                           R.flags.get(), R.counts.get(), d_out, n);
         CHAOS_HIP(hipGetLastError());
-        ctx.sync();
+        // LDS classes C = 256, 1024, 2048 (ref_lds_bytes: 5.4, 21.5, 43 KiB)
+        constexpr int kRefClasses = 3;
+        const int caps[kRefClasses] = {256, 1024, 2048};
+        std::vector<int> cls[kRefClasses];
+        std::vector<long long> big, soff;
+        long long tot = 0;
+        for (long long p = 0; p < n; ++p) {
+            if (R.h_flags[p] != 1) continue;
+            const int P = R.h_counts[p] - 1;
+            if (P < 2) continue;
+            int c = 0;
+            while (c < kRefClasses && caps[c] < P) ++c;
+            if (c < kRefClasses) {
+                cls[c].push_back((int)p);
+            } else {
+                long long C = 1;
+                while (C < P) C <<= 1;
+                big.push_back(p);
+                soff.push_back(tot);
+                tot += (ref_global_bytes(C) + 15) / 16 * 16;
+            }
+        }
+        std::vector<int> all;
+        std::vector<size_t> start(kRefClasses + 1, 0);
+        for (int c = 0; c < kRefClasses; ++c) {
+            start[c] = all.size();
+            all.insert(all.end(), cls[c].begin(), cls[c].end());
+        }
+        start[kRefClasses] = all.size();
+        DevBuf<int> d_list(std::max<size_t>(1, all.size()));
+        if (!all.empty())
+            CHAOS_HIP(hipMemcpyAsync(d_list.get(), all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice,
+                                     ctx.stream()));
+        for (int c = 0; c < kRefClasses; ++c) {
+            const int cnt = (int)(start[c + 1] - start[c]);
+            if (cnt == 0) continue;
+            const int C = caps[c];
+            hipLaunchKernelGGL(dbscan_ref_lds_kernel, dim3((unsigned)cnt), dim3(64), ref_lds_bytes(C), ctx.stream(),
+                               d_list.get() + start[c], cnt, R.counts.get(), R.off.get(), R.times.get(), R.hp, eps, C,
+                               d_out);
+            CHAOS_HIP(hipGetLastError());
+            ctx.add_launch();
+        }
+        if (!big.empty()) {
+            DevBuf<long long> d_big(big.size()), d_soff(soff.size());
+            DevBuf<unsigned char> scratch((size_t)tot);
+            CHAOS_HIP(hipMemcpyAsync(d_big.get(), big.data(), big.size() * 8, hipMemcpyHostToDevice, ctx.stream()));
+            CHAOS_HIP(hipMemcpyAsync(d_soff.get(), soff.data(), soff.size() * 8, hipMemcpyHostToDevice, ctx.stream()));
+            hipLaunchKernelGGL(dbscan_ref_big_kernel, dim3((unsigned)big.size()), dim3(64), 0, ctx.stream(),
+                               d_big.get(), (int)big.size(), R.counts.get(), R.off.get(), R.times.get(), R.hp, eps,
+                               scratch.get(), d_soff.get(), d_out);
+            CHAOS_HIP(hipGetLastError());
+            ctx.sync();
+        }
+        ctx.sync();  // the list buffers go back to the allocator at the end of this scope
     } else {
         hipLaunchKernelGGL(flag_out_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx.stream(), R.flags.get(), d_out,
                            n);

@@ -109,6 +109,41 @@ def test_bif2d_rossler_ref_quirky_dbscan(chaos, oracle):
     assert len(np.unique(got)) > 2
 
 
+@pytest.mark.parametrize("ct", [100.0, 500.0, 1200.0, 3000.0])
+def test_bif2d_ref_segment_classes_match_oracle(chaos, oracle, ct):
+    """REFERENCE per-point dbscan over segments of ~130 to ~4000 kept
+    intervals: the LDS classes (<= 256, 1024, 2048) and the global path."""
+    n = 5
+    ranges = [24.0, 60.0, 1.5, 3.0]
+    print(f"bif2d lorenz ref {n}x{n} CT={ct}", flush=True)
+    st = chaos.Stats()
+    for eps in (0.001, 0.05):
+        got = chaos.bif2d(ct, n, 0.01, [1.0, 1.0, 1.0], ranges, [1, 2], 0, 0.0, 10.0, [10.0, 28.0, 8 / 3], 1, eps,
+                          system="lorenz_rk4", semantics=0, stats=st)
+        want = oracle.bif2d(oracle.SYS["lorenz_rk4"], 0, ct, n, 0.01, [1.0, 1.0, 1.0], ranges, [1, 2], 0, 0.0,
+                            10.0, [10.0, 28.0, 8 / 3], 1, eps)
+        assert np.array_equal(got, want), eps
+    per_point = st.peaks / (n * n)
+    print(f"peaks per point {per_point:.0f}", flush=True)
+    assert per_point > ct / 2
+
+
+def test_bif2d_ref_harness_scale_sampled(chaos, oracle):
+    """REFERENCE map at the regression harness's physics (Lorenz, CT 400,
+    TT 2000, eps 0.001) on a 96^2 grid; 4 sampled rows recomputed by the
+    oracle."""
+    n = 96
+    ranges = [20.0, 200.0, 0.5, 4.0]
+    print(f"bif2d lorenz ref {n}x{n} CT=400 TT=2000", flush=True)
+    got = chaos.bif2d(400.0, n, 0.01, [1.0, 1.0, 1.0], ranges, [1, 2], 0, 10000.0, 2000.0, [10.0, 28.0, 8 / 3], 1,
+                      0.001, system="lorenz_rk4", semantics=0)
+    for j in (0, 31, 64, 95):
+        want = oracle.bif2d(oracle.SYS["lorenz_rk4"], 0, 400.0, n, 0.01, [1.0, 1.0, 1.0], ranges, [1, 2], 0,
+                            10000.0, 2000.0, [10.0, 28.0, 8 / 3], 1, 0.001, row_begin=j, row_end=j + 1)
+        assert np.array_equal(got[j], want[0]), j
+    assert len(np.unique(got)) > 10
+
+
 def test_bifurcation2D_file(chaos, tmp_path):
     p = tmp_path / "bif2d.csv"
     ranges = [20.0, 200.0, 0.5, 4.0]

@@ -110,3 +110,116 @@ def test_bif_ref_float_bound_quirk():
     f_ref, _, _ = O.loop(ros, x_prev, (0.2, 0.2, 5.7), 0.01, 1, 1, 0, f, O.CHK_BIF_REF)
     f_cfg, _, _ = O.loop(ros, x_prev, (0.2, 0.2, 5.7), 0.01, 1, 1, 0, f, O.CHK_CONFIG)
     assert f_cfg == 0 and f_ref == 1
+
+
+def test_reference_dbscan_bucketed_walk_equals_oracle(oracle):
+    """The per-point reference dbscan (Bifurcation/src/bifurcationCUDA.cu:
+    516-572) decided in bulk, as the GPU kernel does: indices grouped by
+    equal sample gap, a pass assigns whole buckets within eps of its centre
+    (a contiguous run, walked through "nearest live bucket" links), a pass
+    centred on an already expanded gap is skipped, and runs of such stack
+    pops are taken up to `batch` at a time.  This model of that procedure
+    must give the oracle's result on random interval sets (duplicates, near
+    ties, eps below / at / above the sample spacing, eps = 0)."""
+    import numpy as np
+
+    def bucketed(dt, hp, eps, batch):
+        P = len(dt)
+        if P <= 1:
+            return P
+        y = [float(d) * hp for d in dt]
+        order = sorted(range(P), key=lambda k: (dt[k], k))
+        bstart, bucket_of, bdt = [], [0] * P, []
+        for s, k in enumerate(order):
+            if s == 0 or dt[k] != dt[order[s - 1]]:
+                bstart.append(s)
+                bdt.append(dt[k])
+            bucket_of[k] = len(bstart) - 1
+        nb = len(bstart)
+        bstart.append(P)
+        live = [any(order[s] < P - 1 for s in range(bstart[b], bstart[b + 1])) for b in range(nb)]
+        lp = [b if live[b] else b - 1 for b in range(nb)]
+        rp = [b if live[b] else b + 1 for b in range(nb)]
+        expanded = [False] * nb
+        h = [0] * (2 * P)
+        st = {"cluster": 0, "num": 0}
+
+        def find(links, b, end):
+            while b != end and links[b] != b:
+                b = links[b]
+            return b
+
+        def qualifies(b, yi):
+            yb = float(bdt[b]) * hp
+            return (0.0 if yb == yi else float(abs(np.float32(yb - yi)))) < eps
+
+        def assign(b, i):
+            found = 0
+            for s in range(bstart[b], bstart[b + 1]):
+                k = order[s]
+                if k < P - 1 and k != i and h[k] == 0:
+                    h[k] = st["cluster"]
+                    h[P + k] = k
+                    found += 1
+            lp[b], rp[b] = b - 1, b + 1
+            return found
+
+        def run_pass(i):
+            b0 = bucket_of[i]
+            if expanded[b0]:
+                return
+            expanded[b0] = True
+            found = 0
+            if lp[b0] == b0 and qualifies(b0, y[i]):
+                found += assign(b0, i)
+            b = find(lp, b0 - 1, -1)
+            while b >= 0 and qualifies(b, y[i]):
+                found += assign(b, i)
+                b = find(lp, b - 1, -1)
+            b = find(rp, b0 + 1, nb)
+            while b < nb and qualifies(b, y[i]):
+                found += assign(b, i)
+                b = find(rp, b + 1, nb)
+            st["num"] += found
+
+        i = 0
+        while i < P:
+            if st["num"] >= 1:
+                m = st["num"]
+                vals = [h[P + m - 1 - q] for q in range(min(batch, m))]
+                eff = [not expanded[bucket_of[v]] for v in vals]
+                q0 = eff.index(True) if True in eff else len(vals)
+                for r in range(q0):
+                    h[P + m - 1 - r] = 0
+                if q0 > 0:
+                    st["num"] -= q0
+                    i = vals[q0 - 1] + 1
+                    continue
+                i = vals[0]
+                h[P + m - 1] = 0
+                st["num"] -= 1
+                run_pass(i)
+            else:
+                while i < P and h[i] != 0:
+                    i += 1
+                if i >= P:
+                    break
+                st["cluster"] += 1
+                h[i] = st["cluster"]
+                run_pass(i)
+            i += 1
+        return st["cluster"] - 1
+
+    rng = np.random.default_rng(11)
+    choices = [lambda P: rng.integers(1, 5, P + 1), lambda P: rng.integers(60, 80, P + 1),
+               lambda P: rng.choice([10, 11, 30, 31, 32], P + 1), lambda P: rng.integers(1, 200, P + 1),
+               lambda P: rng.choice([5, 6, 7, 50, 51], P + 1)]
+    for trial in range(1500):
+        P = int(rng.integers(1, 90))
+        t = np.cumsum(choices[trial % 5](P))
+        dt = [int(t[k + 1]) - int(t[k]) for k in range(P)]
+        hp = (0.01, 0.02, 0.005)[trial % 3]
+        eps = (0.001, 0.01, 0.05, 0.2, 0.011, 0.0)[(trial // 3) % 6]
+        want = oracle.dbscan_ref(np.array([float(d) * hp for d in dt]), eps)
+        for batch in (1, 64):
+            assert bucketed(dt, hp, eps, batch) == want, (trial, P, eps, hp, batch)
