@@ -178,9 +178,11 @@ struct SweepProg {
     static constexpr int ND = NX + 8;  // X, p, dn, first, last, s0, s1, sx, sz
     static constexpr int NI = 4;       // pstart, cnt, prev_t, bits
     double X[NX], a[NA];
-    PeakDetector det;
+    // CONFIG basins use peak values only
+    WavePeakDetector<KIND != KIND_BASINS_CFG> det;
     int cnt, prev_t;
-    bool alive, fixed;
+    unsigned long long alive;  // wave mask of the lanes still integrating
+    bool fixed;
     double s0, s1, sx, sz, first, last;
 
     __device__ __forceinline__ static long long total(const SweepArgs& A) { return (long long)A.S + A.B; }
@@ -223,7 +225,7 @@ struct SweepProg {
         det.reset();
         cnt = 0;
         prev_t = 0;
-        alive = in_range;
+        alive = wave_mask(in_range);
         fixed = false;
         s0 = s1 = sx = sz = first = last = 0.0;
     }
@@ -236,10 +238,8 @@ struct SweepProg {
     __device__ __forceinline__ void check(const SweepArgs& A, int wv)
     {
         const double v = pick<NX, WVT>(X, wv);
-        const bool out = alive && !(fabs(v) < A.L);
-        if (__ballot(out)) {
-            if (out && diverged_exact(v, A.maxValue, A.check)) alive = false;
-        }
+        const unsigned long long out = wave_mask(!(fabs(v) < A.L)) & alive;
+        if (out) alive &= ~wave_mask(lane_bit(out) && diverged_exact(v, A.maxValue, A.check));
     }
 
     // One recorded sample t: peak detection on the sample, the kind's
@@ -257,9 +257,10 @@ struct SweepProg {
         }
         double pv = 0.0;
         int pt = 0;
-        const bool e = (BODY ? det.feed_body(t - 1, v, pv, pt) : det.feed(t, v, pv, pt)) && alive;
+        const unsigned long long em = (BODY ? det.feed_body(t - 1, v, pv, pt) : det.feed(t, v, pv, pt)) & alive;
+        const bool e = lane_bit(em);
         if (KIND == KIND_BIF) {
-            append_peaks(A.pool, cur, wave, e, pv, pt, cnt);
+            append_peaks(A.pool, cur, wave, em, pv, pt, cnt);
             if (e) ++cnt;
         } else if (KIND == KIND_BASINS_REF) {
             // running sums of the kept peaks and intervals
@@ -288,7 +289,7 @@ struct SweepProg {
             for (int k = 0; k < NX; ++k) xp[k] = X[k];
             for (int r = 0; r < A.pre; ++r) Sy::step(X, a, A.sc);
             check<WVT>(A, wv);
-            fixed = alive && fixed_point(step_delta2<NX>(X, xp), A.check);
+            fixed = lane_bit(alive) && fixed_point(step_delta2<NX>(X, xp), A.check);
         } else {
             if (PRE1) Sy::step(X, a, A.sc);
             else
@@ -311,8 +312,7 @@ struct SweepProg {
                 const long long e = g1 < A.S ? g1 : A.S;
                 for (long long s = g; s < e; ++s) {
                     Sy::step(X, a, A.sc);
-                    const bool bad = alive && !isfinite(X[0]);
-                    if (__ballot(bad) && bad) alive = false;
+                    alive &= ~wave_mask(!isfinite(X[0]));
                 }
                 g = e;
             }
@@ -333,7 +333,7 @@ struct SweepProg {
                 for (int k = 0; k < NX; ++k) xp[k] = X[k];
                 Sy::step(X, a, A.sc);
                 check<WVT>(A, wv);
-                if (alive && fixed_point(step_delta2<NX>(X, xp), A.check)) alive = false;
+                alive &= ~wave_mask(fixed_point(step_delta2<NX>(X, xp), A.check));
                 ++g;
             }
         } else if (g < A.S) {
@@ -360,7 +360,7 @@ struct SweepProg {
     __device__ __forceinline__ void finish(const SweepArgs& A, long long q) const
     {
         const int B = A.B;
-        const int flag = alive ? (fixed ? -1 : 1) : 0;
+        const int flag = lane_bit(alive) ? (fixed ? -1 : 1) : 0;
         if (A.flags) A.flags[q] = flag;
         if (KIND == KIND_BIF) {
             A.counts[q] = cnt;
@@ -423,10 +423,10 @@ struct SweepProg {
         cnt = L.i[L.np + q];
         prev_t = L.i[2 * L.np + q];
         const int bits = L.i[3 * L.np + q];
-        alive = bits & 1;
+        alive = wave_mask(bits & 1);
         fixed = bits & 2;
-        det.plat = bits & 4;
-        det.skip = bits & 8;
+        det.plat = wave_mask(bits & 4);
+        det.skip = wave_mask(bits & 8);
     }
     __device__ __forceinline__ void store(const LaneStore& L, long long q) const
     {
@@ -444,7 +444,8 @@ struct SweepProg {
         L.i[q] = det.pstart;
         L.i[L.np + q] = cnt;
         L.i[2 * L.np + q] = prev_t;
-        L.i[3 * L.np + q] = (alive ? 1 : 0) | (fixed ? 2 : 0) | (det.plat ? 4 : 0) | (det.skip ? 8 : 0);
+        L.i[3 * L.np + q] =
+            (lane_bit(alive) ? 1 : 0) | (fixed ? 2 : 0) | (lane_bit(det.plat) ? 4 : 0) | (lane_bit(det.skip) ? 8 : 0);
     }
 };
 

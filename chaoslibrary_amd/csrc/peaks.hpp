@@ -103,6 +103,75 @@ struct PeakDetector {
     }
 };
 
+// Lane mask <-> lane predicate: a compare written straight into a wave mask,
+// and a wave mask used as a lane predicate, both without VALU work.  Every
+// lane of the wave must be active where a mask is formed.
+__device__ __forceinline__ unsigned long long wave_mask(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+__device__ __forceinline__ bool lane_bit(unsigned long long m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
+
+// PeakDetector for a whole wave, as the sweeps use it: the same transitions
+// with the two state flags held as wave masks (SGPRs).  The three tests are
+// compares written into masks, the state machine runs on the scalar unit and
+// the emit mask feeds the wave's append directly -- a per-lane bool state
+// costs two VALU instructions per sample to form each ballot.  All 64 lanes
+// must take every call (wave-uniform control flow).  TIMES: track the
+// plateau start and report peak times (not needed when only peak values are
+// used).
+template <bool TIMES>
+struct WavePeakDetector {
+    double p;                 // last sample d[t-1]
+    double dn;                // d[t-1] - d[t-2]
+    int pstart;               // plateau start index
+    unsigned long long plat;  // lanes scanning a plateau
+    unsigned long long skip;  // lanes whose next index is not a candidate
+
+    __device__ __forceinline__ void reset()
+    {
+        p = 0.0;
+        dn = 0.0;
+        pstart = 0;
+        plat = 0ull;
+        skip = 0ull;
+    }
+
+    // t >= 2, c = t - 1: PeakDetector::feed_body over the wave; returns the
+    // mask of lanes with a peak at c (value val, time time).
+    __device__ __forceinline__ unsigned long long feed_body(int c, double cur, double& val, int& time)
+    {
+        const double dnx = cur - p;
+        const unsigned long long c1 = wave_mask(dn > 1e-13);
+        const unsigned long long c2 = wave_mask(dnx <= 0.0);
+        const unsigned long long c3 = wave_mask(dnx < -1e-13);
+        const unsigned long long cand = ~plat & ~skip & c1 & c2;
+        const unsigned long long act = plat | cand;
+        const unsigned long long emit = act & c3;
+        const unsigned long long flat = c2 & ~c3;
+        val = p;
+        if (TIMES) {
+            time = lane_bit(plat) ? (pstart + c) >> 1 : c;
+            pstart = lane_bit(cand & flat) ? c : pstart;
+        } else {
+            time = 0;
+        }
+        skip = (plat & ~flat) | (cand & c3);
+        plat = act & flat;
+        dn = dnx;
+        p = cur;
+        return emit;
+    }
+
+    // Any t: samples 0 and 1 only fill the history.
+    __device__ __forceinline__ unsigned long long feed(int t, double cur, double& val, int& time)
+    {
+        if (t >= 2) return feed_body(t - 1, cur, val, time);
+        dn = cur - p;
+        p = cur;
+        val = 0.0;
+        time = 0;
+        return 0ull;
+    }
+};
+
 // One stored peak: value, sample index, and (sequence number << 6 | lane).
 struct PeakEntry {
     double v;
@@ -143,12 +212,13 @@ __device__ __forceinline__ void close_chunk(const PeakPool& pool, WaveCursor& cu
     if (cur.chunk < pool.nchunks && (int)__lane_id() == first_lane(act)) pool.fill[cur.chunk] = cur.fill;
 }
 
-// Append the emitting lanes' entries (wave-uniform control flow required).
-__device__ __forceinline__ void append_peaks(const PeakPool& pool, WaveCursor& cur, int wave, bool emit,
-                                             double v, int t, int k)
+// Append the entries of the lanes in mask m (wave-uniform control flow
+// required).
+__device__ __forceinline__ void append_peaks(const PeakPool& pool, WaveCursor& cur, int wave,
+                                             unsigned long long m, double v, int t, int k)
 {
-    const unsigned long long m = __ballot(emit);
     if (m == 0ull) return;
+    const bool emit = lane_bit(m);
     const int n = __popcll(m);
     if (cur.fill + n > kChunk) {
         const unsigned long long act = __ballot(1);
