@@ -110,6 +110,73 @@ def test_chameleon_is_second_order():
     assert all(3.3 < r < 4.7 for r in ratios), (ratios, errs)
 
 
+def ode_basins(t, x, a0, b, *unused):
+    return [np.sin(x[1]) - b * x[0], np.sin(x[2]) - b * x[1], np.sin(x[0]) - b * x[2]]
+
+
+def _order_per_step(sysid, f, params, x0, T, hs, per_step, method="DOP853", norm=np.linalg.norm):
+    """Error ratios of the map applied T / (per_step * h) times against the
+    ODE flow at T (a step of the map advances the flow by per_step * h)."""
+    ref = solve_ivp(f, (0, T), x0, args=params, method=method, rtol=1e-12, atol=1e-14).y[:, -1]
+    errs = [norm(O.steps(sysid, list(x0) + [0.0] * (O.dims(sysid)[0] - len(x0)), params, h,
+                         int(round(T / (per_step * h))))[:len(x0)] - ref) for h in hs]
+    return [errs[i] / errs[i + 1] for i in range(len(errs) - 1)], errs
+
+
+def test_basins_map_converges_to_its_ode():
+    # systems/systems.cuh:67-83: a Gauss-Seidel explicit sweep of length h
+    # followed by a reversed semi-implicit sweep of length h2 = h*(1-a[0])
+    # (rounded to float) of x'=sin(y)-b x, y'=sin(z)-b y, z'=sin(x)-b z; a
+    # step advances the flow by h + h2.  a[0] = 0 makes the two sweeps
+    # adjoint (second order); otherwise first order.  Powers of two keep h2
+    # exact in float.
+    hs = (2.0 ** -7, 2.0 ** -8, 2.0 ** -9, 2.0 ** -10)
+    r2, e2 = _order_per_step(O.SYS["basins"], ode_basins, (0.0, 0.2, 1.4, 15.552, 2.0), (0.3, -1.0, 2.0), 2.0,
+                             hs, 2.0)
+    assert all(3.9 < r < 4.1 for r in r2), (r2, e2)
+    r1, e1 = _order_per_step(O.SYS["basins"], ode_basins, (0.5, 0.2, 1.4, 15.552, 2.0), (0.3, -1.0, 2.0), 3.0,
+                             hs, 1.5)
+    assert all(1.9 < r < 2.1 for r in r1), (r1, e1)
+
+
+def ode_basins2(t, x, a0, a1):
+    return [-x[1], a1 * x[0] + np.sin(x[1])]
+
+
+@pytest.mark.parametrize("a0", [0.5, 0.3])
+def test_basins2_map_converges_to_its_ode(a0):
+    # systems/systems.cuh:85-101: stages of length h*a[0] and h*(1-a[0]) of
+    # x'=-y, y'=a[1] x + sin(y); first order (the float-rounded y between the
+    # stages stays far below the discretisation error at these h).
+    r, e = _order_per_step(O.SYS["basins_2"], ode_basins2, (a0, -1.5), (1.0, 0.5), 2.0,
+                           (2.0 ** -4, 2.0 ** -5, 2.0 ** -6, 2.0 ** -7), 1.0)
+    assert all(1.9 < x < 2.1 for x in r), (r, e)
+
+
+def ode_damir(t, y, A, w):
+    # the circuit of systems/systems.cuh:103-178 written from its equations:
+    # capacitor voltage Vc and memristor state XSV, sinusoidal drive A sin(w t)
+    Vc, XSV = y
+    Vd, Vm = Vc - 0.04, Vc
+    Id = (1.15e-7 * (np.exp(15 * Vd) - np.exp(-15 * Vd)) + 2.1e-5 / 0.09 * Vd * np.exp(-(Vd - 0.09) / 0.09)
+          - 3e-6 * (np.arctan(26 * (Vd - 0.14)) + np.arctan(26 * (Vd + 0.14))))
+    kT = 2.0 * 1.380649e-23 / 1.602176634e-19
+    f1 = 1 / (1 + np.exp(-(Vm - 0.28) * (Vm + 0.12) / kT))
+    f2 = 1 / (1 + np.exp(-(Vm + 0.006) * (Vm - 0.14) / kT))
+    g = XSV / 1434.0 + (1 - XSV) / 1e6
+    return [(A * np.sin(w * t) - Vm * g - Id) / 22e-9, (f1 * (1 - XSV) - (1 - f2) * XSV) / 1e-7]
+
+
+def test_damir_map_is_euler_of_its_ode():
+    # explicit Euler with the time in X[2]: first-order convergence of both
+    # the voltage and the memristor state (mid-transition at T = 2 TAU)
+    r, e = _order_per_step(O.SYS["damir"], ode_damir, (2e-4, 5e6), (0.4, 0.2), 2e-7,
+                           (4e-10, 2e-10, 1e-10, 5e-11), 1.0, method="Radau",
+                           norm=lambda d: np.abs(d) / np.array([1e-6, 1e-4]))
+    for ratios in np.array(r).T:
+        assert all(1.95 < x < 2.05 for x in ratios), (r, e)
+
+
 def test_damir_time_channel_advances_by_h():
     x = O.steps(O.SYS["damir"], [0.1, 0.5, 0.0], (1e-5, 1e5), 1e-10, 1000)
     assert abs(x[2] - 1000 * 1e-10) < 1e-20 and np.all(np.isfinite(x))
