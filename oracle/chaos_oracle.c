@@ -34,7 +34,8 @@ int orc_dims(int sys, int* nx, int* na)
     return 0;
 }
 
-/* systems/systems.cuh:11-39 (USE_CHAMELEON_MODEL); __fma_rn -> fma. */
+/* systems/systems.cuh:11-39 (USE_CHAMELEON_MODEL); __fma_rn -> fma.
+ * Pinned by: second-order convergence to its ODE (a[0] = 0.5) against scipy. */
 static void step_chameleon(double* X, const double* a, double h)
 {
     double h1 = a[0] * h;
@@ -48,7 +49,8 @@ static void step_chameleon(double* X, const double* a, double h)
     X[0] = fma(h2, -a[6] * X[1], X[0]);
 }
 
-/* systems/systems.cuh:41-63 (USE_ROSSLER_MODEL, semi-implicit CD scheme). */
+/* systems/systems.cuh:41-63 (USE_ROSSLER_MODEL, semi-implicit CD scheme).
+ * Pinned by: second-order convergence to the Rossler ODE (a[0] = 0). */
 static void step_rossler_cd(double* x, const double* a, double h)
 {
     double h1 = 0.5 * h + a[0];
@@ -63,7 +65,9 @@ static void step_rossler_cd(double* x, const double* a, double h)
     x[0] = fma(h2, -x[1] - x[2], x[0]);
 }
 
-/* systems/systems.cuh:67-83 (USE_SYSTEM_FOR_BASINS); h2 is a float there. */
+/* systems/systems.cuh:67-83 (USE_SYSTEM_FOR_BASINS); h2 is a float there.
+ * Pinned by: convergence to its ODE (order 2 for a[0] = 0, order 1 otherwise)
+ * against scipy in tests/test_oracle_systems.py. */
 static void step_basins(double* X, const double* a, double h)
 {
     float h2 = (float)(h * (1 - a[0]));
@@ -75,7 +79,8 @@ static void step_basins(double* X, const double* a, double h)
     X[0] = (X[0] + (double)h2 * sin(X[1])) / (1 + (double)h2 * a[1]);
 }
 
-/* systems/systems.cuh:85-101 (USE_SYSTEM_FOR_BASINS_2); h1, h2, z are floats. */
+/* systems/systems.cuh:85-101 (USE_SYSTEM_FOR_BASINS_2); h1, h2, z are floats.
+ * Pinned by: first-order convergence to its ODE against scipy. */
 static void step_basins2(double* X, const double* a, double h)
 {
     float h1 = (float)(h * a[0]);
@@ -87,7 +92,9 @@ static void step_basins2(double* X, const double* a, double h)
     X[0] = X[0] + (double)h2 * (-X[1]);
 }
 
-/* systems/systems.cuh:103-178 (USE_DAMIR_SYSTEM, explicit Euler, time in X[2]). */
+/* systems/systems.cuh:103-178 (USE_DAMIR_SYSTEM, explicit Euler, time in X[2]).
+ * Pinned by: first-order convergence of both states to the circuit ODE
+ * (scipy Radau) and the exact time channel. */
 static void step_damir(double* X, const double* a, double h)
 {
     double Vc = X[0], XSV = X[1], t = X[2];
