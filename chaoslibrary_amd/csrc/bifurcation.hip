@@ -37,9 +37,9 @@ __global__ void scatter_kernel(const PeakPool pool, unsigned used, const int* __
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const unsigned c = (unsigned)(g / kChunk);
     const int s = (int)(g % kChunk);
-    if (c >= used || s >= pool.fill[c]) return;
+    if (c >= used || s >= pool.fill()[c]) return;
     const PeakEntry e = pool.entries[(size_t)c * kChunk + s];
-    const long long p = (long long)pool.owner[c] * 64 + (e.kl & 63u);
+    const long long p = (long long)pool.owner()[c] * 64 + (e.kl & 63u);
     if (p >= n || flags[p] != 1) return;
     const long long pos = off[p] + (e.kl >> 6);
     vals[pos] = e.v;
@@ -50,22 +50,16 @@ void PeakPoolBuf::alloc(hipStream_t st, long long chunks)
 {
     nchunks = chunks;
     entries.alloc((size_t)chunks * kChunk);
-    owner.alloc((size_t)chunks);
-    fill.alloc((size_t)chunks);
-    ovf.alloc(1);
-    head.alloc(1);
-    CHAOS_HIP(hipMemsetAsync(head.get(), 0, sizeof(unsigned), st));
-    CHAOS_HIP(hipMemsetAsync(ovf.get(), 0, sizeof(int), st));
+    meta.alloc((size_t)(2 * chunks + 2));
+    // head and overflow start at zero
+    CHAOS_HIP(hipMemsetAsync(meta.get() + 2 * chunks, 0, 2 * sizeof(int), st));
 }
 
 PeakPool PeakPoolBuf::view() const
 {
     PeakPool p;
     p.entries = entries.get();
-    p.owner = owner.get();
-    p.fill = fill.get();
-    p.head = head.get();
-    p.overflow = ovf.get();
+    p.meta = meta.get();
     p.nchunks = (unsigned)nchunks;
     return p;
 }
@@ -142,8 +136,9 @@ bool bif_post(CallCtx& ctx, BifDevice& R, PeakPoolBuf& pool)
     const long long n = R.n;
     unsigned used = 0;
     int overflow = 0;
-    CHAOS_HIP(hipMemcpyAsync(&used, pool.head.get(), sizeof used, hipMemcpyDeviceToHost, ctx.stream()));
-    CHAOS_HIP(hipMemcpyAsync(&overflow, pool.ovf.get(), sizeof overflow, hipMemcpyDeviceToHost, ctx.stream()));
+    const PeakPool v = pool.view();
+    CHAOS_HIP(hipMemcpyAsync(&used, v.head(), sizeof used, hipMemcpyDeviceToHost, ctx.stream()));
+    CHAOS_HIP(hipMemcpyAsync(&overflow, v.overflow(), sizeof overflow, hipMemcpyDeviceToHost, ctx.stream()));
     R.h_flags.resize(n);
     R.h_counts.resize(n);
     R.h_first.resize(n);

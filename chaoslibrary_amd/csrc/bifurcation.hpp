@@ -36,8 +36,7 @@ struct BifDevice {
 // Chunk pool backing a PeakPool view.
 struct PeakPoolBuf {
     DevBuf<PeakEntry> entries;
-    DevBuf<int> owner, fill, ovf;
-    DevBuf<unsigned> head;
+    DevBuf<int> meta;  // PeakPool::meta
     long long nchunks = 0;
     void alloc(hipStream_t st, long long chunks);
     PeakPool view() const;

@@ -184,13 +184,16 @@ constexpr int kChunk = 256;  // entries per chunk (4 KiB)
 // Pool of fixed-size chunks handed out to waves by an atomic head.  Each
 // wave appends its lanes' peaks in time order with one ballot + mbcnt per
 // sample that produced any peak, so a chunk holds one wave's entries densely.
+// The bookkeeping sits in one array meta[2 * nchunks + 2] (owner, fill,
+// head, overflow), so a sweep loop holds one pointer for it, not four.
 struct PeakPool {
     PeakEntry* entries;     // [nchunks * kChunk]
-    int* owner;             // [nchunks] global wave id
-    int* fill;              // [nchunks] entries used
-    unsigned* head;         // next free chunk
-    int* overflow;          // set when the pool ran out
+    int* meta;              // owner[nchunks], fill[nchunks], head, overflow
     unsigned nchunks;
+    __host__ __device__ int* owner() const { return meta; }                    // global wave id per chunk
+    __host__ __device__ int* fill() const { return meta + nchunks; }           // entries used per chunk
+    __host__ __device__ unsigned* head() const { return (unsigned*)(meta + 2 * (size_t)nchunks); }  // next free chunk
+    __host__ __device__ int* overflow() const { return meta + 2 * (size_t)nchunks + 1; }  // pool ran out
 };
 
 struct WaveCursor {
@@ -209,7 +212,7 @@ __device__ __forceinline__ int first_lane(unsigned long long active) { return __
 __device__ __forceinline__ void close_chunk(const PeakPool& pool, WaveCursor& cur)
 {
     const unsigned long long act = __ballot(1);
-    if (cur.chunk < pool.nchunks && (int)__lane_id() == first_lane(act)) pool.fill[cur.chunk] = cur.fill;
+    if (cur.chunk < pool.nchunks && (int)__lane_id() == first_lane(act)) pool.fill()[cur.chunk] = cur.fill;
 }
 
 // Append the entries of the lanes in mask m (wave-uniform control flow
@@ -223,15 +226,15 @@ __device__ __forceinline__ void append_peaks(const PeakPool& pool, WaveCursor& c
     if (cur.fill + n > kChunk) {
         const unsigned long long act = __ballot(1);
         const bool leader = (int)__lane_id() == first_lane(act);
-        if (cur.chunk < pool.nchunks && leader) pool.fill[cur.chunk] = cur.fill;
+        if (cur.chunk < pool.nchunks && leader) pool.fill()[cur.chunk] = cur.fill;
         unsigned nc = 0;
-        if (leader) nc = atomicAdd(pool.head, 1u);
+        if (leader) nc = atomicAdd(pool.head(), 1u);
         // the leader is the first active lane: the cursor stays wave-uniform (SGPRs)
         nc = __builtin_amdgcn_readfirstlane(nc);
         if (nc < pool.nchunks) {
-            if (leader) pool.owner[nc] = wave;
+            if (leader) pool.owner()[nc] = wave;
         } else {
-            if (leader) atomicExch(pool.overflow, 1);
+            if (leader) atomicExch(pool.overflow(), 1);
             nc = pool.nchunks;
         }
         cur.chunk = nc;
