@@ -139,13 +139,23 @@ bool bif_post(CallCtx& ctx, BifDevice& R, PeakPoolBuf& pool)
     const PeakPool v = pool.view();
     CHAOS_HIP(hipMemcpyAsync(&used, v.head(), sizeof used, hipMemcpyDeviceToHost, ctx.stream()));
     CHAOS_HIP(hipMemcpyAsync(&overflow, v.overflow(), sizeof overflow, hipMemcpyDeviceToHost, ctx.stream()));
-    R.h_flags.resize(n);
-    R.h_counts.resize(n);
-    R.h_first.resize(n);
-    CHAOS_HIP(hipMemcpyAsync(R.h_flags.data(), R.flags.get(), n * sizeof(int), hipMemcpyDeviceToHost, ctx.stream()));
-    CHAOS_HIP(hipMemcpyAsync(R.h_counts.data(), R.counts.get(), n * sizeof(int), hipMemcpyDeviceToHost, ctx.stream()));
-    CHAOS_HIP(hipMemcpyAsync(R.h_first.data(), R.first.get(), n * sizeof(double), hipMemcpyDeviceToHost, ctx.stream()));
     ctx.sync();
+    if (overflow || R.need_host) {
+        // the 1-D results are assembled on the host; an overflow sizes the
+        // rerun's pool from the exact per-point counts
+        R.h_counts.resize(n);
+        CHAOS_HIP(hipMemcpyAsync(R.h_counts.data(), R.counts.get(), n * sizeof(int), hipMemcpyDeviceToHost,
+                                 ctx.stream()));
+        if (R.need_host) {
+            R.h_flags.resize(n);
+            R.h_first.resize(n);
+            CHAOS_HIP(hipMemcpyAsync(R.h_flags.data(), R.flags.get(), n * sizeof(int), hipMemcpyDeviceToHost,
+                                     ctx.stream()));
+            CHAOS_HIP(hipMemcpyAsync(R.h_first.data(), R.first.get(), n * sizeof(double), hipMemcpyDeviceToHost,
+                                     ctx.stream()));
+        }
+        ctx.sync();
+    }
     if (overflow) return false;
     ctx.begin();
     DevBuf<long long> kept((size_t)n);
@@ -265,11 +275,87 @@ __global__ __launch_bounds__(64) void dbscan1d_list_kernel(const int* __restrict
     if (threadIdx.x == 0) out[p] = c;
 }
 
+// Bitonic sort of a wave's 64 * R values held in registers: element
+// e = lane * R + r.  Pairs closer than R sit in one lane (compare-exchange
+// of registers); farther pairs are one lane-xor apart and swap through a
+// shuffle, each lane keeping the minimum or the maximum.
+template <int R>
+__device__ __forceinline__ void wave_bitonic(double (&v)[R])
+{
+    const int lane = threadIdx.x & 63;
+    constexpr int M = 64 * R;
+#pragma unroll
+    for (int size = 2; size <= M; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            if (stride >= R) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int e = lane * R + r;
+                    const double o = __shfl_xor(v[r], stride / R);
+                    const bool take_min = ((e & stride) == 0) == ((e & size) == 0);
+                    const bool less = v[r] < o;
+                    const double mn = less ? v[r] : o;
+                    const double mx = less ? o : v[r];
+                    v[r] = take_min ? mn : mx;
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if ((r & stride) == 0) {
+                        const bool up = ((lane * R + r) & size) == 0;
+                        const double a = v[r], b = v[r + stride];
+                        const bool gt = a > b;
+                        const double lo = gt ? b : a, hi = gt ? a : b;
+                        v[r] = up ? lo : hi;
+                        v[r + stride] = up ? hi : lo;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// CONFIG 2-D count for the listed points with P <= 64 * R kept peaks, the
+// sort held in registers (no LDS): one wave per point, then
+// 1 + #{k : v[k] - v[k-1] > eps} over the sorted values.
+template <int R>
+__global__ __launch_bounds__(64) void dbscan1d_reg_kernel(const int* __restrict__ list, int nlist,
+                                                          const int* __restrict__ counts,
+                                                          const long long* __restrict__ off,
+                                                          const double* __restrict__ vals, double eps,
+                                                          int* __restrict__ out)
+{
+    const int b = blockIdx.x;
+    if (b >= nlist) return;
+    const int p = list[b];
+    const int P = counts[p] - 1;
+    const double* src = vals + off[p] + 1;
+    const int lane = threadIdx.x & 63;
+    double v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int e = lane * R + r;
+        v[r] = e < P ? src[e] : INFINITY;
+    }
+    wave_bitonic<R>(v);
+    const double prev = __shfl_up(v[R - 1], 1);  // last element of the lane below
+    int gaps = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int e = lane * R + r;
+        const double before = r == 0 ? prev : v[r - 1];
+        if (e >= 1 && e < P && v[r] - before > eps) ++gaps;
+    }
+    for (int o = 32; o > 0; o >>= 1) gaps += __shfl_down(gaps, o);
+    if (lane == 0) out[p] = 1 + gaps;
+}
+
 __global__ __launch_bounds__(64) void dbscan1d_big_kernel(const int* __restrict__ counts,
                                                           const long long* __restrict__ off,
                                                           const double* __restrict__ vals,
-                                                          const long long* __restrict__ big, int nbig,
-                                                          double* __restrict__ scratch,
+                                                          const int* __restrict__ big, int nbig,
+                                                          unsigned char* __restrict__ scratch,
                                                           const long long* __restrict__ scratch_off, double eps,
                                                           int* __restrict__ out)
 {
@@ -279,7 +365,7 @@ __global__ __launch_bounds__(64) void dbscan1d_big_kernel(const int* __restrict_
     const int P = counts[p] - 1;
     int M = 1;
     while (M < P) M <<= 1;
-    double* w = scratch + scratch_off[b];
+    double* w = reinterpret_cast<double*>(scratch + scratch_off[b]);
     const double* v = vals + off[p] + 1;
     for (int k = threadIdx.x; k < M; k += blockDim.x) w[k] = k < P ? v[k] : INFINITY;
     __syncthreads();
@@ -537,7 +623,7 @@ __global__ __launch_bounds__(64) void dbscan_ref_lds_kernel(const int* __restric
 // 32-bit scratch in global memory, ref_global_bytes(C) at the given offsets.
 constexpr long long ref_global_bytes(long long C) { return C * (8 + 2 * 4 + 4 + 4 + 4 + 4 + 1) + 4 + 15; }
 
-__global__ __launch_bounds__(64) void dbscan_ref_big_kernel(const long long* __restrict__ big, int nbig,
+__global__ __launch_bounds__(64) void dbscan_ref_big_kernel(const int* __restrict__ big, int nbig,
                                                             const int* __restrict__ counts,
                                                             const long long* __restrict__ off,
                                                             const unsigned* __restrict__ times, double hp,
@@ -555,129 +641,165 @@ __global__ __launch_bounds__(64) void dbscan_ref_big_kernel(const long long* __r
     if (threadIdx.x == 0) out[p] = c;
 }
 
+// Size classes of the per-point 2-D analysis: class k < ncls holds the
+// flag-1 points with min_p <= P <= cap[k] kept peaks (and above cap[k-1]);
+// class ncls the longer ones, each with a scratch region of big_bytes(P).
+struct ClassCaps {
+    int cap[8];
+    int ncls;
+    int min_p;
+    int big_mode;  // 0: next_pow2(P) doubles (CONFIG), 1: ref_global_bytes (REFERENCE)
+};
+
+__device__ __forceinline__ long long class_big_bytes(const ClassCaps& K, int P)
+{
+    long long C = 1;
+    while (C < P) C <<= 1;
+    const long long b = K.big_mode == 0 ? C * (long long)sizeof(double) : ref_global_bytes(C);
+    return (b + 15) / 16 * 16;
+}
+
+// Lists every point in its class (order within a class is arbitrary: each
+// point's result depends on its own peaks only).  Wave-aggregated appends;
+// every lane of the grid takes part in the ballots.
+__global__ __launch_bounds__(256) void class_lists_kernel(const int* __restrict__ flags,
+                                                          const int* __restrict__ counts, long long n,
+                                                          const ClassCaps K, int* __restrict__ lists,
+                                                          long long* __restrict__ soff, int* __restrict__ lcount,
+                                                          unsigned long long* __restrict__ big_bytes)
+{
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int c = -1, P = 0;
+    if (p < n && flags[p] == 1) {
+        P = counts[p] - 1;
+        if (P >= K.min_p) {
+            c = 0;
+            while (c < K.ncls && K.cap[c] < P) ++c;
+        }
+    }
+    const int lane = threadIdx.x & 63;
+    for (int k = 0; k <= K.ncls; ++k) {
+        const unsigned long long m = __ballot(c == k);
+        if (m == 0ull) continue;
+        const int leader = __ffsll((long long)m) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&lcount[k], __popcll(m));
+        base = __shfl(base, leader);
+        if (c == k) {
+            const int pos = base + (int)__popcll(m & ((1ull << lane) - 1ull));
+            lists[(long long)k * n + pos] = (int)p;
+            if (k == K.ncls) soff[pos] = (long long)atomicAdd(big_bytes, (unsigned long long)class_big_bytes(K, P));
+        }
+    }
+}
+
+__global__ void zero_class_counts_kernel(int* __restrict__ lcount, int m, unsigned long long* __restrict__ big)
+{
+    if ((int)threadIdx.x < m) lcount[threadIdx.x] = 0;
+    if (threadIdx.x == 0) *big = 0ull;
+}
+
+namespace {
+struct ClassLists {
+    DevBuf<int> lists;         // [(ncls + 1) * n]
+    DevBuf<long long> soff;    // [n] scratch offsets of the long ones
+    DevBuf<int> lcount;        // [ncls + 1]
+    DevBuf<unsigned long long> big;
+    int count[9] = {0};
+    unsigned long long big_bytes = 0;
+};
+}  // namespace
+
+// Builds the class lists on the device and reads back their sizes.
+static void class_lists(CallCtx& ctx, const BifDevice& R, const ClassCaps& K, ClassLists& L)
+{
+    const long long n = R.n;
+    L.lists.alloc((size_t)(K.ncls + 1) * n);
+    L.soff.alloc((size_t)n);
+    L.lcount.alloc((size_t)K.ncls + 1);
+    L.big.alloc(1);
+    hipLaunchKernelGGL(zero_class_counts_kernel, dim3(1), dim3(64), 0, ctx.stream(), L.lcount.get(), K.ncls + 1,
+                       L.big.get());
+    hipLaunchKernelGGL(class_lists_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx.stream(), R.flags.get(),
+                       R.counts.get(), n, K, L.lists.get(), L.soff.get(), L.lcount.get(), L.big.get());
+    CHAOS_HIP(hipGetLastError());
+    CHAOS_HIP(hipMemcpyAsync(L.count, L.lcount.get(), sizeof(int) * (K.ncls + 1), hipMemcpyDeviceToHost,
+                             ctx.stream()));
+    CHAOS_HIP(hipMemcpyAsync(&L.big_bytes, L.big.get(), sizeof L.big_bytes, hipMemcpyDeviceToHost, ctx.stream()));
+    ctx.sync();
+    ctx.add_launch(3);
+}
+
 void bif2d_analysis(CallCtx& ctx, int semantics, BifDevice& R, double eps, int* d_out)
 {
     const long long n = R.n;
     ctx.begin();
+    ClassLists L;
     if (semantics == CHAOS_SEM_REFERENCE) {
         hipLaunchKernelGGL(dbscan_ref_trivial_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx.stream(),
                            R.flags.get(), R.counts.get(), d_out, n);
         CHAOS_HIP(hipGetLastError());
         // LDS classes C = 256, 1024, 2048 (ref_lds_bytes: 5.4, 21.5, 43 KiB)
-        constexpr int kRefClasses = 3;
-        const int caps[kRefClasses] = {256, 1024, 2048};
-        std::vector<int> cls[kRefClasses];
-        std::vector<long long> big, soff;
-        long long tot = 0;
-        for (long long p = 0; p < n; ++p) {
-            if (R.h_flags[p] != 1) continue;
-            const int P = R.h_counts[p] - 1;
-            if (P < 2) continue;
-            int c = 0;
-            while (c < kRefClasses && caps[c] < P) ++c;
-            if (c < kRefClasses) {
-                cls[c].push_back((int)p);
-            } else {
-                long long C = 1;
-                while (C < P) C <<= 1;
-                big.push_back(p);
-                soff.push_back(tot);
-                tot += (ref_global_bytes(C) + 15) / 16 * 16;
-            }
-        }
-        std::vector<int> all;
-        std::vector<size_t> start(kRefClasses + 1, 0);
-        for (int c = 0; c < kRefClasses; ++c) {
-            start[c] = all.size();
-            all.insert(all.end(), cls[c].begin(), cls[c].end());
-        }
-        start[kRefClasses] = all.size();
-        DevBuf<int> d_list(std::max<size_t>(1, all.size()));
-        if (!all.empty())
-            CHAOS_HIP(hipMemcpyAsync(d_list.get(), all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice,
-                                     ctx.stream()));
-        for (int c = 0; c < kRefClasses; ++c) {
-            const int cnt = (int)(start[c + 1] - start[c]);
-            if (cnt == 0) continue;
-            const int C = caps[c];
-            hipLaunchKernelGGL(dbscan_ref_lds_kernel, dim3((unsigned)cnt), dim3(64), ref_lds_bytes(C), ctx.stream(),
-                               d_list.get() + start[c], cnt, R.counts.get(), R.off.get(), R.times.get(), R.hp, eps, C,
-                               d_out);
+        const ClassCaps K{{256, 1024, 2048}, 3, 2, 1};
+        class_lists(ctx, R, K, L);
+        for (int c = 0; c < K.ncls; ++c) {
+            if (L.count[c] == 0) continue;
+            const int C = K.cap[c];
+            hipLaunchKernelGGL(dbscan_ref_lds_kernel, dim3((unsigned)L.count[c]), dim3(64), ref_lds_bytes(C),
+                               ctx.stream(), L.lists.get() + (long long)c * n, L.count[c], R.counts.get(),
+                               R.off.get(), R.times.get(), R.hp, eps, C, d_out);
             CHAOS_HIP(hipGetLastError());
             ctx.add_launch();
         }
-        if (!big.empty()) {
-            DevBuf<long long> d_big(big.size()), d_soff(soff.size());
-            DevBuf<unsigned char> scratch((size_t)tot);
-            CHAOS_HIP(hipMemcpyAsync(d_big.get(), big.data(), big.size() * 8, hipMemcpyHostToDevice, ctx.stream()));
-            CHAOS_HIP(hipMemcpyAsync(d_soff.get(), soff.data(), soff.size() * 8, hipMemcpyHostToDevice, ctx.stream()));
-            hipLaunchKernelGGL(dbscan_ref_big_kernel, dim3((unsigned)big.size()), dim3(64), 0, ctx.stream(),
-                               d_big.get(), (int)big.size(), R.counts.get(), R.off.get(), R.times.get(), R.hp, eps,
-                               scratch.get(), d_soff.get(), d_out);
+        if (L.count[K.ncls] > 0) {
+            DevBuf<unsigned char> scratch((size_t)L.big_bytes);
+            hipLaunchKernelGGL(dbscan_ref_big_kernel, dim3((unsigned)L.count[K.ncls]), dim3(64), 0, ctx.stream(),
+                               L.lists.get() + (long long)K.ncls * n, L.count[K.ncls], R.counts.get(), R.off.get(),
+                               R.times.get(), R.hp, eps, scratch.get(), L.soff.get(), d_out);
             CHAOS_HIP(hipGetLastError());
             ctx.sync();
         }
-        ctx.sync();  // the list buffers go back to the allocator at the end of this scope
     } else {
         hipLaunchKernelGGL(flag_out_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx.stream(), R.flags.get(), d_out,
                            n);
-        // size classes M = 64 << c (c < kClasses) hold the points with
-        // M/2 < P <= M kept peaks (the first class also the smaller ones);
-        // longer segments sort in global memory
-        constexpr int kClasses = 7;  // up to 4096 values = 32 KiB of LDS
-        std::vector<int> cls[kClasses];
-        std::vector<long long> big, soff;
-        long long tot = 0;
-        for (long long p = 0; p < n; ++p) {
-            if (R.h_flags[p] != 1) continue;
-            const int P = R.h_counts[p] - 1;
-            if (P < 1) continue;
-            int c = 0;
-            while (c < kClasses && (64 << c) < P) ++c;
-            if (c < kClasses) {
-                cls[c].push_back((int)p);
-            } else {
-                big.push_back(p);
-                soff.push_back(tot);
-                long long m = 1;
-                while (m < P) m <<= 1;
-                tot += m;
-            }
-        }
-        std::vector<int> all;
-        std::vector<size_t> start(kClasses + 1, 0);
-        for (int c = 0; c < kClasses; ++c) {
-            start[c] = all.size();
-            all.insert(all.end(), cls[c].begin(), cls[c].end());
-        }
-        start[kClasses] = all.size();
-        DevBuf<int> d_list(std::max<size_t>(1, all.size()));
-        if (!all.empty())
-            CHAOS_HIP(hipMemcpyAsync(d_list.get(), all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice,
-                                     ctx.stream()));
-        for (int c = 0; c < kClasses; ++c) {
-            const int cnt = (int)(start[c + 1] - start[c]);
+        // classes M = 64 << c: up to 2048 values the sort runs in registers
+        // (64 * R per wave), 4096 in LDS (32 KiB), longer in global memory
+        const ClassCaps K{{64, 128, 256, 512, 1024, 2048, 4096}, 7, 1, 0};
+        class_lists(ctx, R, K, L);
+        for (int c = 0; c < K.ncls; ++c) {
+            const int cnt = L.count[c];
             if (cnt == 0) continue;
-            const int M = 64 << c;
-            hipLaunchKernelGGL(dbscan1d_list_kernel, dim3((unsigned)cnt), dim3(64), (size_t)M * sizeof(double),
-                               ctx.stream(), d_list.get() + start[c], cnt, R.counts.get(), R.off.get(), R.vals.get(),
-                               eps, M, d_out);
+            const int* lst = L.lists.get() + (long long)c * n;
+            auto reg = [&](auto rtag) {
+                constexpr int RV = decltype(rtag)::value;
+                hipLaunchKernelGGL((dbscan1d_reg_kernel<RV>), dim3((unsigned)cnt), dim3(64), 0, ctx.stream(), lst,
+                                   cnt, R.counts.get(), R.off.get(), R.vals.get(), eps, d_out);
+            };
+            switch (c) {
+            case 0: reg(std::integral_constant<int, 1>{}); break;
+            case 1: reg(std::integral_constant<int, 2>{}); break;
+            case 2: reg(std::integral_constant<int, 4>{}); break;
+            case 3: reg(std::integral_constant<int, 8>{}); break;
+            case 4: reg(std::integral_constant<int, 16>{}); break;
+            case 5: reg(std::integral_constant<int, 32>{}); break;
+            default:
+                hipLaunchKernelGGL(dbscan1d_list_kernel, dim3((unsigned)cnt), dim3(64),
+                                   (size_t)K.cap[c] * sizeof(double), ctx.stream(), lst, cnt, R.counts.get(),
+                                   R.off.get(), R.vals.get(), eps, K.cap[c], d_out);
+            }
             CHAOS_HIP(hipGetLastError());
             ctx.add_launch();
         }
-        if (!big.empty()) {
-            DevBuf<long long> d_big(big.size()), d_soff(soff.size());
-            DevBuf<double> scratch((size_t)tot);
-            CHAOS_HIP(hipMemcpyAsync(d_big.get(), big.data(), big.size() * 8, hipMemcpyHostToDevice, ctx.stream()));
-            CHAOS_HIP(hipMemcpyAsync(d_soff.get(), soff.data(), soff.size() * 8, hipMemcpyHostToDevice, ctx.stream()));
-            hipLaunchKernelGGL(dbscan1d_big_kernel, dim3((unsigned)big.size()), dim3(64), 0, ctx.stream(),
-                               R.counts.get(), R.off.get(), R.vals.get(), d_big.get(), (int)big.size(),
-                               scratch.get(), d_soff.get(), eps, d_out);
+        if (L.count[K.ncls] > 0) {
+            DevBuf<unsigned char> scratch((size_t)L.big_bytes);
+            hipLaunchKernelGGL(dbscan1d_big_kernel, dim3((unsigned)L.count[K.ncls]), dim3(64), 0, ctx.stream(),
+                               R.counts.get(), R.off.get(), R.vals.get(), L.lists.get() + (long long)K.ncls * n,
+                               L.count[K.ncls], scratch.get(), L.soff.get(), eps, d_out);
             CHAOS_HIP(hipGetLastError());
             ctx.sync();
         }
-        ctx.sync();  // the list buffers go back to the allocator at the end of this scope
     }
+    ctx.sync();  // the list buffers go back to the allocator at the end of this scope
     ctx.add_analysis_ms(ctx.end());
     ctx.add_launch();
 }
@@ -822,6 +944,7 @@ static void bif2d_impl(const chaos_options* opt, double tMax, int nPts, double h
     CallCtx ctx(o);
     BifDevice R;
     R.need_times = o.semantics == CHAOS_SEM_REFERENCE;  // the CONFIG count reads peak values only
+    R.need_host = false;                                // the 2-D analysis lists its points on the device
     run_bif(ctx, o, in, R);
     DevBuf<int> tmp;
     int* d_out = out;

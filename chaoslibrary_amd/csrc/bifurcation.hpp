@@ -24,6 +24,7 @@ struct BifDevice {
     long long total = 0;     // stored peaks of flag-1 points (first peak included)
     double hp = 0;           // interval unit
     bool need_times = true;  // scatter the peak times too (intervals)
+    bool need_host = true;   // host copies of flags / counts / first peaks (1-D results)
     DevBuf<int> flags, counts;
     DevBuf<double> first;
     DevBuf<long long> off;   // segment offsets

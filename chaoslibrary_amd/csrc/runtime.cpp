@@ -1,6 +1,7 @@
 // runtime.cpp -- error reporting, options, streams, timing, device helpers.
 #include "runtime.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -50,11 +51,12 @@ namespace {
 struct DevCache {
     std::map<size_t, std::vector<void*>> free_blocks;  // size class -> blocks
     size_t cached_bytes = 0;
+    size_t cap = 0;  // bytes kept: a quarter of the device's memory, at least kCacheCap
 };
 std::mutex g_alloc_mu;
 std::map<int, DevCache> g_cache;                         // device -> cache
 std::map<void*, std::pair<int, size_t>> g_live;          // block -> (device, class)
-constexpr size_t kCacheCap = 16ull << 30;                // bytes kept per device
+constexpr size_t kCacheCap = 16ull << 30;                // least bytes kept per device
 
 size_t size_class(size_t bytes)
 {
@@ -121,7 +123,13 @@ void dev_free(void* p)
     const size_t cls = it->second.second;
     g_live.erase(it);
     DevCache& c = g_cache[dev];
-    if (c.cached_bytes + cls > kCacheCap) {
+    if (c.cap == 0) {
+        // the quarter lets a repeated full-size map reuse its peak store
+        // (tens of GB); a failed hipMalloc empties the cache and retries
+        size_t free_b = 0, total_b = 0;
+        c.cap = hipMemGetInfo(&free_b, &total_b) == hipSuccess ? std::max(kCacheCap, total_b / 4) : kCacheCap;
+    }
+    if (c.cached_bytes + cls > c.cap) {
         (void)hipFree(p);
         return;
     }

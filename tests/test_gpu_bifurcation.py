@@ -109,18 +109,21 @@ def test_bif2d_rossler_ref_quirky_dbscan(chaos, oracle):
     assert len(np.unique(got)) > 2
 
 
-@pytest.mark.parametrize("ct", [100.0, 500.0, 1200.0, 3000.0])
-def test_bif2d_ref_segment_classes_match_oracle(chaos, oracle, ct):
-    """REFERENCE per-point dbscan over segments of ~130 to ~4000 kept
-    intervals: the LDS classes (<= 256, 1024, 2048) and the global path."""
+@pytest.mark.parametrize("sem", [0, 1])
+@pytest.mark.parametrize("ct", [100.0, 500.0, 1200.0, 3000.0, 4500.0])
+def test_bif2d_segment_classes_match_oracle(chaos, oracle, ct, sem):
+    """Per-point 2-D analyses over segments of ~130 to ~6000 kept peaks:
+    every size class (REFERENCE dbscan in LDS <= 256, 1024, 2048 and in
+    global memory; CONFIG count sorted in registers up to 2048, in LDS up to
+    4096 and in global memory)."""
     n = 5
     ranges = [24.0, 60.0, 1.5, 3.0]
-    print(f"bif2d lorenz ref {n}x{n} CT={ct}", flush=True)
+    print(f"bif2d lorenz sem={sem} {n}x{n} CT={ct}", flush=True)
     st = chaos.Stats()
     for eps in (0.001, 0.05):
         got = chaos.bif2d(ct, n, 0.01, [1.0, 1.0, 1.0], ranges, [1, 2], 0, 0.0, 10.0, [10.0, 28.0, 8 / 3], 1, eps,
-                          system="lorenz_rk4", semantics=0, stats=st)
-        want = oracle.bif2d(oracle.SYS["lorenz_rk4"], 0, ct, n, 0.01, [1.0, 1.0, 1.0], ranges, [1, 2], 0, 0.0,
+                          system="lorenz_rk4", semantics=sem, stats=st)
+        want = oracle.bif2d(oracle.SYS["lorenz_rk4"], sem, ct, n, 0.01, [1.0, 1.0, 1.0], ranges, [1, 2], 0, 0.0,
                             10.0, [10.0, 28.0, 8 / 3], 1, eps)
         assert np.array_equal(got, want), eps
     per_point = st.peaks / (n * n)
