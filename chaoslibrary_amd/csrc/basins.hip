@@ -67,13 +67,11 @@ void basins_cluster(CallCtx& ctx, bool ref, long long n, const double* f0, const
         DbscanParams prm{2, eps, 1, true};
         dbscan_device(ctx, pts, flags, n, prm, labels);
     } else {
-        // z-score a copy; the caller keeps the raw features
+        // z-scored into a separate buffer; the caller keeps the raw features
         DevBuf<double> z(3 * (size_t)n);
-        CHAOS_HIP(hipMemcpyAsync(z.get(), f0, n * 8, hipMemcpyDeviceToDevice, ctx.stream()));
-        CHAOS_HIP(hipMemcpyAsync(z.get() + n, f1, n * 8, hipMemcpyDeviceToDevice, ctx.stream()));
-        CHAOS_HIP(hipMemcpyAsync(z.get() + 2 * n, f2, n * 8, hipMemcpyDeviceToDevice, ctx.stream()));
+        const double* src[3] = {f0, f1, f2};
         double* f[3] = {z.get(), z.get() + n, z.get() + 2 * n};
-        zscore_device(ctx, f, 3, flags, n, nullptr);
+        zscore_device(ctx, src, f, 3, flags, n, nullptr);
         const double* pts[3] = {f[0], f[1], f[2]};
         DbscanParams prm{3, eps, minPts, false};
         dbscan_device(ctx, pts, flags, n, prm, labels);

@@ -850,31 +850,54 @@ __global__ void count_mask_kernel(const int* mask, long long n, unsigned long lo
     if ((threadIdx.x & 63) == 0 && b) atomicAdd(cnt, (unsigned long long)__popcll(b));
 }
 
-__global__ void normalize_kernel(double* x, const int* mask, long long n, double mean, double sd)
+// y = z-scored x on the masked points, x elsewhere (y may be x).
+__global__ void normalize_kernel(const double* x, double* y, const int* mask, long long n, double mean, double sd)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || mask[i] == 0) return;
-    x[i] = sd > 0 ? (x[i] - mean) / sd : 0.0;
+    if (i >= n) return;
+    const double v = x[i];
+    y[i] = mask[i] == 0 ? v : (sd > 0 ? (v - mean) / sd : 0.0);
 }
 
-void zscore_device(CallCtx& ctx, double* const* f, int k, const int* mask, long long n, double* moments)
+void zscore_device(CallCtx& ctx, const double* const* src, double* const* dst, int k, const int* mask, long long n,
+                   double* moments)
 {
+    // the block sums of all k features come back in one copy per moment
+    // (two round trips in all), summed on the host in block order
+    const long long nb = (n + 1023) / 1024;
     DevBuf<unsigned long long> c(1);
+    DevBuf<double> bs((size_t)(k * nb));
+    std::vector<double> h((size_t)(k * nb));
+    unsigned long long N = 0;
     CHAOS_HIP(hipMemsetAsync(c.get(), 0, 8, ctx.stream()));
     hipLaunchKernelGGL(count_mask_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx.stream(), mask, n, c.get());
-    unsigned long long N = 0;
-    CHAOS_HIP(hipMemcpyAsync(&N, c.get(), 8, hipMemcpyDeviceToHost, ctx.stream()));
-    CHAOS_HIP(hipStreamSynchronize(ctx.stream()));
-    if (N == 0) return;
-    for (int j = 0; j < k; ++j) {
-        const double mean = det_sum_device(ctx, f[j], mask, n, false, 0.0) / (double)N;
-        const double sd = std::sqrt(det_sum_device(ctx, f[j], mask, n, true, mean) / (double)N);
-        if (moments) {
-            moments[2 * j] = mean;
-            moments[2 * j + 1] = sd;
+    std::vector<double> mean(k), sd(k);
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int j = 0; j < k; ++j)
+            hipLaunchKernelGGL(det_block_sum_kernel, dim3((unsigned)nb), dim3(512), 0, ctx.stream(), src[j], mask, n,
+                               pass == 1, pass == 1 ? mean[j] : 0.0, bs.get() + j * nb);
+        CHAOS_HIP(hipGetLastError());
+        if (pass == 0) CHAOS_HIP(hipMemcpyAsync(&N, c.get(), 8, hipMemcpyDeviceToHost, ctx.stream()));
+        CHAOS_HIP(hipMemcpyAsync(h.data(), bs.get(), k * nb * 8, hipMemcpyDeviceToHost, ctx.stream()));
+        CHAOS_HIP(hipStreamSynchronize(ctx.stream()));
+        if (N == 0) break;
+        for (int j = 0; j < k; ++j) {
+            double tot = 0.0;
+            for (long long b = 0; b < nb; ++b) tot = tot + h[j * nb + b];
+            if (pass == 0) mean[j] = tot / (double)N;
+            else sd[j] = std::sqrt(tot / (double)N);
         }
-        hipLaunchKernelGGL(normalize_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx.stream(), f[j], mask, n, mean,
-                           sd);
+    }
+    for (int j = 0; j < k; ++j) {
+        if (N > 0 && moments) {
+            moments[2 * j] = mean[j];
+            moments[2 * j + 1] = sd[j];
+        }
+        if (N > 0)
+            hipLaunchKernelGGL(normalize_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx.stream(), src[j], dst[j],
+                               mask, n, mean[j], sd[j]);
+        else if (dst[j] != src[j])
+            CHAOS_HIP(hipMemcpyAsync(dst[j], src[j], n * 8, hipMemcpyDeviceToDevice, ctx.stream()));
         CHAOS_HIP(hipGetLastError());
     }
 }

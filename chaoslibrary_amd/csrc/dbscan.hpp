@@ -27,6 +27,7 @@ int dbscan_device(CallCtx& ctx, const double* const* pts, const int* include, lo
 double det_sum_device(CallCtx& ctx, const double* x, const int* mask, long long n, bool sq, double mean);
 
 // In-place z-score of the k feature arrays over points with mask != 0.
-void zscore_device(CallCtx& ctx, double* const* f, int k, const int* mask, long long n, double* moments);
+void zscore_device(CallCtx& ctx, const double* const* src, double* const* dst, int k, const int* mask, long long n,
+                   double* moments);
 
 }  // namespace chaos
