@@ -244,7 +244,7 @@ __device__ __forceinline__ int gap_count(const double* v, int P, double eps)
 }
 
 // CONFIG 2-D, every point: -1 for a fixed point, else 0; the points with
-// kept peaks are overwritten by dbscan1d_list_kernel.
+// kept peaks are overwritten by the 1-D DBSCAN kernels.
 __global__ void flag_out_kernel(const int* __restrict__ flags, int* __restrict__ out, long long n)
 {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
