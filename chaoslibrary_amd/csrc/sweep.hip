@@ -18,7 +18,7 @@
 namespace chaos {
 
 template <int SYS, int WV, int KIND>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Occupancy<SYS>::waves)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Occupancy<SYS>::sweep_min, 8)))
 void sweep_kernel(const SweepArgs A)
 {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;

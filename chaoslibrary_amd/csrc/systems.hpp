@@ -50,17 +50,24 @@ struct System;
 // Waves per SIMD requested from the register allocator for a system's
 // integration kernels: the RK4 systems fit the 64-VGPR budget of 8 waves
 // without spilling; the transcendental maps use the compiler's choice.
+// sweep_min is the lower bound given to the sweep kernels: asked for 8
+// waves, the allocator caps them at 80 SGPRs and spills the peak pool's
+// pointers into VGPR lanes (four v_readlane per append); from 7 it keeps
+// them in ~94 SGPRs and still lands at <= 56 VGPRs, i.e. 8 waves.
 template <int S>
 struct Occupancy {
     static constexpr int waves = 0;
+    static constexpr int sweep_min = 1;
 };
 template <>
 struct Occupancy<5> {
     static constexpr int waves = 8;
+    static constexpr int sweep_min = 7;
 };
 template <>
 struct Occupancy<6> {
     static constexpr int waves = 8;
+    static constexpr int sweep_min = 7;
 };
 
 // systems/systems.cuh:11-39 -- Chameleon, two-stage composition with cos().
