@@ -350,6 +350,12 @@ struct SweepProg {
         const int body_end = te < B - 1 ? te : B - 1;
         for (; t < te && t < 2; ++t) sample<WVT, false, false>(A, wv, cur, wave, t);
         if (A.pre == 1) {
+            // two samples per trip: the detector history and the state
+            // rotate through alternating registers instead of being copied
+            for (; t + 1 < body_end; t += 2) {
+                sample<WVT, true, true>(A, wv, cur, wave, t);
+                sample<WVT, true, true>(A, wv, cur, wave, t + 1);
+            }
             for (; t < body_end; ++t) sample<WVT, true, true>(A, wv, cur, wave, t);
         } else {
             for (; t < body_end; ++t) sample<WVT, true, false>(A, wv, cur, wave, t);
