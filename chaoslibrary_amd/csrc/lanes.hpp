@@ -259,27 +259,31 @@ struct SweepProg {
         int pt = 0;
         const unsigned long long em = (BODY ? det.feed_body(t - 1, v, pv, pt) : det.feed(t, v, pv, pt)) & alive;
         const bool e = lane_bit(em);
-        if (KIND == KIND_BIF) {
-            append_peaks(A.pool, cur, wave, em, pv, pt, cnt);
-            if (e) ++cnt;
-        } else if (KIND == KIND_BASINS_REF) {
-            // running sums of the kept peaks and intervals
-            // (avgPeakFinderCUDA, basinsCUDA.cu:841-861)
-            if (e) {
-                if (cnt >= 1) {
-                    s0 = s0 + pv;
-                    s1 = s1 + (double)(pt - prev_t) * A.hp;
+        // em is wave-uniform: a wave with no peak at this sample skips the
+        // per-peak work altogether
+        if (em) {
+            if (KIND == KIND_BIF) {
+                append_peaks(A.pool, cur, wave, em, pv, pt, cnt);
+                if (e) ++cnt;
+            } else if (KIND == KIND_BASINS_REF) {
+                // running sums of the kept peaks and intervals
+                // (avgPeakFinderCUDA, basinsCUDA.cu:841-861)
+                if (e) {
+                    if (cnt >= 1) {
+                        s0 = s0 + pv;
+                        s1 = s1 + (double)(pt - prev_t) * A.hp;
+                    }
+                    prev_t = pt;
+                    ++cnt;
                 }
-                prev_t = pt;
-                ++cnt;
-            }
-        } else {
-            if (e) {
-                if (cnt >= 1) {
-                    s0 = s0 + pv;
-                    s1 = fma(pv, pv, s1);
+            } else {
+                if (e) {
+                    if (cnt >= 1) {
+                        s0 = s0 + pv;
+                        s1 = fma(pv, pv, s1);
+                    }
+                    ++cnt;
                 }
-                ++cnt;
             }
         }
         if (!BODY && t == A.B - 1) {

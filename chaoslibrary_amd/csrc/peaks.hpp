@@ -148,8 +148,13 @@ struct WavePeakDetector {
         const unsigned long long flat = c2 & ~c3;
         val = p;
         if (TIMES) {
-            time = lane_bit(plat) ? (pstart + c) >> 1 : c;
-            pstart = lane_bit(cand & flat) ? c : pstart;
+            // a peak's time is its sample index, or the middle of the
+            // plateau for a lane that ends one; plateaus are rare, so both
+            // lane updates sit behind wave-uniform branches
+            time = c;
+            if (plat & c3) time = lane_bit(plat) ? (pstart + c) >> 1 : c;
+            const unsigned long long ps = cand & flat;
+            if (ps) pstart = lane_bit(ps) ? c : pstart;
         } else {
             time = 0;
         }
@@ -241,12 +246,14 @@ __device__ __forceinline__ void append_peaks(const PeakPool& pool, WaveCursor& c
         cur.fill = 0;
     }
     if (emit && cur.chunk < pool.nchunks) {
-        const unsigned pos = (unsigned)cur.fill + lane_prefix(m);
+        // wave-uniform chunk base + a 32-bit byte offset inside the chunk
+        const unsigned off = ((unsigned)cur.fill + lane_prefix(m)) * (unsigned)sizeof(PeakEntry);
+        char* base = (char*)(pool.entries + (size_t)cur.chunk * kChunk);
         PeakEntry e;
         e.v = v;
         e.t = (unsigned)t;
         e.kl = ((unsigned)k << 6) | (unsigned)__lane_id();
-        pool.entries[(size_t)cur.chunk * kChunk + pos] = e;
+        *(PeakEntry*)(base + off) = e;
     }
     cur.fill += n;
 }
