@@ -6,10 +6,11 @@
 //   kept_kernel + scan       per-point segment offsets (flag == 1 points)
 //   scatter_kernel           chunk entries -> per-point contiguous (value, time)
 //   1-D: host assembles the kept peaks / intervals (first peak dropped)
-//   2-D: one wave per point -- CONFIG: sort (LDS bitonic, launched per
-//        power-of-two size class so LDS follows the segment) + gap count =
-//        1-D DBSCAN(eps, minPts=1) cluster count; REFERENCE: the reference's
-//        per-point dbscan on the kept intervals.
+//   2-D: one wave per point, points listed on the device by power-of-two
+//        size class -- CONFIG: sort (bitonic over 64 lanes x R registers up
+//        to 2048 peaks, in LDS up to 4096, in global memory beyond) + gap
+//        count = 1-D DBSCAN(eps, minPts=1) cluster count; REFERENCE: the
+//        reference's per-point dbscan on the kept intervals.
 #include <algorithm>
 #include <cmath>
 #include <memory>

@@ -38,12 +38,14 @@ def test_basins_ref_prescaller_and_fixed_points(chaos, oracle):
     assert np.array_equal(lab.reshape(-1), want)
 
 
-@pytest.mark.parametrize("minpts", [20, 5])
-def test_basins_config_matches_oracle(chaos, oracle, minpts):
+# tMax 30 / 30.011: 3000 / 3001 recorded samples (odd and even detector-body
+# counts; the record loop takes two samples per trip)
+@pytest.mark.parametrize("minpts,tmax", [(20, 30.0), (5, 30.0), (20, 30.011)])
+def test_basins_config_matches_oracle(chaos, oracle, minpts, tmax):
     n = 64
     ranges = [-20.0, 20.0, -20.0, 20.0]
-    args = (30.0, n, 0.01, [0.0, 0.0, 25.0], ranges, [0, 1], 0, 0.0, 30.0, LOR, 1)
-    print(f"basins config lorenz {n}x{n} minpts={minpts}", flush=True)
+    args = (tmax, n, 0.01, [0.0, 0.0, 25.0], ranges, [0, 1], 0, 0.0, 30.0, LOR, 1)
+    print(f"basins config lorenz {n}x{n} minpts={minpts} tMax={tmax}", flush=True)
     lab, feat, flags = chaos.basins2d(*args, eps=0.5, system="lorenz_rk4", semantics=1, min_pts=minpts)
     f0, f1, f2, ofl = oracle.basins_features(oracle.SYS["lorenz_rk4"], 1, *args)
     assert np.array_equal(flags.reshape(-1), ofl)

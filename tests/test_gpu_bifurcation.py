@@ -12,10 +12,13 @@ def _oracle_bif1d(oracle, sysname, sem, tMax, nPts, h, ic, ranges, idx, wv, maxv
     return oracle.bif1d(oracle.SYS[sysname], sem, tMax, nPts, h, ic, ranges, idx, wv, maxv, tT, values, pre)
 
 
+# tMax 300 / 300.011: 30000 / 30001 recorded samples, i.e. an odd and an
+# even number of detector-body samples (the record loop takes two per trip)
+@pytest.mark.parametrize("tmax", [300.0, 300.011])
 @pytest.mark.parametrize("sem", [0, 1])
-def test_bif1d_rossler_matches_oracle(chaos, oracle, sem):
-    args = (300.0, 96, 0.01, ROS["ic"], [2.0, 6.0], [2], 0, 0.0, 200.0, ROS["values"], 1)
-    print(f"bif1d rossler 96 sem={sem}", flush=True)
+def test_bif1d_rossler_matches_oracle(chaos, oracle, sem, tmax):
+    args = (tmax, 96, 0.01, ROS["ic"], [2.0, 6.0], [2], 0, 0.0, 200.0, ROS["values"], 1)
+    print(f"bif1d rossler 96 sem={sem} tMax={tmax}", flush=True)
     c, f, pk, iv = chaos.bif1d(*args, system="rossler_rk4", semantics=sem)
     oc, of, opk, oiv = _oracle_bif1d(oracle, "rossler_rk4", sem, *args)
     assert np.array_equal(c, oc)
@@ -85,14 +88,15 @@ def _map_text(ranges, vals, n, fmt):
     return s + "".join(out)
 
 
+@pytest.mark.parametrize("tmax", [30.0, 30.011])  # 3000 / 3001 recorded samples
 @pytest.mark.parametrize("sem", [0, 1])
-def test_bif2d_lorenz_matches_oracle(chaos, oracle, sem):
+def test_bif2d_lorenz_matches_oracle(chaos, oracle, sem, tmax):
     n = 12
     ranges = [20.0, 200.0, 0.5, 4.0]
-    args = (30.0, n, 0.01, [1.0, 1.0, 1.0], ranges, [1, 2], 0, 0.0, 20.0, [10.0, 28.0, 8 / 3], 1, 1e-2)
-    print(f"bif2d lorenz {n}x{n} sem={sem}", flush=True)
+    args = (tmax, n, 0.01, [1.0, 1.0, 1.0], ranges, [1, 2], 0, 0.0, 20.0, [10.0, 28.0, 8 / 3], 1, 1e-2)
+    print(f"bif2d lorenz {n}x{n} sem={sem} tMax={tmax}", flush=True)
     got = chaos.bif2d(*args, system="lorenz_rk4", semantics=sem)
-    want = oracle.bif2d(oracle.SYS["lorenz_rk4"], sem, 30.0, n, 0.01, [1.0, 1.0, 1.0], ranges, [1, 2], 0, 0.0,
+    want = oracle.bif2d(oracle.SYS["lorenz_rk4"], sem, tmax, n, 0.01, [1.0, 1.0, 1.0], ranges, [1, 2], 0, 0.0,
                         20.0, [10.0, 28.0, 8 / 3], 1, 1e-2)
     assert np.array_equal(got, want)
 
