@@ -152,6 +152,10 @@ def run_lle(args):
                      "seed": w["seed"], "rk4_steps_per_map": per_map, "flop_per_step": 63}
     res["roofline"] = roofline(w["system"], per_map, kms / 1e3, "lle_map_kernel<5>", kms,
                                f"lle_map_kernel_rossler_{n}" if n == 2048 else None)
+    # the same rate as RHS evaluations (4 per RK4 step) and algorithmic FLOP
+    res["rhs_evals_per_s"] = 4.0 * res["value"]
+    res["tflop_per_s"] = res["value"] * 63 / 1e12
+    res["pct_fp64_peak"] = 100.0 * res["tflop_per_s"] / 78.6
     if not args.no_cpu:
         log("cpu baseline ...")
         O = oracle()
