@@ -246,7 +246,9 @@ def bif1d(tMax, nPts, h, initialConditions, ranges, indicesOfMutVars, writableVa
             transientTime, _dp(vals), vals.size, preScaller, _ip(counts), _dp(first))
     # one sweep when the peak buffers are large enough (one peak per 32
     # samples); a second, exactly sized one only when they are not
-    B = max(1, int(round(tMax / h / max(1, preScaller))))
+    # (an invalid h reaches the library, which reports it)
+    q = tMax / h / max(1, preScaller) if h > 0 else 0.0
+    B = max(1, int(round(q))) if np.isfinite(q) else 1
     cap = max(1, nPts * min(B // 2 + 1, max(256, B // 32)))
     pk, iv = np.zeros(cap), np.zeros(cap)
     rc = lib.chaos_bif1d(*args, _dp(pk), _dp(iv), cap, C.byref(total))
@@ -427,7 +429,9 @@ def legacy_bif1d_ic(tMax, nPts, h, initialConditions, ranges, indicesOfMutVars, 
     total = C.c_longlong(0)
     args = (C.byref(o), tMax, nPts, h, ic.size, _dp(ic), _dp(rg), _ip(ix), writableVar, maxValue, transientTime,
             _dp(vals), vals.size, preScaller, C.c_void_p(counts.ctypes.data))
-    B = max(1, int(round(tMax / h / max(1, preScaller))))
+    # (an invalid h reaches the library, which reports it)
+    q = tMax / h / max(1, preScaller) if h > 0 else 0.0
+    B = max(1, int(round(q))) if np.isfinite(q) else 1
     cap = max(1, nPts * min(B // 2 + 1, max(256, B // 32)))
     pk = np.zeros(cap)
     rc = lib.chaos_legacy_bif1d_ic(*args, C.c_void_p(pk.ctypes.data), cap, C.byref(total))
