@@ -63,6 +63,45 @@ def same(a, b):
     return np.array_equal(np.asarray(a), np.asarray(b))
 
 
+def tiny_jobs(chaos):
+    J = chaos.make_job
+    return [
+        # records of one to five samples (the detector warm-up meets the
+        # first and last samples), h = 0.1
+        *[J(chaos.JOB_BIF1D, "rossler_rk4", sem, 40, ROS["ic"], ROS["values"], [2.0, 6.0], [2], h=0.1,
+            tMax=round(0.1 * b + 0.001, 4), transientTime=5.0) for b in (1, 2, 5) for sem in (0, 1)],
+        # LLE horizons shorter than one window
+        J(chaos.JOB_LLE2D, "rossler_rk4", 0, 3, ROS["ic"], ROS["values"], [0.05, 0.35, 2.0, 10.0], [0, 2],
+          tMax=0.3, NT=0.5, eps=1e-8, transientTime=1.0),
+        J(chaos.JOB_LLE2D, "rossler_rk4", 1, 3, ROS["ic"], ROS["values"], [0.05, 0.35, 2.0, 10.0], [0, 2],
+          tMax=0.51, NT=0.5, eps=1e-8, transientTime=1.0),
+        # 2x2 and 3x3 grids
+        J(chaos.JOB_BIF2D, "lorenz_rk4", 1, 2, LOR["ic"], LOR["values"], [20.0, 200.0, 0.5, 4.0], [1, 2], h=0.05,
+          tMax=5.05, transientTime=5.0, eps=1e-2),
+        J(chaos.JOB_BASINS2D, "lorenz_rk4", 0, 3, [0.0, 0.0, 25.0], [10.0, 24.5, 8 / 3], [-20.0, 20.0, -20.0, 20.0],
+          [0, 1], tMax=5.0, transientTime=5.0, eps=0.5),
+        J(chaos.JOB_BASINS2D, "lorenz_rk4", 1, 3, [0.0, 0.0, 25.0], [10.0, 24.5, 8 / 3], [-20.0, 20.0, -20.0, 20.0],
+          [0, 1], tMax=0.01, transientTime=5.0, eps=0.5, min_pts=2),
+    ]
+
+
+@pytest.mark.parametrize("steps", [3, 700])
+def test_tiny_jobs_equal_one_shot(chaos, steps):
+    """Jobs of a few points and a few samples, tiles of 3 ticks (every tile
+    boundary falls inside the warm-up, the first or the last sample)."""
+    jobs = tiny_jobs(chaos)
+    srv = chaos.Server(params=chaos.engine_params(steps_per_launch=steps))
+    ids = [srv.submit(j) for j in jobs]
+    print(f"engine tiny jobs steps={steps}", flush=True)
+    srv.drain()
+    for jid, j in zip(ids, jobs):
+        assert srv.info(jid).status == 2
+        got = srv.result(jid, j)
+        want = one_shot(chaos, j)
+        assert same(got, want), (j.kind, j.semantics, j.nPts, j.tMax)
+    srv.close()
+
+
 @pytest.mark.parametrize("steps,graph", [(700, 1), (20000, 1), (700, 0)])
 def test_mixed_batch_equals_one_shot(chaos, steps, graph):
     jobs = mixed_jobs(chaos)
