@@ -224,23 +224,25 @@ def run_bif2d(args):
     import chaoslibrary_amd as ch
     stats = ch.Stats()
     n = args.grid if args.grid != 2048 else 1024
+    sem = 1 if args.semantics is None else args.semantics
     dptr = C.c_void_p()
     ch._check(ch.lib.chaos_device_malloc(C.byref(dptr), n * n * 4, 0))
 
     def one():
         ch.bif2d(500.0, n, 0.01, [1.0, 1.0, 1.0], [20.0, 200.0, 0.5, 4.0], [1, 2], 0, 0.0, 500.0,
-                 [10.0, 28.0, 8 / 3], 1, 1e-2, system="lorenz_rk4", semantics=1, stats=stats,
+                 [10.0, 28.0, 8 / 3], 1, 1e-2, system="lorenz_rk4", semantics=sem, stats=stats,
                  out_device_ptr=dptr.value)
         return stats.kernel_ms, stats.analysis_ms, stats.rk4_steps
 
     wall, kms, ams, steps = timed(args, one)
     ch._check(ch.lib.chaos_device_free(dptr))
     res = base_result(args, steps / wall, wall, steps)
-    res["metric"] = f"FP64 RK4 steps/s, Lorenz peak-count map {n}^2 (whole pipeline)"
+    res["metric"] = f"FP64 RK4 steps/s, Lorenz peak-count map {n}^2 (whole pipeline)" + (
+        ", reference per-point dbscan" if sem == 0 else "")
     res["data"] = "synthetic: rho = linspace(20,200), beta = linspace(0.5,4), sigma = 10"
     res["config"] = {"workload": f"bif2d_lorenz_rk4_{n}x{n}", "grid": n, "transient_time": 500.0, "tMax": 500.0,
-                     "h": 0.01, "dbscan_eps": 1e-2, "peaks": stats.peaks}
-    summary = f"sweep_kernel_bif2d_lorenz_{n}" if n == 1024 else None
+                     "h": 0.01, "dbscan_eps": 1e-2, "peaks": stats.peaks, "semantics": sem}
+    summary = f"sweep_kernel_bif2d_lorenz_{n}" if n == 1024 and sem == 1 else None
     res["roofline"] = roofline("lorenz_rk4", steps / args.steps, kms / 1e3, "sweep_kernel<6,0,0>", kms, summary)
     res["analysis_ms"] = ams
     # peak output and 1-D DBSCAN stages: measured HBM bytes over their live time
@@ -253,11 +255,12 @@ def run_basins(args):
     import chaoslibrary_amd as ch
     stats = ch.Stats()
     n = args.grid
+    sem = 1 if args.semantics is None else args.semantics
     dptr = C.c_void_p()
     ch._check(ch.lib.chaos_device_malloc(C.byref(dptr), n * n * 4, 0))
 
     def one():
-        o = ch.make_options("lorenz_rk4", 1, stats=stats, device_buffers=True)
+        o = ch.make_options("lorenz_rk4", sem, stats=stats, device_buffers=True)
         ic, rg, ix = ch._f64([0.0, 0.0, 25.0]), ch._f64([-20.0, 20.0, -20.0, 20.0]), ch._i32([0, 1])
         vals = ch._f64([10.0, 24.5, 8 / 3])
         ch._check(ch.lib.chaos_basins2d(C.byref(o), 300.0, n, 0.01, 3, ch._dp(ic), ch._dp(rg), ch._ip(ix), 0, 0.0,
@@ -267,12 +270,14 @@ def run_basins(args):
     wall, kms, ams, steps = timed(args, one)
     ch._check(ch.lib.chaos_device_free(dptr))
     res = base_result(args, steps / wall, wall, steps)
-    res["metric"] = f"FP64 RK4 steps/s, Lorenz basins {n}^2 (features + z-score + 3-D DBSCAN)"
+    res["metric"] = f"FP64 RK4 steps/s, Lorenz basins {n}^2 " + (
+        "(features + z-score + 3-D DBSCAN)" if sem == 1 else "(reference features + single linkage)")
     res["data"] = "synthetic: IC grid x0, y0 in [-20,20], z0 = 25; rho = 24.5"
     res["config"] = {"workload": f"basins_lorenz_rk4_{n}x{n}", "grid": n, "transient_time": 300.0, "tMax": 300.0,
-                     "h": 0.01, "eps": 0.5, "min_pts": 20}
-    summary = f"sweep_kernel_basins_lorenz_{n}" if n == 2048 else None
-    res["roofline"] = roofline("lorenz_rk4", steps / args.steps, kms / 1e3, "sweep_kernel<6,0,2>", kms, summary)
+                     "h": 0.01, "eps": 0.5, "min_pts": 20, "semantics": sem}
+    summary = f"sweep_kernel_basins_lorenz_{n}" if n == 2048 and sem == 1 else None
+    res["roofline"] = roofline("lorenz_rk4", steps / args.steps, kms / 1e3,
+                               "sweep_kernel<6,0,2>" if sem == 1 else "sweep_kernel<6,0,1>", kms, summary)
     res["analysis_ms"] = ams
     # z-score + 3-D DBSCAN stages: measured HBM bytes over their live time
     ab = profile_figure(summary, "analysis_bytes_per_run") if summary else None
@@ -482,7 +487,7 @@ def run_regression(args):
             "warmup": 0, "ms_per_step": ms / max(1, len(per_job)), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f64", "data": "synthetic parameter / initial-condition grids",
             "config": {"workload": f"regression_{args.params}", "resolutions": args.resolutions or "100..600",
-                       "modelling_times": args.times or "1000..6000", "semantics": args.semantics},
+                       "modelling_times": args.times or "1000..6000", "semantics": args.semantics or 0},
             "suites": by_suite, "harness": summary, "harness_s": round(harness_s, 1), "results_dir": results,
             "cpu_rows": (f"oracle on whole middle grid rows (>= 4 points per thread, {cpu_threads()} OpenMP "
                          f"threads) scaled to the full grid; basins: per-point features only")
@@ -529,7 +534,8 @@ def main():
     ap.add_argument("--mode", default="both", choices=["both", "batched", "per_job"], help="stream modes to run")
     ap.add_argument("--bytes", type=int, default=1 << 30)
     ap.add_argument("--params", default="rk4", choices=["rk4", "reference"], help="regression parameter sets")
-    ap.add_argument("--semantics", type=int, default=0, help="regression: 0 reference, 1 config")
+    ap.add_argument("--semantics", type=int, default=None, choices=[0, 1],
+                    help="0 reference, 1 config (default: config for bif2d / basins, reference for regression)")
     ap.add_argument("--resolutions", default="", help="regression: comma list (default 100..600)")
     ap.add_argument("--times", default="", help="regression: comma list of modelling times (default 1000..6000)")
     ap.add_argument("--fixed-resolution", type=int, default=0, help="regression: resolution of the time sweep")
