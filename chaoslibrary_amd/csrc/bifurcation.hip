@@ -187,6 +187,7 @@ static void run_bif(CallCtx& ctx, const chaos_options& o, const BifInputs& in, B
     const long long n = A.n_points;
     R.n = n;
     R.hp = A.hp;
+    R.samples = A.B;
     R.flags.alloc(n);
     R.counts.alloc(n);
     R.first.alloc(n);
@@ -396,10 +397,12 @@ __global__ __launch_bounds__(64) void dbscan1d_big_kernel(const int* __restrict_
 //    reads slot P+NumNeibor-1, clears it and moves the loop index, exactly as
 //    the reference's quirky stack does.
 // T is the index type of the scratch arrays (16-bit in LDS, 32-bit in global
-// memory); keys holds C >= P 64-bit slots (C a power of two).
-template <typename T>
+// memory); keys holds C >= P slots (C a power of two) of type K: 64-bit
+// (dt << 32) | k, or 32-bit (dt << 16) | k when every gap dt and index k fits
+// 16 bits -- the same order in half the LDS, so more waves share a CU.
+template <typename T, typename K>
 struct RefScratch {
-    unsigned long long* keys;  // (dt << 32) | k, sorted
+    K* keys;                   // (dt << shift) | k, sorted
     T* h;                      // [2P] the reference's helpfulArray
     T* bucket_of;              // [P] bucket of index k
     T* bstart;                 // [nb + 1] first sorted position of bucket b
@@ -438,14 +441,16 @@ __device__ __forceinline__ int ref_find_right(T* rp, int b, int nb)
     return nb;
 }
 
-template <typename T>
-__device__ int dbscan_ref_wave(const unsigned* __restrict__ t, int P, double hp, double eps, RefScratch<T> S,
+template <typename T, typename K>
+__device__ int dbscan_ref_wave(const unsigned* __restrict__ t, int P, double hp, double eps, RefScratch<T, K> S,
                                int C)
 {
+    constexpr int SH = (int)sizeof(K) * 4;
+    constexpr K LO = (K(1) << SH) - 1;
     const int lane = threadIdx.x;
-    unsigned long long* keys = S.keys;
+    K* keys = S.keys;
     for (int s = lane; s < C; s += 64)
-        keys[s] = s < P ? ((unsigned long long)(t[s + 1] - t[s]) << 32) | (unsigned)s : ~0ull;
+        keys[s] = s < P ? ((K)(t[s + 1] - t[s]) << SH) | (K)s : ~K(0);
     for (int k = lane; k < 2 * P; k += 64) S.h[k] = 0;
     __syncthreads();
     bitonic_sort(keys, C);
@@ -454,12 +459,12 @@ __device__ int dbscan_ref_wave(const unsigned* __restrict__ t, int P, double hp,
     for (int base = 0; base < P; base += 64) {
         const int s = base + lane;
         bool head = false;
-        if (s < P) head = s == 0 || (keys[s] >> 32) != (keys[s - 1] >> 32);
+        if (s < P) head = s == 0 || (keys[s] >> SH) != (keys[s - 1] >> SH);
         const unsigned long long m = __ballot(head);
         const int bid = nb + (int)__popcll(m & ((2ull << lane) - 1ull)) - 1;
         if (s < P) {
             if (head) S.bstart[bid] = (T)s;
-            S.bucket_of[(int)(keys[s] & 0xffffffffull)] = (T)bid;
+            S.bucket_of[(int)(keys[s] & LO)] = (T)bid;
         }
         nb += (int)__popcll(m);
     }
@@ -469,7 +474,7 @@ __device__ int dbscan_ref_wave(const unsigned* __restrict__ t, int P, double hp,
     __syncthreads();
     // live: the bucket holds a candidate (k < P-1)
     for (int s = lane; s < P; s += 64)
-        if ((int)(keys[s] & 0xffffffffull) < P - 1) S.lp[(int)S.bucket_of[(int)(keys[s] & 0xffffffffull)]] = 1;
+        if ((int)(keys[s] & LO) < P - 1) S.lp[(int)S.bucket_of[(int)(keys[s] & LO)]] = 1;
     __syncthreads();
     for (int b = lane; b < nb; b += 64) {
         const bool live = S.lp[b] != 0;
@@ -487,7 +492,7 @@ __device__ int dbscan_ref_wave(const unsigned* __restrict__ t, int P, double hp,
         for (int s = s0 + lane; s - lane < s1; s += 64) {
             bool hit = false;
             if (s < s1) {
-                const int k = (int)(keys[s] & 0xffffffffull);
+                const int k = (int)(keys[s] & LO);
                 if (k < P - 1 && k != i && S.h[k] == 0) {
                     S.h[k] = (T)cluster;
                     S.h[P + k] = (T)k;
@@ -509,7 +514,7 @@ __device__ int dbscan_ref_wave(const unsigned* __restrict__ t, int P, double hp,
         if (lane == 0) S.expd[b0] = 1;
         const double yi = (double)((long long)t[i + 1] - (long long)t[i]) * hp;
         auto qualifies = [&](int b) {
-            const double yb = (double)(long long)(keys[(int)S.bstart[b]] >> 32) * hp;
+            const double yb = (double)(long long)(keys[(int)S.bstart[b]] >> SH) * hp;
             const double d = (yb == yi) ? 0.0 : (double)fabsf((float)(yb - yi));
             return __builtin_amdgcn_readfirstlane((int)(d < eps)) != 0;
         };
@@ -586,15 +591,18 @@ __global__ void dbscan_ref_trivial_kernel(const int* __restrict__ flags, const i
     else if (P <= 1) out[p] = P;
 }
 
-// LDS bytes of dbscan_ref_lds_kernel for capacity C: 64-bit keys, 16-bit
+// LDS bytes of dbscan_ref_lds_kernel for capacity C: keys of key_bytes, 16-bit
 // helpful[2C], bucket_of, bstart (C + 1), lp, rp and the expanded flags.
-constexpr size_t ref_lds_bytes(int C) { return (size_t)C * (8 + 2 * 2 + 2 + 2 + 2 + 2 + 1) + 2 + 8; }
-
-template <typename T>
-__device__ __forceinline__ RefScratch<T> ref_scratch(void* base, int C)
+constexpr size_t ref_lds_bytes(int C, int key_bytes)
 {
-    RefScratch<T> S;
-    S.keys = static_cast<unsigned long long*>(base);
+    return (size_t)C * (key_bytes + 2 * 2 + 2 + 2 + 2 + 2 + 1) + 2 + 8;
+}
+
+template <typename T, typename K>
+__device__ __forceinline__ RefScratch<T, K> ref_scratch(void* base, int C)
+{
+    RefScratch<T, K> S;
+    S.keys = static_cast<K*>(base);
     S.h = reinterpret_cast<T*>(S.keys + C);
     S.bucket_of = S.h + 2 * C;
     S.bstart = S.bucket_of + C;
@@ -605,6 +613,7 @@ __device__ __forceinline__ RefScratch<T> ref_scratch(void* base, int C)
 }
 
 // The listed points (2 <= P <= C kept intervals each), scratch in LDS.
+template <typename K>
 __global__ __launch_bounds__(64) void dbscan_ref_lds_kernel(const int* __restrict__ list, int nlist,
                                                             const int* __restrict__ counts,
                                                             const long long* __restrict__ off,
@@ -616,7 +625,8 @@ __global__ __launch_bounds__(64) void dbscan_ref_lds_kernel(const int* __restric
     if (b >= nlist) return;
     const int p = list[b];
     const int P = counts[p] - 1;
-    const int c = dbscan_ref_wave<short>(times + off[p], P, hp, eps, ref_scratch<short>(lds_keys, C), C);
+    const int c =
+        dbscan_ref_wave<short, K>(times + off[p], P, hp, eps, ref_scratch<short, K>(lds_keys, C), C);
     if (threadIdx.x == 0) out[p] = c;
 }
 
@@ -638,7 +648,8 @@ __global__ __launch_bounds__(64) void dbscan_ref_big_kernel(const int* __restric
     const int P = counts[p] - 1;
     int C = 1;
     while (C < P) C <<= 1;
-    const int c = dbscan_ref_wave<int>(times + off[p], P, hp, eps, ref_scratch<int>(scratch + soff[b], C), C);
+    const int c = dbscan_ref_wave<int, unsigned long long>(times + off[p], P, hp, eps,
+                                                           ref_scratch<int, unsigned long long>(scratch + soff[b], C), C);
     if (threadIdx.x == 0) out[p] = c;
 }
 
@@ -740,15 +751,24 @@ void bif2d_analysis(CallCtx& ctx, int semantics, BifDevice& R, double eps, int* 
         hipLaunchKernelGGL(dbscan_ref_trivial_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx.stream(),
                            R.flags.get(), R.counts.get(), d_out, n);
         CHAOS_HIP(hipGetLastError());
-        // LDS classes C = 256, 1024, 2048 (ref_lds_bytes: 5.4, 21.5, 43 KiB)
+        // LDS classes C = 256, 1024, 2048 (ref_lds_bytes with 64-bit keys: 5.4,
+        // 21.5, 43 KiB; with 32-bit keys 4.4, 17.4, 34.8 KiB).  Sample gaps are
+        // below the recorded sample count and indices below C, so 16 bits
+        // each suffice when the record holds fewer than 65536 samples.
         const ClassCaps K{{256, 1024, 2048}, 3, 2, 1};
+        const bool keys32 = R.samples > 0 && R.samples < 65536;
         class_lists(ctx, R, K, L);
         for (int c = 0; c < K.ncls; ++c) {
             if (L.count[c] == 0) continue;
             const int C = K.cap[c];
-            hipLaunchKernelGGL(dbscan_ref_lds_kernel, dim3((unsigned)L.count[c]), dim3(64), ref_lds_bytes(C),
-                               ctx.stream(), L.lists.get() + (long long)c * n, L.count[c], R.counts.get(),
-                               R.off.get(), R.times.get(), R.hp, eps, C, d_out);
+            if (keys32)
+                hipLaunchKernelGGL(dbscan_ref_lds_kernel<unsigned>, dim3((unsigned)L.count[c]), dim3(64),
+                                   ref_lds_bytes(C, 4), ctx.stream(), L.lists.get() + (long long)c * n, L.count[c],
+                                   R.counts.get(), R.off.get(), R.times.get(), R.hp, eps, C, d_out);
+            else
+                hipLaunchKernelGGL(dbscan_ref_lds_kernel<unsigned long long>, dim3((unsigned)L.count[c]), dim3(64),
+                                   ref_lds_bytes(C, 8), ctx.stream(), L.lists.get() + (long long)c * n, L.count[c],
+                                   R.counts.get(), R.off.get(), R.times.get(), R.hp, eps, C, d_out);
             CHAOS_HIP(hipGetLastError());
             ctx.add_launch();
         }

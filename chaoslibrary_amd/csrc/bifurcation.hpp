@@ -23,6 +23,7 @@ struct BifDevice {
     long long n = 0;         // points
     long long total = 0;     // stored peaks of flag-1 points (first peak included)
     double hp = 0;           // interval unit
+    int samples = 0;         // recorded samples per point (bounds every peak-time gap); 0: unknown
     bool need_times = true;  // scatter the peak times too (intervals)
     bool need_host = true;   // host copies of flags / counts / first peaks (1-D results)
     DevBuf<int> flags, counts;

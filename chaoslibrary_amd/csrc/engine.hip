@@ -371,6 +371,7 @@ void Engine::allocate(JobRec& r)
         if (r.prog == PROG_BIF) {
             r.bif.n = r.n;
             r.bif.hp = r.sw.hp;
+            r.bif.samples = r.sw.B;
             r.bif.need_times = !(r.spec.kind == CHAOS_JOB_BIF2D && r.spec.semantics == CHAOS_SEM_CONFIG);
             r.bif.need_host = r.spec.kind == CHAOS_JOB_BIF1D;
             r.bif.flags.alloc(r.n);
