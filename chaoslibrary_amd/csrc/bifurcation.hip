@@ -281,8 +281,8 @@ __global__ __launch_bounds__(64) void dbscan1d_list_kernel(const int* __restrict
 // e = lane * R + r.  Pairs closer than R sit in one lane (compare-exchange
 // of registers); farther pairs are one lane-xor apart and swap through a
 // shuffle, each lane keeping the minimum or the maximum.
-template <int R>
-__device__ __forceinline__ void wave_bitonic(double (&v)[R])
+template <int R, typename V>
+__device__ __forceinline__ void wave_bitonic(V (&v)[R])
 {
     const int lane = threadIdx.x & 63;
     constexpr int M = 64 * R;
@@ -294,11 +294,11 @@ __device__ __forceinline__ void wave_bitonic(double (&v)[R])
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const int e = lane * R + r;
-                    const double o = __shfl_xor(v[r], stride / R);
+                    const V o = __shfl_xor(v[r], stride / R);
                     const bool take_min = ((e & stride) == 0) == ((e & size) == 0);
                     const bool less = v[r] < o;
-                    const double mn = less ? v[r] : o;
-                    const double mx = less ? o : v[r];
+                    const V mn = less ? v[r] : o;
+                    const V mx = less ? o : v[r];
                     v[r] = take_min ? mn : mx;
                 }
             } else {
@@ -306,9 +306,9 @@ __device__ __forceinline__ void wave_bitonic(double (&v)[R])
                 for (int r = 0; r < R; ++r) {
                     if ((r & stride) == 0) {
                         const bool up = ((lane * R + r) & size) == 0;
-                        const double a = v[r], b = v[r + stride];
+                        const V a = v[r], b = v[r + stride];
                         const bool gt = a > b;
-                        const double lo = gt ? b : a, hi = gt ? a : b;
+                        const V lo = gt ? b : a, hi = gt ? a : b;
                         v[r] = up ? lo : hi;
                         v[r + stride] = up ? hi : lo;
                     }
@@ -441,7 +441,9 @@ __device__ __forceinline__ int ref_find_right(T* rp, int b, int nb)
     return nb;
 }
 
-template <typename T, typename K>
+// RS > 0: C = 64 * RS and the keys are sorted in registers (wave_bitonic),
+// then stored in order; RS = 0: sorted in place in the scratch array.
+template <typename T, typename K, int RS>
 __device__ int dbscan_ref_wave(const unsigned* __restrict__ t, int P, double hp, double eps, RefScratch<T, K> S,
                                int C)
 {
@@ -449,11 +451,23 @@ __device__ int dbscan_ref_wave(const unsigned* __restrict__ t, int P, double hp,
     constexpr K LO = (K(1) << SH) - 1;
     const int lane = threadIdx.x;
     K* keys = S.keys;
-    for (int s = lane; s < C; s += 64)
-        keys[s] = s < P ? ((K)(t[s + 1] - t[s]) << SH) | (K)s : ~K(0);
+    if constexpr (RS > 0) {
+        K kv[RS];
+#pragma unroll
+        for (int r = 0; r < RS; ++r) {
+            const int e = lane * RS + r;
+            kv[r] = e < P ? ((K)(t[e + 1] - t[e]) << SH) | (K)e : ~K(0);
+        }
+        wave_bitonic<RS>(kv);
+#pragma unroll
+        for (int r = 0; r < RS; ++r) keys[lane * RS + r] = kv[r];
+    } else {
+        for (int s = lane; s < C; s += 64)
+            keys[s] = s < P ? ((K)(t[s + 1] - t[s]) << SH) | (K)s : ~K(0);
+    }
     for (int k = lane; k < 2 * P; k += 64) S.h[k] = 0;
     __syncthreads();
-    bitonic_sort(keys, C);
+    if constexpr (RS == 0) bitonic_sort(keys, C);
     // buckets of equal dt over the sorted positions [0, P)
     int nb = 0;
     for (int base = 0; base < P; base += 64) {
@@ -612,8 +626,9 @@ __device__ __forceinline__ RefScratch<T, K> ref_scratch(void* base, int C)
     return S;
 }
 
-// The listed points (2 <= P <= C kept intervals each), scratch in LDS.
-template <typename K>
+// The listed points (2 <= P <= C kept intervals each), scratch in LDS; RS > 0
+// sorts the keys in registers (C = 64 * RS).
+template <typename K, int RS>
 __global__ __launch_bounds__(64) void dbscan_ref_lds_kernel(const int* __restrict__ list, int nlist,
                                                             const int* __restrict__ counts,
                                                             const long long* __restrict__ off,
@@ -626,7 +641,7 @@ __global__ __launch_bounds__(64) void dbscan_ref_lds_kernel(const int* __restric
     const int p = list[b];
     const int P = counts[p] - 1;
     const int c =
-        dbscan_ref_wave<short, K>(times + off[p], P, hp, eps, ref_scratch<short, K>(lds_keys, C), C);
+        dbscan_ref_wave<short, K, RS>(times + off[p], P, hp, eps, ref_scratch<short, K>(lds_keys, C), C);
     if (threadIdx.x == 0) out[p] = c;
 }
 
@@ -648,7 +663,7 @@ __global__ __launch_bounds__(64) void dbscan_ref_big_kernel(const int* __restric
     const int P = counts[p] - 1;
     int C = 1;
     while (C < P) C <<= 1;
-    const int c = dbscan_ref_wave<int, unsigned long long>(times + off[p], P, hp, eps,
+    const int c = dbscan_ref_wave<int, unsigned long long, 0>(times + off[p], P, hp, eps,
                                                            ref_scratch<int, unsigned long long>(scratch + soff[b], C), C);
     if (threadIdx.x == 0) out[p] = c;
 }
@@ -761,14 +776,17 @@ void bif2d_analysis(CallCtx& ctx, int semantics, BifDevice& R, double eps, int* 
         for (int c = 0; c < K.ncls; ++c) {
             if (L.count[c] == 0) continue;
             const int C = K.cap[c];
-            if (keys32)
-                hipLaunchKernelGGL(dbscan_ref_lds_kernel<unsigned>, dim3((unsigned)L.count[c]), dim3(64),
-                                   ref_lds_bytes(C, 4), ctx.stream(), L.lists.get() + (long long)c * n, L.count[c],
-                                   R.counts.get(), R.off.get(), R.times.get(), R.hp, eps, C, d_out);
-            else
-                hipLaunchKernelGGL(dbscan_ref_lds_kernel<unsigned long long>, dim3((unsigned)L.count[c]), dim3(64),
-                                   ref_lds_bytes(C, 8), ctx.stream(), L.lists.get() + (long long)c * n, L.count[c],
-                                   R.counts.get(), R.off.get(), R.times.get(), R.hp, eps, C, d_out);
+            const int* lst = L.lists.get() + (long long)c * n;
+            auto go = [&](auto kern, int key_bytes) {
+                hipLaunchKernelGGL(kern, dim3((unsigned)L.count[c]), dim3(64), ref_lds_bytes(C, key_bytes),
+                                   ctx.stream(), lst, L.count[c], R.counts.get(), R.off.get(), R.times.get(), R.hp,
+                                   eps, C, d_out);
+            };
+            // 32-bit keys are sorted in registers (64 lanes x C / 64)
+            if (!keys32) go(dbscan_ref_lds_kernel<unsigned long long, 0>, 8);
+            else if (C == 256) go(dbscan_ref_lds_kernel<unsigned, 4>, 4);
+            else if (C == 1024) go(dbscan_ref_lds_kernel<unsigned, 16>, 4);
+            else go(dbscan_ref_lds_kernel<unsigned, 32>, 4);
             CHAOS_HIP(hipGetLastError());
             ctx.add_launch();
         }
