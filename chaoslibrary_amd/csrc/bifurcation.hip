@@ -43,7 +43,7 @@ __global__ void scatter_kernel(const PeakPool pool, unsigned used, const int* __
     const long long p = (long long)pool.owner()[c] * 64 + (e.kl & 63u);
     if (p >= n || flags[p] != 1) return;
     const long long pos = off[p] + (e.kl >> 6);
-    vals[pos] = e.v;
+    if (vals) vals[pos] = e.v;
     if (times) times[pos] = e.t;
 }
 
@@ -165,7 +165,8 @@ bool bif_post(CallCtx& ctx, BifDevice& R, PeakPoolBuf& pool)
                        R.counts.get(), kept.get(), n);
     CHAOS_HIP(hipGetLastError());
     R.total = exclusive_scan(ctx.stream(), kept.get(), R.off.get(), n);
-    R.vals.alloc((size_t)std::max(1LL, R.total));
+    if (R.need_vals) R.vals.alloc((size_t)std::max(1LL, R.total));
+    else R.vals.release();
     if (R.need_times) R.times.alloc((size_t)std::max(1LL, R.total));
     else R.times.release();
     if (used > (unsigned)pool.nchunks) used = (unsigned)pool.nchunks;
@@ -983,6 +984,7 @@ static void bif2d_impl(const chaos_options* opt, double tMax, int nPts, double h
     CallCtx ctx(o);
     BifDevice R;
     R.need_times = o.semantics == CHAOS_SEM_REFERENCE;  // the CONFIG count reads peak values only
+    R.need_vals = o.semantics != CHAOS_SEM_REFERENCE;   // the reference dbscan reads peak times only
     R.need_host = false;                                // the 2-D analysis lists its points on the device
     run_bif(ctx, o, in, R);
     DevBuf<int> tmp;

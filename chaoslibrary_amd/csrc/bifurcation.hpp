@@ -25,6 +25,7 @@ struct BifDevice {
     double hp = 0;           // interval unit
     int samples = 0;         // recorded samples per point (bounds every peak-time gap); 0: unknown
     bool need_times = true;  // scatter the peak times too (intervals)
+    bool need_vals = true;   // scatter the peak values (the REFERENCE 2-D dbscan reads times only)
     bool need_host = true;   // host copies of flags / counts / first peaks (1-D results)
     DevBuf<int> flags, counts;
     DevBuf<double> first;

@@ -373,6 +373,7 @@ void Engine::allocate(JobRec& r)
             r.bif.hp = r.sw.hp;
             r.bif.samples = r.sw.B;
             r.bif.need_times = !(r.spec.kind == CHAOS_JOB_BIF2D && r.spec.semantics == CHAOS_SEM_CONFIG);
+            r.bif.need_vals = !(r.spec.kind == CHAOS_JOB_BIF2D && r.spec.semantics == CHAOS_SEM_REFERENCE);
             r.bif.need_host = r.spec.kind == CHAOS_JOB_BIF1D;
             r.bif.flags.alloc(r.n);
             r.bif.counts.alloc(r.n);
