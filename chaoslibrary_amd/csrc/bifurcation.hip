@@ -442,8 +442,9 @@ __device__ __forceinline__ int ref_find_right(T* rp, int b, int nb)
     return nb;
 }
 
-// RS > 0: C = 64 * RS and the keys are sorted in registers (wave_bitonic),
-// then stored in order; RS = 0: sorted in place in the scratch array.
+// RS > 0: the keys are sorted in registers (wave_bitonic over 64 * RS >= C
+// slots, a power of two), then the first C stored in order; RS = 0: sorted
+// in place in the scratch array (C a power of two).
 template <typename T, typename K, int RS>
 __device__ int dbscan_ref_wave(const unsigned* __restrict__ t, int P, double hp, double eps, RefScratch<T, K> S,
                                int C)
@@ -461,7 +462,8 @@ __device__ int dbscan_ref_wave(const unsigned* __restrict__ t, int P, double hp,
         }
         wave_bitonic<RS>(kv);
 #pragma unroll
-        for (int r = 0; r < RS; ++r) keys[lane * RS + r] = kv[r];
+        for (int r = 0; r < RS; ++r)
+            if (lane * RS + r < C) keys[lane * RS + r] = kv[r];
     } else {
         for (int s = lane; s < C; s += 64)
             keys[s] = s < P ? ((K)(t[s + 1] - t[s]) << SH) | (K)s : ~K(0);
@@ -628,7 +630,7 @@ __device__ __forceinline__ RefScratch<T, K> ref_scratch(void* base, int C)
 }
 
 // The listed points (2 <= P <= C kept intervals each), scratch in LDS; RS > 0
-// sorts the keys in registers (C = 64 * RS).
+// sorts the keys in registers (64 * RS >= C).
 template <typename K, int RS>
 __global__ __launch_bounds__(64) void dbscan_ref_lds_kernel(const int* __restrict__ list, int nlist,
                                                             const int* __restrict__ counts,
@@ -767,12 +769,15 @@ void bif2d_analysis(CallCtx& ctx, int semantics, BifDevice& R, double eps, int* 
         hipLaunchKernelGGL(dbscan_ref_trivial_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx.stream(),
                            R.flags.get(), R.counts.get(), d_out, n);
         CHAOS_HIP(hipGetLastError());
-        // LDS classes C = 256, 1024, 2048 (ref_lds_bytes with 64-bit keys: 5.4,
-        // 21.5, 43 KiB; with 32-bit keys 4.4, 17.4, 34.8 KiB).  Sample gaps are
-        // below the recorded sample count and indices below C, so 16 bits
-        // each suffice when the record holds fewer than 65536 samples.
-        const ClassCaps K{{256, 1024, 2048}, 3, 2, 1};
+        // Sample gaps are below the recorded sample count and indices below
+        // C, so 16 bits each suffice when the record holds fewer than 65536
+        // samples.  32-bit keys sort in registers, so the LDS classes need not
+        // be powers of two: C = 256 ... 2048 in six steps (17 bytes per slot,
+        // 4.4 to 34.8 KiB), each point in the smallest that holds it.  64-bit
+        // keys sort in LDS: C = 256, 1024, 2048 (21 bytes per slot).
         const bool keys32 = R.samples > 0 && R.samples < 65536;
+        const ClassCaps K = keys32 ? ClassCaps{{256, 512, 768, 1024, 1536, 2048}, 6, 2, 1}
+                                   : ClassCaps{{256, 1024, 2048}, 3, 2, 1};
         class_lists(ctx, R, K, L);
         for (int c = 0; c < K.ncls; ++c) {
             if (L.count[c] == 0) continue;
@@ -783,10 +788,11 @@ void bif2d_analysis(CallCtx& ctx, int semantics, BifDevice& R, double eps, int* 
                                    ctx.stream(), lst, L.count[c], R.counts.get(), R.off.get(), R.times.get(), R.hp,
                                    eps, C, d_out);
             };
-            // 32-bit keys are sorted in registers (64 lanes x C / 64)
+            // 32-bit keys are sorted in registers (64 lanes x the power of two >= C)
             if (!keys32) go(dbscan_ref_lds_kernel<unsigned long long, 0>, 8);
-            else if (C == 256) go(dbscan_ref_lds_kernel<unsigned, 4>, 4);
-            else if (C == 1024) go(dbscan_ref_lds_kernel<unsigned, 16>, 4);
+            else if (C <= 256) go(dbscan_ref_lds_kernel<unsigned, 4>, 4);
+            else if (C <= 512) go(dbscan_ref_lds_kernel<unsigned, 8>, 4);
+            else if (C <= 1024) go(dbscan_ref_lds_kernel<unsigned, 16>, 4);
             else go(dbscan_ref_lds_kernel<unsigned, 32>, 4);
             CHAOS_HIP(hipGetLastError());
             ctx.add_launch();

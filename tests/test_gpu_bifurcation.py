@@ -135,6 +135,23 @@ def test_bif2d_segment_classes_match_oracle(chaos, oracle, ct, sem):
     assert per_point > ct / 2
 
 
+def test_bif2d_ref_keys32_size_classes(chaos, oracle):
+    """REFERENCE map with a record of 65,000 samples (below 65,536: 32-bit
+    sort keys) over rho x beta in [20,200] x [0.5,4]: points with 257 to
+    ~1,700 kept intervals, i.e. the 512, 768, 1024, 1536 and 2048 classes
+    (the 256 class is covered at CT 100 above)."""
+    n = 8
+    ranges = [20.0, 200.0, 0.5, 4.0]
+    print(f"bif2d lorenz ref {n}x{n} CT=650", flush=True)
+    st = chaos.Stats()
+    got = chaos.bif2d(650.0, n, 0.01, [1.0, 1.0, 1.0], ranges, [1, 2], 0, 0.0, 20.0, [10.0, 28.0, 8 / 3], 1, 0.01,
+                      system="lorenz_rk4", semantics=0, stats=st)
+    want = oracle.bif2d(oracle.SYS["lorenz_rk4"], 0, 650.0, n, 0.01, [1.0, 1.0, 1.0], ranges, [1, 2], 0, 0.0, 20.0,
+                        [10.0, 28.0, 8 / 3], 1, 0.01)
+    assert np.array_equal(got, want)
+    assert st.peaks / (n * n) > 700 and len(np.unique(got)) > 5
+
+
 def test_bif2d_ref_harness_scale_sampled(chaos, oracle):
     """REFERENCE map at the regression harness's physics (Lorenz, CT 400,
     TT 2000, eps 0.001) on a 96^2 grid; 4 sampled rows recomputed by the
