@@ -21,6 +21,9 @@ crosses PCIe in the timed region.
           1000..6000) of bifurcation2D / LLE2D / basinsOfAttraction_2 through
           chaoslibrary_amd/chaos_regression, rows Test,Parameter,Value,Library,
           ExecutionTime_ms with Library cpu | hip-per-job | hip-batched
+--semantics 0 runs bif2d / basins with the reference's own analyses (the
+per-point dbscan on peak intervals; mean-peak / mean-interval features and
+single linkage) instead of the configured ones.
 
 The roofline block prices the dominant kernel's algorithmic FP64 work
 (63 FLOP per Rossler RK4 step, 71 per Lorenz step, FMA = 2) against MI355X's
