@@ -677,6 +677,9 @@ int orc_lle2d_omp(int sys, int sem, double tMax, double NT, int nPts, double h,
  * from s0 = sum p, s1 = sum fma(p,p,.): m = s0/n, var = s1/n - m*m,
  * std = sqrt(var) if var > 0 and n >= 2, else 0; divergent points get
  * (0,0,0) and flag 0.
+ * Pinned by tests/test_oracle_basins.py: the sentinels, the Lorenz
+ * equilibrium closed forms, the Lorenz mirror map, and an independent
+ * fsum / two-pass-std / telescoped-interval evaluation of chaotic records.
  */
 int orc_basins_features(int sys, int sem, double tMax, int nPts, double h,
                         const double* ic, int nx, const double* ranges,
