@@ -9,6 +9,7 @@
 // stream never leaves VGPRs: the peak detector runs online and only the
 // peaks (Bifurcation) or a few running sums (Basins) are produced.  The
 // per-lane program (SweepProg) is shared with the batched engine.
+#include <cfloat>
 #include <cmath>
 
 #include "dispatch.hpp"
@@ -36,11 +37,14 @@ void sweep_kernel(const SweepArgs A)
 double divergence_fast_bound(int check, double maxValue)
 {
     // |v| < 2^53 keeps v == v + 1.0 false; |v| < M (1 - 2^-22) keeps the
-    // float-rounded |v| <= M.  Non-finite v always fails the bound.
+    // float-rounded |v| <= M.  Non-finite v always fails the bound.  The
+    // float-cast tests also need |v| < FLT_MAX: a larger double rounds to
+    // inf in (float)v, which exceeds any finite maxValue.
     double L = INFINITY;
     if (check == CHK_BIF_REF) L = 9007199254740992.0;
     if (maxValue != 0) {
-        const double m = check == CHK_CONFIG ? maxValue : maxValue * (1.0 - 0x1p-22);
+        double m = check == CHK_CONFIG ? maxValue : maxValue * (1.0 - 0x1p-22);
+        if (check != CHK_CONFIG && m > (double)FLT_MAX) m = (double)FLT_MAX;
         L = m > 0 ? (m < L ? m : L) : 0.0;
     }
     return L;

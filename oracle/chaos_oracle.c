@@ -239,6 +239,8 @@ static int diverged(double v, double maxValue, int check)
     return 0;
 }
 
+int orc_diverged(double v, double maxValue, int check) { return diverged(v, maxValue, check); }
+
 static int fixed_point(double t, int check)
 {
     if (check == ORC_CHK_BIF_REF)           /* bifurcationCUDA.cu:462 */

@@ -52,6 +52,10 @@ void orc_integrate(int sys, int n, const double* params, double* states,
 double orc_grid_value(int k, int n, double start, double end);
 double orc_grid_value_pow(int idx, int n, double start, double end, int dim);
 
+/* The divergence test of a record loop (see orc_loop): 1 if v counts as
+ * diverged under the check profile and maxValue. */
+int orc_diverged(double v, double maxValue, int check);
+
 int orc_loop(int sys, double* X, const double* a, double h, int amount,
              int preScaller, int writableVar, double maxValue, int check,
              double* data);

@@ -33,6 +33,7 @@ def _load():
         "orc_grid_value_pow": (C.c_double, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int]),
         "orc_loop": (C.c_int, [C.c_int, _D, _D, C.c_double, C.c_int, C.c_int, C.c_int,
                                C.c_double, C.c_int, _D]),
+        "orc_diverged": (C.c_int, [C.c_double, C.c_double, C.c_int]),
         "orc_peak_finder": (C.c_int, [_D, C.c_int, _D, _D, C.c_double]),
         "orc_bif1d": (C.c_longlong, [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, _D, C.c_int,
                                      _D, _I, C.c_int, C.c_double, C.c_double, _D, C.c_int, C.c_int,
