@@ -539,7 +539,7 @@ class Job(C.Structure):
 class EngineParams(C.Structure):
     _fields_ = [("block", C.c_int), ("waves_per_simd", C.c_int), ("steps_per_launch", C.c_int),
                 ("max_jobs", C.c_int), ("max_tiles", C.c_int), ("use_graph", C.c_int),
-                ("hbm_fraction", C.c_double)]
+                ("hbm_fraction", C.c_double), ("points_per_tile", C.c_int), ("reserved", C.c_int)]
 
 
 class JobInfo(C.Structure):
@@ -594,6 +594,7 @@ _SIGS3 = {
     "chaos_server_write_result": (C.c_int, [C.c_void_p, C.c_longlong, C.c_char_p, C.c_int]),
     "chaos_server_release": (C.c_int, [C.c_void_p, C.c_longlong]),
     "chaos_stream_config_default": (C.c_int, [C.POINTER(StreamConfig)]),
+    "chaos_stream_jobs": (C.c_int, [C.POINTER(StreamConfig), C.POINTER(Job), _D, C.c_int, _I]),
     "chaos_stream_run": (C.c_int, [C.c_int, C.POINTER(StreamConfig), C.POINTER(EngineParams),
                                    C.POINTER(StreamReport), _D, C.c_int]),
     "chaos_autotune": (C.c_int, [C.c_int, C.c_char_p, C.POINTER(EngineParams), C.POINTER(C.c_double)]),
@@ -718,6 +719,17 @@ def stream_config(**kw):
     for k, v in kw.items():
         setattr(c, "lambda_" if k == "lam" else k, v)
     return c
+
+
+def stream_jobs(cfg):
+    """The stream's jobs in arrival order: (list of Job, arrival_ms array)."""
+    cnt = C.c_int(0)
+    _check(lib.chaos_stream_jobs(C.byref(cfg), None, None, 0, C.byref(cnt)))
+    n = cnt.value
+    jobs = (Job * max(1, n))()
+    arr = np.zeros(max(1, n))
+    _check(lib.chaos_stream_jobs(C.byref(cfg), jobs, _dp(arr), n, C.byref(cnt)))
+    return [jobs[k] for k in range(n)], arr[:n].copy()
 
 
 def stream_run(cfg, params=None, device=0):

@@ -1,14 +1,18 @@
-// engine.hip -- batched execution of many sweeps in one persistent kernel.
+// engine.hip -- batched execution of many sweeps in persistent kernels.
 //
-// Work unit: a tile = (job slot, 64-point group, tick range).  Each launch
-// carries one tile per 64-point group of every active job; waves of the
-// persistent kernel pull tiles from a device queue (one atomic per tile),
-// resume the group's lane state from HBM, advance it, and park it again (or
-// write the job's outputs when its last tick is done).  The per-lane
-// programs are the ones the one-shot kernels run (lanes.hpp), so a job's
-// results do not depend on how it was batched.  The host side admits jobs
-// FIFO while job slots and the HBM budget allow, replays the launch sequence
-// from a captured hipGraph, and finishes completed jobs with the same
+// Work unit: a tile = (job slot, run of 64-point groups, tick range).  Each
+// round carries one tile per run of groups of every active job.  Jobs fall
+// into families (program x system x recorded variable); every family has its
+// own persistent kernel, compiled alone so that it keeps the one-shot
+// kernels' register budget, and its own queue in the round's tile list.
+// Waves pull tiles from their family's queue (one atomic per tile), resume
+// the groups' lane state from HBM, advance it, and park it again (or write
+// the job's outputs when its last tick is done).  The per-lane programs are
+// the ones the one-shot kernels run (lanes.hpp), so a job's results do not
+// depend on how it was batched.  The host side admits jobs FIFO while job
+// slots and the HBM budget allow, replays each round from a hipGraph
+// captured per set of families, checks the round's device error word before
+// it advances any job, and finishes completed jobs with the same
 // post-processing stages as the one-shot entry points.
 #include <algorithm>
 #include <chrono>
@@ -16,6 +20,8 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <string>
+#include <utility>
 #include <vector>
 
 #include "basins.hpp"
@@ -32,11 +38,17 @@ namespace chaos {
 // device side
 // ---------------------------------------------------------------------------
 
-template <int SYS>
-__device__ __forceinline__ void lle_tile(const JobDev& J, const Tile& T, int lane)
+__device__ __forceinline__ void report(int* err, int bits)
 {
-    const LleArgs p = J.lle;
-    const long long q = (long long)T.group * 64 + lane;
+    if (__lane_id() == 0) atomicOr(err, bits);
+}
+
+// One 64-point group of an LLE job over ticks [T.g0, T.g1).
+template <int SYS>
+__device__ __forceinline__ void lle_group(const JobDev& J, const Tile& T, long long group, int lane, int* err)
+{
+    const LleArgs& p = J.lle;
+    const long long q = group * 64 + lane;
     const bool valid = q < J.n_points;
     const long long qq = valid ? q : 0;
     const int i = (int)(qq % p.nPts), j = (int)(qq / p.nPts);
@@ -44,7 +56,11 @@ __device__ __forceinline__ void lle_tile(const JobDev& J, const Tile& T, int lan
     L.params(p, i, j);
     if (T.g0 == 0) L.init(p);
     else L.load(J.st, q);
-    L.ticks(p, (long long)j * p.nPts + i, T.g0, T.g1);
+    // the perturbation was stored with the job (lle_offset_kernel)
+    if (!L.template ticks<true>(p, 0, T.g0, T.g1, LleProg<SYS>::dir_of(J.st, q), J.st.np)) {
+        report(err, ENG_ERR_PROGRESS);
+        return;
+    }
     if (T.g1 == J.total) {
         if (valid) J.lle_out[q] = L.result(p);
     } else {
@@ -52,11 +68,12 @@ __device__ __forceinline__ void lle_tile(const JobDev& J, const Tile& T, int lan
     }
 }
 
-template <int SYS, int KIND>
-__device__ __forceinline__ void sweep_tile(const JobDev& J, const Tile& T, int lane)
+// One 64-point group of a sweep job; WV is the recorded variable.
+template <int SYS, int KIND, int WV>
+__device__ __forceinline__ void sweep_group(const JobDev& J, const Tile& T, long long group, int lane)
 {
-    const SweepArgs A = J.sw;
-    const long long q = (long long)T.group * 64 + lane;
+    const SweepArgs& A = J.sw;
+    const long long q = group * 64 + lane;
     const bool in_range = q < A.n_points;
     const long long qq = in_range ? q : 0;
     SweepProg<SYS, KIND> L;
@@ -67,53 +84,102 @@ __device__ __forceinline__ void sweep_tile(const JobDev& J, const Tile& T, int l
         L.params(A, qq);
         L.load(J.st, q);
         if (KIND == KIND_BIF) {
-            const WaveCursor c = J.cursors[T.group];
+            const WaveCursor c = J.cursors[group];
             cur.chunk = __builtin_amdgcn_readfirstlane(c.chunk);
             cur.fill = __builtin_amdgcn_readfirstlane(c.fill);
         }
     }
-    L.template ticks<-1>(A, J.wv, cur, T.group, T.g0, T.g1);
+    L.template ticks<WV>(A, WV, cur, (int)group, T.g0, T.g1);
     if (T.g1 == J.total) {
         if (KIND == KIND_BIF) close_chunk(A.pool, cur);
         if (in_range) L.finish(A, q);
     } else {
         L.store(J.st, q);
-        if (KIND == KIND_BIF && lane == 0) J.cursors[T.group] = cur;
+        if (KIND == KIND_BIF && lane == 0) J.cursors[group] = cur;
     }
 }
 
-__global__ void reset_head_kernel(unsigned* head)
+// Tile checks shared by the families: the tile's job must be of the
+// kernel's family, its tick range inside [0, total] and its groups inside
+// the job.  A failed check sets an error bit and skips the tile; the host
+// reports the round as failed.
+__device__ __forceinline__ bool tile_ok(const JobDev& J, const Tile& T, int fam, int* err)
 {
-    if (threadIdx.x == 0) *head = 0u;
+    if (engine_family(J.prog, J.lorenz, J.prog == PROG_LLE ? 0 : J.wv) != fam) {
+        report(err, ENG_ERR_FAMILY);
+        return false;
+    }
+    if (T.g0 < 0 || T.g1 <= T.g0 || T.g1 > J.total || T.group < 0 || (long long)(T.group + 1) * 64 > J.np) {
+        report(err, ENG_ERR_RANGE);
+        return false;
+    }
+    return true;
 }
 
-// Persistent workers: every wave pulls tiles until the queue is empty, so
-// the grid drains as soon as the last tile is taken.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
-void engine_kernel(const JobDev* __restrict__ jobs, const Tile* __restrict__ tiles, const int* __restrict__ ntiles,
-                   unsigned* __restrict__ head)
+// Persistent loop of one family: every wave claims `take` consecutive tiles
+// of its family's queue per atomic (points per tile = 64 x take) until the
+// queue is empty; each wave reaches the exit test after every claim, so the
+// grid drains once the last tile is taken.
+template <class Body>
+__device__ __forceinline__ void drain_family(const Tile* __restrict__ tiles, RoundCtl* __restrict__ ctl, int fam,
+                                             Body&& body)
 {
     const int lane = threadIdx.x & 63;
-    const int n = *ntiles;
+    FamQueue* Q = &ctl->fam[fam];
+    const int off = Q->off, n = Q->n, take = ctl->take;
     for (;;) {
         unsigned t = 0;
-        if (lane == 0) t = atomicAdd(head, 1u);
+        if (lane == 0) t = atomicAdd(&Q->head, (unsigned)take);
         t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
         if ((int)t >= n) break;
-        const Tile T = tiles[t];
-        const JobDev& J = jobs[T.slot];
-        switch (J.prog * 2 + J.lorenz) {
-        case 0: lle_tile<SYS_ROSSLER_RK4>(J, T, lane); break;
-        case 1: lle_tile<SYS_LORENZ_RK4>(J, T, lane); break;
-        case 2: sweep_tile<SYS_ROSSLER_RK4, KIND_BIF>(J, T, lane); break;
-        case 3: sweep_tile<SYS_LORENZ_RK4, KIND_BIF>(J, T, lane); break;
-        case 4: sweep_tile<SYS_ROSSLER_RK4, KIND_BASINS_REF>(J, T, lane); break;
-        case 5: sweep_tile<SYS_LORENZ_RK4, KIND_BASINS_REF>(J, T, lane); break;
-        case 6: sweep_tile<SYS_ROSSLER_RK4, KIND_BASINS_CFG>(J, T, lane); break;
-        case 7: sweep_tile<SYS_LORENZ_RK4, KIND_BASINS_CFG>(J, T, lane); break;
-        default: break;
-        }
+        const int e = min((int)t + take, n);
+        for (int u = (int)t; u < e; ++u) body(tiles[off + u], lane);
     }
+}
+
+template <int SYS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
+void engine_lle_kernel(const JobDev* __restrict__ jobs, const Tile* __restrict__ tiles, RoundCtl* __restrict__ ctl)
+{
+    constexpr int FAM = SYS == SYS_LORENZ_RK4 ? 1 : 0;
+    drain_family(tiles, ctl, FAM, [&](const Tile& T, int lane) {
+        const JobDev& J = jobs[T.slot];
+        if (!tile_ok(J, T, FAM, &ctl->err)) return;
+        lle_group<SYS>(J, T, T.group, lane, &ctl->err);
+    });
+}
+
+// The sweep programs need up to 80 VGPRs here (lane state load/store on
+// top of the one-shot kernels' loop): 6 waves per SIMD, no spills.
+template <int SYS, int KIND, int WV>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8)))
+void engine_sweep_kernel(const JobDev* __restrict__ jobs, const Tile* __restrict__ tiles, RoundCtl* __restrict__ ctl)
+{
+    constexpr int FAM = 2 + (((SYS == SYS_LORENZ_RK4 ? 1 : 0) * 3 + KIND) * 3 + WV);
+    drain_family(tiles, ctl, FAM, [&](const Tile& T, int lane) {
+        const JobDev& J = jobs[T.slot];
+        if (!tile_ok(J, T, FAM, &ctl->err)) return;
+        sweep_group<SYS, KIND, WV>(J, T, T.group, lane);
+    });
+}
+
+// Stores eps * (seeded unit vector) of every lane of an LLE job beside its
+// parked state, once, when the job is admitted: the tile loop then adds it
+// at tick S instead of generating it (the generator's registers would
+// otherwise sit in the tile loop).  Same value as the one-shot kernel's
+// perturbation (LleProg::offset).
+template <int SYS>
+__global__ void lle_offset_kernel(const JobDev* __restrict__ jobs, int slot)
+{
+    const JobDev& J = jobs[slot];
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= J.np) return;
+    const long long qq = q < J.n_points ? q : 0;
+    const int i = (int)(qq % J.lle.nPts), j = (int)(qq / J.lle.nPts);
+    double d[LleProg<SYS>::NX];
+    LleProg<SYS>::offset(J.lle, (long long)j * J.lle.nPts + i, d);
+    double* dir = J.st.d + LleProg<SYS>::ND * J.st.np + q;
+    for (int k = 0; k < LleProg<SYS>::NX; ++k) dir[k * J.st.np] = d[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -147,6 +213,10 @@ Engine::Engine(int device, const chaos_engine_params& prm) : prm_(prm), device_(
     CHAOS_REQUIRE(prm_.waves_per_simd >= 1 && prm_.waves_per_simd <= 8, "waves_per_simd must be in [1,8]");
     CHAOS_REQUIRE(prm_.steps_per_launch >= 2, "steps_per_launch must be >= 2");
     CHAOS_REQUIRE(prm_.max_jobs >= 1 && prm_.max_tiles >= 1, "max_jobs and max_tiles must be >= 1");
+    if (prm_.points_per_tile == 0) prm_.points_per_tile = 64;
+    CHAOS_REQUIRE(prm_.points_per_tile == 64 || prm_.points_per_tile == 128 || prm_.points_per_tile == 256,
+                  "points_per_tile must be 64, 128 or 256");
+    gpt_ = prm_.points_per_tile / 64;
     chaos_options o = resolve_options(nullptr);
     o.device = device;
     ctx_.reset(new CallCtx(o));
@@ -155,13 +225,20 @@ Engine::Engine(int device, const chaos_engine_params& prm) : prm_(prm), device_(
     ctx_.reset(new CallCtx(o));
     CHAOS_HIP(hipEventCreate(&ev0_));
     CHAOS_HIP(hipEventCreate(&ev1_));
+    for (int k = 0; k < kLanes - 1; ++k) {
+        CHAOS_HIP(hipStreamCreateWithFlags(&aux_[k], hipStreamNonBlocking));  // default priority
+        CHAOS_HIP(hipEventCreateWithFlags(&join_[k], hipEventDisableTiming));
+    }
+    CHAOS_HIP(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
     d_jobs_.alloc(prm_.max_jobs);
     d_tiles_.alloc(prm_.max_tiles);
-    d_ntiles_.alloc(1);
-    d_head_.alloc(1);
+    d_ctl_.alloc(1);
     CHAOS_HIP(hipHostMalloc((void**)&h_jobs_, sizeof(JobDev) * prm_.max_jobs, hipHostMallocDefault));
     CHAOS_HIP(hipHostMalloc((void**)&h_tiles_, sizeof(Tile) * prm_.max_tiles, hipHostMallocDefault));
-    CHAOS_HIP(hipHostMalloc((void**)&h_ntiles_, sizeof(int), hipHostMallocDefault));
+    CHAOS_HIP(hipHostMalloc((void**)&h_ctl_, sizeof(RoundCtl), hipHostMallocDefault));
+    CHAOS_HIP(hipHostMalloc((void**)&h_err_, sizeof(int), hipHostMallocDefault));
+    *h_ctl_ = RoundCtl{};
+    *h_err_ = 0;
     for (int s = prm_.max_jobs - 1; s >= 0; --s) free_slots_.push_back(s);
     const hipDeviceProp_t& props = device_props(device);
     const int waves = props.multiProcessorCount * 4 * prm_.waves_per_simd;
@@ -239,17 +316,22 @@ Engine::~Engine()
     if (worker_.joinable()) worker_.join();
     if (fstream_) (void)hipStreamDestroy(fstream_);
     jobs_.clear();
-    if (exec_) (void)hipGraphExecDestroy(exec_);
-    if (graph_) (void)hipGraphDestroy(graph_);
+    for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+    graphs_.clear();
     if (h_jobs_) (void)hipHostFree(h_jobs_);
     if (h_tiles_) (void)hipHostFree(h_tiles_);
-    if (h_ntiles_) (void)hipHostFree(h_ntiles_);
+    if (h_ctl_) (void)hipHostFree(h_ctl_);
+    if (h_err_) (void)hipHostFree(h_err_);
+    for (int k = 0; k < kLanes - 1; ++k) {
+        if (join_[k]) (void)hipEventDestroy(join_[k]);
+        if (aux_[k]) (void)hipStreamDestroy(aux_[k]);
+    }
+    if (fork_) (void)hipEventDestroy(fork_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
     d_jobs_.release();
     d_tiles_.release();
-    d_ntiles_.release();
-    d_head_.release();
+    d_ctl_.release();
     ctx_.reset();
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -313,13 +395,16 @@ void Engine::plan(JobRec& r)
     } else {
         fail(CHAOS_E_INVALID, "unknown job kind %d", j.kind);
     }
+    CHAOS_REQUIRE(j.writableVar >= 0 && j.writableVar < nx, "writableVar %d out of range [0,%d)", j.writableVar, nx);
+    r.family = engine_family(r.prog, r.lorenz, r.prog == PROG_LLE ? 0 : j.writableVar);
     r.groups = (r.n + 63) / 64;
     r.np = r.groups * 64;
-    CHAOS_REQUIRE(r.groups <= prm_.max_tiles, "job needs %lld tiles, queue holds %d", r.groups, prm_.max_tiles);
+    r.tiles = r.groups;
+    CHAOS_REQUIRE(r.tiles <= prm_.max_tiles, "job needs %lld tiles, queue holds %d", r.tiles, prm_.max_tiles);
     // HBM footprint: lane state + outputs + per-kind working sets
     long long fp = 0;
     if (r.prog == PROG_LLE) {
-        fp = r.np * (2 * nx + 1) * 8 + r.n * 8;
+        fp = r.np * (3 * nx + 1) * 8 + r.n * 8;  // parked state + stored perturbation, map
     } else {
         fp = r.np * ((nx + 8) * 8 + 4 * 4);
         if (r.prog == PROG_BIF)
@@ -358,7 +443,7 @@ void Engine::allocate(JobRec& r)
     D.n_points = r.n;
     D.total = r.total;
     if (r.prog == PROG_LLE) {
-        r.dstate.alloc((size_t)r.np * (2 * nx + 1));
+        r.dstate.alloc((size_t)r.np * (3 * nx + 1));
         r.out.alloc((size_t)r.n);
         D.st = LaneStore{r.dstate.get(), nullptr, r.np};
         D.lle = r.lle;
@@ -396,6 +481,15 @@ void Engine::allocate(JobRec& r)
         }
     }
     CHAOS_HIP(hipMemcpyAsync(d_jobs_.get() + r.slot, &D, sizeof(JobDev), hipMemcpyHostToDevice, stream_));
+    if (r.prog == PROG_LLE) {
+        const unsigned g = ceil_div((unsigned long long)r.np, 256);
+        if (r.lorenz)
+            hipLaunchKernelGGL(lle_offset_kernel<SYS_LORENZ_RK4>, dim3(g), dim3(256), 0, stream_, d_jobs_.get(), r.slot);
+        else
+            hipLaunchKernelGGL(lle_offset_kernel<SYS_ROSSLER_RK4>, dim3(g), dim3(256), 0, stream_, d_jobs_.get(),
+                               r.slot);
+        CHAOS_HIP(hipGetLastError());
+    }
 }
 
 void Engine::admit()
@@ -407,14 +501,14 @@ void Engine::admit()
             if (hbm_used_ + r.footprint > hbm_budget_) break;
             hbm_used_ += r.footprint;
         }
-        if (active_groups_ + r.groups > prm_.max_tiles) {
+        if (active_tiles_ + r.tiles > prm_.max_tiles) {
             std::lock_guard<std::mutex> lk(mu_);
             hbm_used_ -= r.footprint;
             break;
         }
         r.slot = free_slots_.back();
         free_slots_.pop_back();
-        active_groups_ += r.groups;
+        active_tiles_ += r.tiles;
         r.status = 1;
         r.t_admit = now_ms();
         allocate(r);
@@ -423,25 +517,104 @@ void Engine::admit()
     }
 }
 
-void Engine::capture()
+namespace {
+
+template <int F>
+struct FamilyKernel {
+    static constexpr int sweep = F - 2;
+    static constexpr int sys = (F < 2 ? F : sweep / 9) == 1 ? SYS_LORENZ_RK4 : SYS_ROSSLER_RK4;
+    static constexpr int kind = (sweep / 3) % 3;
+    static constexpr int wv = sweep % 3;
+    static void launch(int grid, int block, hipStream_t s, const JobDev* jobs, const Tile* tiles, RoundCtl* ctl)
+    {
+        if constexpr (F < 2)
+            hipLaunchKernelGGL(engine_lle_kernel<sys>, dim3(grid), dim3(block), 0, s, jobs, tiles, ctl);
+        else
+            hipLaunchKernelGGL((engine_sweep_kernel<sys, kind, wv>), dim3(grid), dim3(block), 0, s, jobs, tiles, ctl);
+    }
+};
+
+template <int... F>
+void launch_family_of(int f, std::integer_sequence<int, F...>, int grid, int block, hipStream_t s,
+                      const JobDev* jobs, const Tile* tiles, RoundCtl* ctl)
 {
-    CHAOS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-    enqueue_launch();
-    CHAOS_HIP(hipStreamEndCapture(stream_, &graph_));
-    CHAOS_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+    ((f == F ? FamilyKernel<F>::launch(grid, block, s, jobs, tiles, ctl) : void()), ...);
 }
 
-// The per-round launch sequence: tile list and count to the device, queue
-// head reset by a kernel (no memset node), persistent kernel.
-void Engine::enqueue_launch()
+}  // namespace
+
+void Engine::launch_family(int f, hipStream_t s)
+{
+    launch_family_of(f, std::make_integer_sequence<int, kFamilies>{}, grid_, prm_.block, s, d_jobs_.get(),
+                     d_tiles_.get(), d_ctl_.get());
+    CHAOS_HIP(hipGetLastError());
+}
+
+// The per-round launch sequence: the tile list and the round control (queue
+// heads and error word at 0: a copy, no memset node) to the device, the
+// kernels of the round's families spread over the lanes' streams (fork /
+// join by events), the error word back to pinned host memory.
+void Engine::enqueue_round(unsigned mask)
 {
     CHAOS_HIP(hipMemcpyAsync(d_tiles_.get(), h_tiles_, sizeof(Tile) * (size_t)prm_.max_tiles, hipMemcpyHostToDevice,
                              stream_));
-    CHAOS_HIP(hipMemcpyAsync(d_ntiles_.get(), h_ntiles_, sizeof(int), hipMemcpyHostToDevice, stream_));
-    hipLaunchKernelGGL(reset_head_kernel, dim3(1), dim3(64), 0, stream_, d_head_.get());
-    hipLaunchKernelGGL(engine_kernel, dim3(grid_), dim3(prm_.block), 0, stream_, d_jobs_.get(), d_tiles_.get(),
-                       d_ntiles_.get(), d_head_.get());
-    CHAOS_HIP(hipGetLastError());
+    CHAOS_HIP(hipMemcpyAsync(d_ctl_.get(), h_ctl_, sizeof(RoundCtl), hipMemcpyHostToDevice, stream_));
+    int used = 0;
+    for (int f = 0; f < kFamilies; ++f)
+        if (mask >> f & 1u) ++used;
+    const int lanes = std::min(used, (int)kLanes);
+    if (lanes > 1) {
+        CHAOS_HIP(hipEventRecord(fork_, stream_));
+        for (int k = 0; k < lanes - 1; ++k) CHAOS_HIP(hipStreamWaitEvent(aux_[k], fork_, 0));
+    }
+    int i = 0;
+    for (int f = 0; f < kFamilies; ++f) {
+        if (!(mask >> f & 1u)) continue;
+        const int lane = i++ % lanes;
+        launch_family(f, lane == 0 ? stream_ : aux_[lane - 1]);
+    }
+    if (lanes > 1) {
+        for (int k = 0; k < lanes - 1; ++k) {
+            CHAOS_HIP(hipEventRecord(join_[k], aux_[k]));
+            CHAOS_HIP(hipStreamWaitEvent(stream_, join_[k], 0));
+        }
+    }
+    CHAOS_HIP(hipMemcpyAsync(h_err_, &d_ctl_.get()->err, sizeof(int), hipMemcpyDeviceToHost, stream_));
+}
+
+hipGraphExec_t Engine::capture(unsigned mask)
+{
+    auto it = graphs_.find(mask);
+    if (it != graphs_.end()) return it->second;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
+    CHAOS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    enqueue_round(mask);
+    CHAOS_HIP(hipStreamEndCapture(stream_, &g));
+    const hipError_t ie = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    CHAOS_HIP(ie);
+    graphs_[mask] = e;
+    return e;
+}
+
+// A device error fails every job still integrating (their lane state can no
+// longer be trusted) and the call that ran the round.
+void Engine::fail_active(const std::string& msg)
+{
+    for (long long id : active_) {
+        JobRec& r = *jobs_[id];
+        r.error = msg;
+        r.t_done = now_ms();
+        r.status = -1;
+        active_tiles_ -= r.tiles;
+        free_slots_.push_back(r.slot);
+        r.slot = -1;
+        std::lock_guard<std::mutex> lk(mu_);
+        hbm_used_ -= r.footprint;
+    }
+    active_.clear();
+    fail(CHAOS_E_HIP, "%s", msg.c_str());
 }
 
 void Engine::finalize(JobRec& r, CallCtx& ctx)
@@ -528,11 +701,27 @@ void Engine::finalize(JobRec& r, CallCtx& ctx)
 int Engine::step()
 {
     admit();
-    int nt = 0;
+    // tiles grouped by family: family f owns tiles [off_f, off_f + n_f)
+    int cnt[kFamilies] = {};
     for (long long id : active_) {
-        JobRec& r = *jobs_[id];
+        const JobRec& r = *jobs_[id];
+        cnt[r.family] += (int)r.tiles;
+    }
+    int fill[kFamilies];
+    unsigned mask = 0;
+    int nt = 0;
+    for (int f = 0; f < kFamilies; ++f) {
+        h_ctl_->fam[f] = FamQueue{nt, cnt[f], 0u, 0};
+        fill[f] = nt;
+        nt += cnt[f];
+        if (cnt[f]) mask |= 1u << f;
+    }
+    h_ctl_->err = 0;
+    h_ctl_->take = gpt_;
+    for (long long id : active_) {
+        const JobRec& r = *jobs_[id];
         const long long g0 = r.done, g1 = std::min(r.total, r.done + r.ticks_per_launch);
-        for (long long g = 0; g < r.groups; ++g) h_tiles_[nt++] = Tile{r.slot, (int)g, g0, g1};
+        for (long long g = 0; g < r.groups; ++g) h_tiles_[fill[r.family]++] = Tile{r.slot, (int)g, g0, g1};
     }
     if (nt == 0) {
         // nothing to integrate: wait for a job being finished (it may free
@@ -540,14 +729,13 @@ int Engine::step()
         wait_finisher_progress();
         return take_completed();
     }
-    *h_ntiles_ = nt;
+    *h_err_ = 0;
     CHAOS_HIP(hipEventRecord(ev0_, stream_));
     if (prm_.use_graph) {
-        if (!exec_) capture();
-        CHAOS_HIP(hipGraphLaunch(exec_, stream_));
+        CHAOS_HIP(hipGraphLaunch(capture(mask), stream_));
         ++graph_replays_;
     } else {
-        enqueue_launch();
+        enqueue_round(mask);
     }
     CHAOS_HIP(hipEventRecord(ev1_, stream_));
     CHAOS_HIP(hipEventSynchronize(ev1_));
@@ -556,6 +744,10 @@ int Engine::step()
     busy_ms_ += ms;
     ++launches_;
     tiles_ += nt;
+    // the jobs advance only when the device confirms the round
+    if (*h_err_ != 0)
+        fail_active("engine round failed on the device (error bits " + std::to_string(*h_err_) +
+                    ": 1 tile of another family, 2 tile range outside its job, 4 program made no progress)");
     std::vector<long long> still;
     still.reserve(active_.size());
     std::vector<JobRec*> done;
@@ -569,7 +761,7 @@ int Engine::step()
         // integration finished: the slot and tile budget are free now, the
         // HBM footprint stays charged until the worker has finished the job
         r.status = 3;
-        active_groups_ -= r.groups;
+        active_tiles_ -= r.tiles;
         free_slots_.push_back(r.slot);
         r.slot = -1;
         done.push_back(&r);
@@ -645,6 +837,8 @@ extern "C" int chaos_engine_params_default(chaos_engine_params* p)
     p->max_tiles = 65536;
     p->use_graph = 1;
     p->hbm_fraction = 0.6;
+    p->points_per_tile = 64;
+    p->reserved = 0;
     return CHAOS_OK;
 }
 

@@ -38,10 +38,46 @@ struct JobDev {
 };
 
 // One unit of queued work: ticks [g0, g1) of a 64-point group of a job.
+// A job's groups sit consecutively in its family's queue; a wave claims
+// `take` of them at once (RoundCtl.take).
 struct Tile {
     int slot;
     int group;
     long long g0, g1;
+};
+
+// Kernel families: one persistent kernel per (program, system, recorded
+// variable), each compiled on its own: 2 LLE families (Rossler, Lorenz) and
+// 2 systems x 3 sweep kinds x 3 recorded variables.
+constexpr int kFamilies = 2 + 2 * 3 * 3;
+
+__host__ __device__ inline int engine_family(int prog, int lorenz, int wv)
+{
+    return prog == PROG_LLE ? lorenz : 2 + ((lorenz * 3 + (prog - 1)) * 3 + wv);
+}
+
+// Per-round queue of one family: its tiles are tiles[off, off + n), taken
+// in order through the atomic head.
+struct FamQueue {
+    int off;
+    int n;
+    unsigned head;
+    int pad;
+};
+
+// Everything the host hands the device per round in one copy: the family
+// queues (heads at 0), the error word (0) and the tiles a wave claims per
+// atomic.  Kernels set error bits instead of looping on a malformed tile.
+enum EngineError : int {
+    ENG_ERR_FAMILY = 1,    // a tile's job is not of the kernel's family
+    ENG_ERR_RANGE = 2,     // tick or group range outside the job
+    ENG_ERR_PROGRESS = 4   // a program could not advance its tick range
+};
+struct RoundCtl {
+    FamQueue fam[kFamilies];
+    int err;
+    int take;
+    int pad[2];
 };
 
 struct JobRec {
@@ -50,11 +86,11 @@ struct JobRec {
     std::atomic<int> status{0};  // 0 pending, 1 active, 3 finishing, 2 done, -1 failed
     std::string error;           // failure message (status -1)
     int slot = -1;
-    int prog = 0, lorenz = 0, reran = 0;
+    int prog = 0, lorenz = 0, reran = 0, family = 0;
     double t_submit = 0, t_admit = 0, t_done = 0;
     double rk4_steps = 0;
     long long footprint = 0;
-    long long n = 0, np = 0, groups = 0, total = 0, done = 0, ticks_per_launch = 0, pool_chunks = 0;
+    long long n = 0, np = 0, groups = 0, tiles = 0, total = 0, done = 0, ticks_per_launch = 0, pool_chunks = 0;
     LleArgs lle{};
     SweepArgs sw{};
     // device working set (released when the job finishes)
@@ -89,8 +125,10 @@ private:
     void admit();
     void allocate(JobRec& r);
     void finalize(JobRec& r, CallCtx& ctx);
-    void capture();
-    void enqueue_launch();
+    hipGraphExec_t capture(unsigned mask);
+    void enqueue_round(unsigned mask);
+    void launch_family(int f, hipStream_t s);
+    void fail_active(const std::string& msg);
     void finisher();                 // worker thread body
     int take_completed();            // jobs finished since the last call
     void wait_finisher_progress();   // block until a finishing job completes
@@ -103,19 +141,24 @@ private:
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     DevBuf<JobDev> d_jobs_;
     DevBuf<Tile> d_tiles_;
-    DevBuf<int> d_ntiles_;
-    DevBuf<unsigned> d_head_;
+    DevBuf<RoundCtl> d_ctl_;
     JobDev* h_jobs_ = nullptr;   // pinned
     Tile* h_tiles_ = nullptr;    // pinned
-    int* h_ntiles_ = nullptr;    // pinned
-    hipGraph_t graph_ = nullptr;
-    hipGraphExec_t exec_ = nullptr;
-    int grid_ = 0;
+    RoundCtl* h_ctl_ = nullptr;  // pinned
+    int* h_err_ = nullptr;       // pinned: the round's error word, copied back
+    // family kernels of one round run on up to kLanes streams (the tail of
+    // one family's queue overlaps the next family's kernel)
+    static constexpr int kLanes = 4;
+    hipStream_t aux_[kLanes - 1] = {};
+    hipEvent_t fork_ = nullptr, join_[kLanes - 1] = {};
+    std::map<unsigned, hipGraphExec_t> graphs_;  // family set -> captured round
+    int grid_ = 0, gpt_ = 1;     // persistent grid; groups per tile
+    long long active_tiles_ = 0;
     std::deque<long long> pending_;
     std::vector<long long> active_;
     std::map<long long, std::unique_ptr<JobRec>> jobs_;
     std::vector<int> free_slots_;
-    long long next_id_ = 1, hbm_budget_ = 0, hbm_used_ = 0, active_groups_ = 0;
+    long long next_id_ = 1, hbm_budget_ = 0, hbm_used_ = 0;
     long long launches_ = 0, graph_replays_ = 0, tiles_ = 0;
     double busy_ms_ = 0, finalize_ms_ = 0, steps_done_ = 0, t0_ = 0;
     // Finishing stages (peak post-processing, clustering, result copies) run

@@ -78,22 +78,28 @@ struct LaneStore {
 // LLE: Benettin with two trajectories (LLE/src/LLECUDA.cu:56-166)
 // ticks [0, S): transient on X; tick S starts with the perturbation; ticks
 // [S, S + W*NTp) step X and Xp once each, renormalising after every NTp.
+// Host-callable as well, so the tick-split invariance and the progress guard
+// are checked on the CPU.
 // ---------------------------------------------------------------------------
 template <int SYS>
 struct LleProg {
     using Sy = System<SYS>;
     static constexpr int NX = Sy::NX, NA = Sy::NA;
-    static constexpr int ND = 2 * NX + 1;
+    static constexpr int ND = 2 * NX + 1;   // parked state: X, Xp, sum
+    static constexpr int NDIR = NX;         // engine: eps * direction, after the parked state
     double X[NX], Xp[NX], a[NA], sum;
 
-    __device__ __forceinline__ static long long total(const LleArgs& p) { return (long long)p.S + (long long)p.W * p.NTp; }
+    __host__ __device__ __forceinline__ static long long total(const LleArgs& p)
+    {
+        return (long long)p.S + (long long)p.W * p.NTp;
+    }
 
-    __device__ __forceinline__ void params(const LleArgs& p, int i, int j)
+    __host__ __device__ __forceinline__ void params(const LleArgs& p, int i, int j)
     {
         build_params<NA>(a, p.values, p.idxA, grid_value(i, p.nPts, p.rA0, p.rA1), p.idxB,
                          grid_value(j, p.nPts, p.rB0, p.rB1));
     }
-    __device__ __forceinline__ void init(const LleArgs& p)
+    __host__ __device__ __forceinline__ void init(const LleArgs& p)
     {
 #pragma unroll
         for (int k = 0; k < NX; ++k) {
@@ -102,8 +108,28 @@ struct LleProg {
         }
         sum = 0.0;
     }
-    __device__ __forceinline__ void ticks(const LleArgs& p, long long point, long long g0, long long g1)
+    // The perturbation added at tick S: eps * v with v the point's seeded
+    // unit vector (LLE/src/LLECUDA.cu:88-102 draws it from curand instead).
+    __host__ __device__ __forceinline__ static void offset(const LleArgs& p, long long point, double* d)
     {
+        double v[NX];
+        unit_vector<NX>(p.seed, point, v);
+#pragma unroll
+        for (int k = 0; k < NX; ++k) d[k] = p.eps * v[k];
+    }
+
+    // Advance over ticks [g0, g1).  The perturbation at tick S comes from
+    // offset() (DIR_STORED false) or is read from dir[k * dstride] (the
+    // engine stores it when the job starts, which keeps the unit-vector code
+    // out of its tile loop).  Returns false without stepping when the range
+    // cannot make progress -- NTp < 1 would leave every window empty and the
+    // window loop would never advance -- so a caller handed a malformed job
+    // reports it instead of spinning.
+    template <bool DIR_STORED = false>
+    __host__ __device__ __forceinline__ bool ticks(const LleArgs& p, long long point, long long g0, long long g1,
+                                                   const double* dir = nullptr, long long dstride = 0)
+    {
+        if (p.NTp < 1 || p.S < 0 || g0 < 0 || g1 < g0) return false;
         long long g = g0;
         if (g < p.S) {
             const long long e = g1 < p.S ? g1 : p.S;
@@ -111,18 +137,20 @@ struct LleProg {
             g = e;
         }
         if (g == p.S && g < g1) {
-            double v[NX];
-            unit_vector<NX>(p.seed, point, v);
+            double d[NX];
+            if (DIR_STORED) {
 #pragma unroll
-            for (int k = 0; k < NX; ++k) {
-                const double d = p.eps * v[k];
-                Xp[k] = X[k] + d;
+                for (int k = 0; k < NX; ++k) d[k] = dir[k * dstride];
+            } else {
+                offset(p, point, d);
             }
+#pragma unroll
+            for (int k = 0; k < NX; ++k) Xp[k] = X[k] + d[k];
         }
         int local = g < g1 ? (int)((g - p.S) % p.NTp) : 0;  // position in the first window
         while (g < g1) {
             const long long left = g1 - g;
-            const int n = (p.NTp - local) < left ? (p.NTp - local) : (int)left;
+            const int n = (p.NTp - local) < left ? (p.NTp - local) : (int)left;  // >= 1: local < NTp
             for (int s = 0; s < n; ++s) {
                 Sy::step(X, a, p.sc);
                 Sy::step(Xp, a, p.sc);
@@ -143,10 +171,11 @@ struct LleProg {
                 for (int l = 0; l < NX; ++l) Xp[l] = X[l] - ((X[l] - Xp[l]) / dist);
             }
         }
+        return true;
     }
-    __device__ __forceinline__ double result(const LleArgs& p) const { return sum / p.tMax; }
+    __host__ __device__ __forceinline__ double result(const LleArgs& p) const { return sum / p.tMax; }
 
-    __device__ __forceinline__ void load(const LaneStore& L, long long q)
+    __host__ __device__ __forceinline__ void load(const LaneStore& L, long long q)
     {
 #pragma unroll
         for (int k = 0; k < NX; ++k) {
@@ -155,7 +184,7 @@ struct LleProg {
         }
         sum = L.d[2 * NX * L.np + q];
     }
-    __device__ __forceinline__ void store(const LaneStore& L, long long q) const
+    __host__ __device__ __forceinline__ void store(const LaneStore& L, long long q) const
     {
 #pragma unroll
         for (int k = 0; k < NX; ++k) {
@@ -163,6 +192,11 @@ struct LleProg {
             L.d[(NX + k) * L.np + q] = Xp[k];
         }
         L.d[2 * NX * L.np + q] = sum;
+    }
+    // Stored perturbation of lane q (engine): dir[k * np] for k < NX.
+    __host__ __device__ __forceinline__ static const double* dir_of(const LaneStore& L, long long q)
+    {
+        return L.d + ND * L.np + q;
     }
 };
 

@@ -9,6 +9,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <initializer_list>
+#include <string>
+#include <utility>
 #include <thread>
 #include <vector>
 
@@ -338,20 +341,36 @@ extern "C" int chaos_stream_run(int device, const chaos_stream_config* cfg, cons
     });
 }
 
+extern "C" int chaos_stream_jobs(const chaos_stream_config* cfg, chaos_job* jobs, double* arrival_ms, int cap,
+                                 int* count)
+{
+    return guarded("chaos_stream_jobs", [&] {
+        CHAOS_REQUIRE(cfg && count, "NULL argument");
+        CHAOS_REQUIRE(cfg->lambda > 0 && cfg->duration_s > 0 && cfg->horizon_scale > 0,
+                      "lambda, duration and scale must be > 0");
+        const std::vector<StreamJob> v = make_stream(*cfg);
+        *count = (int)v.size();
+        for (int k = 0; k < cap && k < (int)v.size(); ++k) {
+            if (jobs) jobs[k] = v[k].job;
+            if (arrival_ms) arrival_ms[k] = v[k].arrival_ms;
+        }
+    });
+}
+
 // ---------------------------------------------------------------------------
 // autotuner
 // ---------------------------------------------------------------------------
 
+// Calibration mix: the stream's three kinds at the configurations' physics,
+// several sizes each, submitted together; about 2e11 RK4 steps (a few tenths
+// of a second of device time) so the rate reflects the kernels, not the
+// per-round overheads.
 static double calibrate(int device, const chaos_engine_params& prm)
 {
-    // fixed mix: all three kinds, several sizes, submitted together
     std::vector<chaos_job> mix;
-    const int sizes[4] = {4096, 9216, 16384, 36864};
-    for (int k = 0; k < 4; ++k) {
-        mix.push_back(make_stream_job(1, sizes[k], 0.02));
-        mix.push_back(make_stream_job(0, 512 * (k + 1), 0.05));
-        mix.push_back(make_stream_job(2, sizes[k], 0.05));
-    }
+    for (int side : {128, 256, 512, 724}) mix.push_back(make_stream_job(1, side * side, 0.5));
+    for (int side : {256, 512}) mix.push_back(make_stream_job(2, side * side, 0.5));
+    for (int n : {2048, 8192}) mix.push_back(make_stream_job(0, n, 0.5));
     double steps = 0;
     for (const chaos_job& j : mix) steps += job_steps(j);
     Engine eng(device, prm);
@@ -361,44 +380,69 @@ static double calibrate(int device, const chaos_engine_params& prm)
     return steps / ((clock_ms() - t0) / 1000.0);
 }
 
+static bool same_params(const chaos_engine_params& a, const chaos_engine_params& b)
+{
+    return a.block == b.block && a.waves_per_simd == b.waves_per_simd && a.steps_per_launch == b.steps_per_launch &&
+           a.points_per_tile == b.points_per_tile && a.use_graph == b.use_graph;
+}
+
+// Coordinate search from the defaults: steps per launch, points per tile,
+// persistent waves per SIMD, block size and graph replay in turn, each over
+// its candidates with the others at the best found so far.  Every candidate
+// measured is written to the cache file after the chosen parameters.
 extern "C" int chaos_autotune(int device, const char* cache_path, chaos_engine_params* best,
                               double* best_steps_per_s)
 {
     return guarded("chaos_autotune", [&] {
         CHAOS_REQUIRE(best, "best is NULL");
-        chaos_engine_params base;
-        chaos_engine_params_default(&base);
-        calibrate(device, base);  // warm-up: code objects, allocator
-        double best_rate = -1.0;
-        chaos_engine_params b = base;
-        const int blocks[2] = {64, 256};
-        const int waves[3] = {4, 6, 8};
-        const int steps[3] = {4000, 16000, 64000};
-        for (int bl : blocks)
-            for (int w : waves)
-                for (int k : steps) {
-                    chaos_engine_params p = base;
-                    p.block = bl;
-                    p.waves_per_simd = w;
-                    p.steps_per_launch = k;
-                    const double r = calibrate(device, p);
-                    if (r > best_rate) {
-                        best_rate = r;
-                        b = p;
-                    }
+        chaos_engine_params b;
+        chaos_engine_params_default(&b);
+        calibrate(device, b);  // warm-up: code objects, allocator
+        std::vector<std::pair<chaos_engine_params, double>> tried;
+        auto eval = [&](const chaos_engine_params& p) {
+            for (const auto& t : tried)
+                if (same_params(t.first, p)) return t.second;
+            const double r = calibrate(device, p);
+            tried.emplace_back(p, r);
+            return r;
+        };
+        double best_rate = eval(b);
+        auto sweep = [&](auto set, std::initializer_list<int> values) {
+            for (int v : values) {
+                chaos_engine_params p = b;
+                set(p, v);
+                const double r = eval(p);
+                if (r > best_rate) {
+                    best_rate = r;
+                    b = p;
                 }
+            }
+        };
+        sweep([](chaos_engine_params& p, int v) { p.steps_per_launch = v; }, {5000, 10000, 20000, 40000});
+        sweep([](chaos_engine_params& p, int v) { p.points_per_tile = v; }, {64, 128, 256});
+        sweep([](chaos_engine_params& p, int v) { p.waves_per_simd = v; }, {4, 6, 8});
+        sweep([](chaos_engine_params& p, int v) { p.block = v; }, {64, 128, 256});
+        sweep([](chaos_engine_params& p, int v) { p.use_graph = v; }, {0, 1});
         *best = b;
         if (best_steps_per_s) *best_steps_per_s = best_rate;
         if (cache_path) {
             FILE* f = std::fopen(cache_path, "w");
             if (!f) fail(CHAOS_E_IO, "cannot write autotune cache '%s'", cache_path);
-            std::fprintf(f, "block %d\nwaves_per_simd %d\nsteps_per_launch %d\nsteps_per_s %.6e\n", b.block,
-                         b.waves_per_simd, b.steps_per_launch, best_rate);
+            std::fprintf(f, "block %d\nwaves_per_simd %d\nsteps_per_launch %d\npoints_per_tile %d\nuse_graph %d\n"
+                         "steps_per_s %.6e\n",
+                         b.block, b.waves_per_simd, b.steps_per_launch, b.points_per_tile, b.use_graph, best_rate);
+            for (const auto& t : tried)
+                std::fprintf(f, "# candidate block %d waves_per_simd %d steps_per_launch %d points_per_tile %d "
+                             "use_graph %d steps_per_s %.6e\n",
+                             t.first.block, t.first.waves_per_simd, t.first.steps_per_launch, t.first.points_per_tile,
+                             t.first.use_graph, t.second);
             std::fclose(f);
         }
     });
 }
 
+// Reads the chosen parameters of a cache file (lines "key value"; lines
+// starting with '#' are the candidates' record and are skipped).
 extern "C" int chaos_autotune_load(const char* cache_path, chaos_engine_params* p)
 {
     return guarded("chaos_autotune_load", [&] {
@@ -406,13 +450,16 @@ extern "C" int chaos_autotune_load(const char* cache_path, chaos_engine_params* 
         FILE* f = std::fopen(cache_path, "r");
         if (!f) fail(CHAOS_E_IO, "cannot read autotune cache '%s'", cache_path);
         chaos_engine_params_default(p);
-        char key[64];
+        char line[512], key[64];
         double val;
-        while (std::fscanf(f, "%63s %lf", key, &val) == 2) {
+        while (std::fgets(line, sizeof line, f)) {
+            if (line[0] == '#' || std::sscanf(line, "%63s %lf", key, &val) != 2) continue;
             const std::string k(key);
             if (k == "block") p->block = (int)val;
             else if (k == "waves_per_simd") p->waves_per_simd = (int)val;
             else if (k == "steps_per_launch") p->steps_per_launch = (int)val;
+            else if (k == "points_per_tile") p->points_per_tile = (int)val;
+            else if (k == "use_graph") p->use_graph = (int)val;
         }
         std::fclose(f);
     });

@@ -74,7 +74,7 @@ struct Occupancy<6> {
 template <>
 struct System<SYS_CHAMELEON> {
     static constexpr int NX = 3, NA = 7;
-    __device__ __forceinline__ static void step(double* X, const double* a, const StepConst& c)
+    __host__ __device__ __forceinline__ static void step(double* X, const double* a, const StepConst& c)
     {
         const double h1 = a[0] * c.h;
         const double h2 = (1 - a[0]) * c.h;
@@ -93,7 +93,7 @@ struct System<SYS_CHAMELEON> {
 template <>
 struct System<SYS_ROSSLER_CD> {
     static constexpr int NX = 3, NA = 4;
-    __device__ __forceinline__ static void step(double* x, const double* a, const StepConst& c)
+    __host__ __device__ __forceinline__ static void step(double* x, const double* a, const StepConst& c)
     {
         const double h1 = 0.5 * c.h + a[0];
         const double h2 = 0.5 * c.h - a[0];
@@ -112,7 +112,7 @@ struct System<SYS_ROSSLER_CD> {
 template <>
 struct System<SYS_BASINS> {
     static constexpr int NX = 3, NA = 5;
-    __device__ __forceinline__ static void step(double* X, const double* a, const StepConst& c)
+    __host__ __device__ __forceinline__ static void step(double* X, const double* a, const StepConst& c)
     {
         const double h = c.h;
         const double h2 = (double)(float)(h * (1 - a[0]));
@@ -129,7 +129,7 @@ struct System<SYS_BASINS> {
 template <>
 struct System<SYS_BASINS_2> {
     static constexpr int NX = 2, NA = 2;
-    __device__ __forceinline__ static void step(double* X, const double* a, const StepConst& c)
+    __host__ __device__ __forceinline__ static void step(double* X, const double* a, const StepConst& c)
     {
         const double h1 = (double)(float)(c.h * a[0]);
         const double h2 = (double)(float)(c.h * (1 - a[0]));
@@ -146,7 +146,7 @@ struct System<SYS_BASINS_2> {
 template <>
 struct System<SYS_DAMIR> {
     static constexpr int NX = 3, NA = 2;
-    __device__ __forceinline__ static void step(double* X, const double* a, const StepConst& c)
+    __host__ __device__ __forceinline__ static void step(double* X, const double* a, const StepConst& c)
     {
         const double h = c.h;
         const double Vc = X[0], XSV = X[1], t = X[2];
@@ -184,7 +184,7 @@ struct System<SYS_DAMIR> {
 // The weighted sum is accumulated as the stages complete, so only X, Y, k
 // and s are live: 12 doubles per trajectory.
 template <class F>
-__device__ __forceinline__ void rk4_step(double* X, const double* a, const StepConst& c)
+__host__ __device__ __forceinline__ void rk4_step(double* X, const double* a, const StepConst& c)
 {
     double k[3], Y[3], s[3];
     F::rhs(X, a, k);
@@ -218,13 +218,13 @@ __device__ __forceinline__ void rk4_step(double* X, const double* a, const StepC
 template <>
 struct System<SYS_ROSSLER_RK4> {
     static constexpr int NX = 3, NA = 3;
-    __device__ __forceinline__ static void rhs(const double* x, const double* a, double* f)
+    __host__ __device__ __forceinline__ static void rhs(const double* x, const double* a, double* f)
     {
         f[0] = -(x[1] + x[2]);
         f[1] = fma(a[0], x[1], x[0]);
         f[2] = fma(x[2], x[0] - a[2], a[1]);
     }
-    __device__ __forceinline__ static void step(double* X, const double* a, const StepConst& c)
+    __host__ __device__ __forceinline__ static void step(double* X, const double* a, const StepConst& c)
     {
         rk4_step<System>(X, a, c);
     }
@@ -234,13 +234,13 @@ struct System<SYS_ROSSLER_RK4> {
 template <>
 struct System<SYS_LORENZ_RK4> {
     static constexpr int NX = 3, NA = 3;
-    __device__ __forceinline__ static void rhs(const double* x, const double* a, double* f)
+    __host__ __device__ __forceinline__ static void rhs(const double* x, const double* a, double* f)
     {
         f[0] = a[0] * (x[1] - x[0]);
         f[1] = fma(x[0], a[1] - x[2], -x[1]);
         f[2] = fma(x[0], x[1], -(a[2] * x[2]));
     }
-    __device__ __forceinline__ static void step(double* X, const double* a, const StepConst& c)
+    __host__ __device__ __forceinline__ static void step(double* X, const double* a, const StepConst& c)
     {
         rk4_step<System>(X, a, c);
     }
@@ -259,7 +259,7 @@ __host__ __device__ inline double grid_value(int k, int n, double start, double 
 // Parameter vector with up to two swept entries, written without dynamic
 // register indexing (idxA/idxB are launch-uniform).
 template <int NA>
-__device__ __forceinline__ void build_params(double* a, const double* values, int idxA, double va,
+__host__ __device__ __forceinline__ void build_params(double* a, const double* values, int idxA, double va,
                                              int idxB, double vb)
 {
 #pragma unroll
@@ -280,7 +280,7 @@ __host__ __device__ inline unsigned long long splitmix64(unsigned long long& sta
 // (exact 53-bit uniforms, 1e-6 < r2 <= 1, at most 64 tries), then division
 // by sqrt(r2).  Replaces curand XORWOW at LLE/src/LLECUDA.cu:88-101.
 template <int NX>
-__device__ inline void unit_vector(unsigned long long seed, long long point, double* v)
+__host__ __device__ inline void unit_vector(unsigned long long seed, long long point, double* v)
 {
     unsigned long long st = seed ^ ((unsigned long long)(point + 1) * 0xD1B54A32D192ED03ULL);
     for (int tr = 0; tr < 64; ++tr) {

@@ -408,6 +408,8 @@ typedef struct chaos_engine_params {
     int max_tiles;          /* tile queue capacity per launch                 */
     int use_graph;          /* replay the launch sequence from a hipGraph     */
     double hbm_fraction;    /* share of free HBM admitted jobs may occupy     */
+    int points_per_tile;    /* points one wave takes per tile: 64, 128 or 256 */
+    int reserved;
 } chaos_engine_params;
 
 typedef struct chaos_job_info {
@@ -497,6 +499,11 @@ typedef struct chaos_stream_report {
 } chaos_stream_report;
 
 int chaos_stream_config_default(chaos_stream_config* cfg);
+/* The jobs of the stream cfg describes, in arrival order (what
+ * chaos_stream_run serves): jobs[k] and arrival_ms[k] for k < cap (either may
+ * be NULL); *count receives the number of jobs in the stream. */
+int chaos_stream_jobs(const chaos_stream_config* cfg, chaos_job* jobs, double* arrival_ms, int cap,
+                      int* count);
 int chaos_stream_run(int device, const chaos_stream_config* cfg,
                      const chaos_engine_params* p, chaos_stream_report* rep,
                      double* latencies_ms /* [jobs] or NULL */, int latencies_cap);

@@ -180,8 +180,11 @@ def test_bifurcation2D_file(chaos, tmp_path):
 
 
 def test_bif2d_full_config_sampled(chaos, oracle):
-    """configs: Lorenz peak-count map 1024^2, 500 + 500 t.u.; 16 sampled
-    points recomputed by the oracle."""
+    """configs: Lorenz peak-count map 1024^2, 500 + 500 t.u.; 16 whole rows
+    (16,384 points) recomputed by the oracle and compared point for point:
+    the agreement fraction is reported against the 99.5 % tolerance for
+    peak-count maps, and the map must in fact be equal on every point
+    (same arithmetic on both sides)."""
     n = 1024
     ranges = [20.0, 200.0, 0.5, 4.0]
     print("bif2d lorenz 1024x1024 full horizon", flush=True)
@@ -190,9 +193,15 @@ def test_bif2d_full_config_sampled(chaos, oracle):
                       1e-2, system="lorenz_rk4", semantics=1, stats=st)
     print(f"kernel {st.kernel_ms:.1f} ms analysis {st.analysis_ms:.1f} ms peaks {st.peaks}", flush=True)
     rng = np.random.default_rng(9)
-    for _ in range(16):
-        i, j = (int(v) for v in rng.integers(0, n, 2))
+    rows = sorted(set(int(v) for v in rng.choice(n, 16, replace=False)) | {0, n - 1})
+    same = total = 0
+    for j in rows:
         want = oracle.bif2d(oracle.SYS["lorenz_rk4"], 1, 500.0, n, 0.01, [1.0, 1.0, 1.0], ranges, [1, 2], 0, 0.0,
                             500.0, [10.0, 28.0, 8 / 3], 1, 1e-2, row_begin=j, row_end=j + 1)
-        assert got[j, i] == want[0, i], (i, j)
+        same += int((got[j] == want[0]).sum())
+        total += n
+    frac = same / total
+    print(f"peak-count map: {same} of {total} points equal the oracle ({100 * frac:.3f} %)", flush=True)
+    assert frac >= 0.995
+    assert same == total
     assert (got > 0).mean() > 0.9

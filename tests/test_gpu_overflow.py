@@ -1,0 +1,81 @@
+"""The peak store's overflow path (chaoslibrary_amd/csrc/bifurcation.hip:
+first guess max(256, B/32) peaks per point, overflow flagged on the device,
+sweep rerun with the exact per-wave need; the engine reruns the job one-shot).
+
+Lorenz RK4 at h = 0.05 peaks about once every 15 samples, far above the
+first guess of one per 32, so the first sweep must overflow.  Results are
+compared with the oracle (bit-exact) and, for the engine, with the one-shot
+entry point; the engine reports the rerun in chaos_job_info.reran."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+LOR = dict(ic=[1.0, 1.0, 1.0], values=[10.0, 28.0, 8 / 3])
+H = 0.05
+T_MAX, T_T = 600.0, 10.0
+B = int(T_MAX / H)
+
+
+def _first_guess_overflows(stored, n):
+    """The first pool holds ceil(n * max_pp / 192) + waves chunks of 256
+    entries and every chunk a wave fills holds at most 256 of its peaks, so
+    more than 256 * chunks stored peaks cannot have fitted."""
+    max_pp = min(B / 2 + 1, max(256.0, B / 32))
+    chunks = int(np.ceil(n * max_pp / 192)) + (n + 63) // 64
+    return stored > 256 * chunks
+
+
+@pytest.mark.parametrize("sem", [0, 1])
+def test_bif1d_overflow_rerun_matches_oracle(chaos, oracle, sem):
+    n = 96
+    print(f"bif1d overflow sem={sem}", flush=True)
+    st = chaos.Stats()
+    got = chaos.bif1d(T_MAX, n, H, LOR["ic"], [24.0, 60.0], [1], 0, 0.0, T_T, LOR["values"], 1,
+                      system="lorenz_rk4", semantics=sem, stats=st)
+    want = oracle.bif1d(oracle.SYS["lorenz_rk4"], sem, T_MAX, n, H, LOR["ic"], [24.0, 60.0], [1], 0, 0.0, T_T,
+                        LOR["values"], 1)
+    assert _first_guess_overflows(st.peaks, n), st.peaks / n
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w)
+
+
+def test_bif2d_overflow_rerun_matches_oracle(chaos, oracle):
+    n = 8
+    ranges = [40.0, 80.0, 2.0, 3.0]
+    print("bif2d overflow", flush=True)
+    st = chaos.Stats()
+    got = chaos.bif2d(T_MAX, n, H, LOR["ic"], ranges, [1, 2], 0, 0.0, T_T, LOR["values"], 1, 0.01,
+                      system="lorenz_rk4", semantics=1, stats=st)
+    want = oracle.bif2d(oracle.SYS["lorenz_rk4"], 1, T_MAX, n, H, LOR["ic"], ranges, [1, 2], 0, 0.0, T_T,
+                        LOR["values"], 1, 0.01)
+    assert _first_guess_overflows(st.peaks, n * n), st.peaks / (n * n)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("kind", ["bif1d", "bif2d"])
+def test_engine_overflow_reruns_job(chaos, oracle, kind):
+    J = chaos.make_job
+    if kind == "bif1d":
+        job = J(chaos.JOB_BIF1D, "lorenz_rk4", 1, 96, LOR["ic"], LOR["values"], [24.0, 60.0], [1], h=H,
+                tMax=T_MAX, transientTime=T_T)
+    else:
+        job = J(chaos.JOB_BIF2D, "lorenz_rk4", 1, 8, LOR["ic"], LOR["values"], [40.0, 80.0, 2.0, 3.0], [1, 2],
+                h=H, tMax=T_MAX, transientTime=T_T, eps=0.01)
+    print(f"engine overflow {kind}", flush=True)
+    srv = chaos.Server(params=chaos.engine_params(steps_per_launch=4000))
+    jid = srv.submit(job)
+    srv.drain()
+    info = srv.info(jid)
+    assert info.status == 2 and info.reran == 1
+    got = srv.result(jid, job)
+    if kind == "bif1d":
+        want = oracle.bif1d(oracle.SYS["lorenz_rk4"], 1, T_MAX, 96, H, LOR["ic"], [24.0, 60.0], [1], 0, 0.0, T_T,
+                            LOR["values"], 1)
+        for g, w in zip(got, want):
+            assert np.array_equal(g, w)
+    else:
+        want = oracle.bif2d(oracle.SYS["lorenz_rk4"], 1, T_MAX, 8, H, LOR["ic"], [40.0, 80.0, 2.0, 3.0], [1, 2], 0,
+                            0.0, T_T, LOR["values"], 1, 0.01)
+        assert np.array_equal(got, want)
+    srv.close()
