@@ -67,14 +67,20 @@ void basins_cluster(CallCtx& ctx, bool ref, long long n, const double* f0, const
         DbscanParams prm{2, eps, 1, true};
         dbscan_device(ctx, pts, flags, n, prm, labels);
     } else {
-        // z-scored into a separate buffer; the caller keeps the raw features
-        DevBuf<double> z(3 * (size_t)n);
+        // z-scored into a separate buffer (the caller keeps the raw
+        // features); the z-score leaves the clustering's bounding box behind,
+        // and its scratch shares that buffer (no synchronisation between)
+        const size_t zb = zscore_scratch_bytes(n, 3);
+        DevBuf<char> buf(align256(3 * (size_t)n * sizeof(double)) + zb + 256);
+        double* z = (double*)buf.get();
+        unsigned long long* box = (unsigned long long*)(buf.get() + align256(3 * (size_t)n * sizeof(double)));
+        void* zs = buf.get() + align256(3 * (size_t)n * sizeof(double)) + 256;
         const double* src[3] = {f0, f1, f2};
-        double* f[3] = {z.get(), z.get() + n, z.get() + 2 * n};
-        zscore_device(ctx, src, f, 3, flags, n, nullptr);
+        double* f[3] = {z, z + n, z + 2 * n};
+        zscore_device(ctx, src, f, 3, flags, n, nullptr, box, zs);
         const double* pts[3] = {f[0], f[1], f[2]};
         DbscanParams prm{3, eps, minPts, false};
-        dbscan_device(ctx, pts, flags, n, prm, labels);
+        dbscan_device(ctx, pts, flags, n, prm, labels, nullptr, 0, box);
     }
 }
 

@@ -1,60 +1,151 @@
 // dbscan.hip -- grid DBSCAN for the Basins labels and the deterministic
-// z-score reduction.
+// z-score that feeds it.
 //
 // Replaces the reference's host-driven single linkage
 // (Basins/src/basinsCUDA.cu:35-179, 866-923: one kernel launch and host sync
 // per visited point, O(N) launches and O(N^2) work) with a fixed sequence of
-// launches:
-//   bbox -> hash cells of side eps/sqrt(dim) -> per-cell point lists
-//   per-cell exact bounding box of the member points: a neighbour cell whose
-//   box lies wholly beyond eps of a point is skipped, one wholly within eps is
-//   taken as a whole, only the rest are scanned point by point
+// launches and one synchronisation at the end:
+//   bounding box -> hash cells of side eps/sqrt(dim): per-cell count, smallest
+//   member index and exact bounding box of the members, aggregated per block
+//   in LDS first (a basin that converges to a fixed point puts millions of
+//   points into one cell; per-point atomics on it would serialise)
+//   per-cell point lists (block-reserved ranges of each cell's segment)
 //   core test: a cell holding >= minPts points is all-core (its diagonal is
 //   below eps); every other point is tested by one wavefront that sums whole
-//   neighbour cells and scans the undecided ones 64 points at a time
+//   neighbour cells (exact boxes within eps) and scans the undecided ones 64
+//   points at a time
 //   union-find: cell members to the cell's smallest core index, then one
 //   witness pair per neighbouring cell pair, searched by one wavefront per
-//   cell in 64 x 64 tiles, nearest neighbour rings first so that most far
-//   pairs are already connected; links always go larger -> smaller index, so every
-//   root is its component's smallest core index
+//   cell in 64 x 64 tiles, nearest neighbour rings first; links always go
+//   larger -> smaller index, so every root is its component's smallest core
+//   index
 //   border points take their smallest-index core neighbour's root
-//   numbering by mark + exclusive scan over point indices.
-// Crowded cells (a basin that converges to a fixed point puts millions of
-// points into one cell) would serialise the per-point atomics on a few
-// addresses: lanes of a wavefront that share a cell combine their update and
-// one lane issues it.
+//   numbering: one bit per root in 64-point words, popcount prefix sums.
+// All scratch lives in one workspace (dbscan_ws_bytes), so the same kernels
+// serve the allocating entry point and chaos_dbscan_noalloc.
 #include <climits>
 #include <cmath>
-#include <vector>
+#include <mutex>
 
 #include "dbscan.hpp"
-#include "scan.hpp"
 
 namespace chaos {
 
 constexpr unsigned long long kEmpty = ~0ull;
 
+enum DbFlags : int {
+    DB_FULL = 1,       // cell table more than half full (redo with the larger table)
+    DB_RANGE = 2,      // feature range too wide for eps (cell coordinates overflow the key)
+    DB_NONFINITE = 4   // non-finite coordinate among the included points
+};
+
+// Device control block at the start of the workspace.
+struct DbCtl {
+    unsigned long long box[6];  // encoded min x, y, z, max x, y, z of the included points
+    double lo[3];               // decoded minimum: the grid origin
+    int ncells;                 // distinct cells claimed
+    int ncand;                  // points of sparse cells (core candidates)
+    int ncore_cells;            // cells holding a core point
+    int flags;                  // DbFlags
+    int K;                      // positive clusters
+    int any;                    // some point is included
+};
+
 struct GridDev {
     const double* x[3];
+    const int* include;  // NULL: every point included
     int dim;
-    double lo[3];
-    double s;                  // cell side
+    int sign;            // fixed-point sign rule
+    double s;            // cell side
     double eps;
-    double prune2;             // (eps * (1 + 1e-7))^2 for conservative box pruning
-    unsigned long long hmask;  // hash capacity - 1
-    unsigned long long* hkey;
-    unsigned long long* hcount;
-    long long* hstart;
-    int* hcur;
-    int* hrep;      // smallest core index in the cell (INT_MAX if none)
-    unsigned long long* bbox;  // [6][H]: per-cell min x,y,z then max x,y,z (order-preserving encoding)
+    double prune2;       // (eps * (1 + 1e-7))^2 for conservative box pruning
+    long long n, nw;     // points, 64-point words
+    unsigned long long hmask;
     long long H;
-    int* pslot;     // slot of each point's cell
-    int* cellpts;   // points grouped by cell
-    const int* include;
-    int* core;
-    int* parent;
+    // cell tables [H]
+    unsigned long long* hkey;
+    unsigned* hcount;
+    int* hmin;     // smallest member index
+    int* hstart;   // segment of the cell in cellpts
+    int* hcur;     // fill cursor of the segment
+    int* hrep;     // smallest core index (INT_MAX if none)
+    unsigned long long* bbox;  // [6][H]: member min x, y, z, max x, y, z (order-preserving encoding)
+    int* cells;    // compacted cells holding a core point
+    // point tables [n]
+    int* pslot;    // cell slot, -1 when excluded
+    int* cellpts;  // points grouped by cell
+    int* parent;   // union-find parent of core points, -1 for non-core points
+    int* labroot;  // component root of the point's label, -1 for none
+    int* cand;     // points of sparse cells
+    int* minfix;   // sign rule: smallest fixed-point core index per root
+    // numbering [nw]
+    unsigned long long* bitsP;  // roots of positive clusters
+    unsigned long long* bitsN;  // smallest fixed point of each negative cluster
+    int* rankP;
+    int* rankN;
+    int* scan_tmp;
+    DbCtl* ctl;
 };
+
+// ---------------------------------------------------------------------------
+// workspace layout
+// ---------------------------------------------------------------------------
+
+static_assert(sizeof(DbCtl) <= 256, "control block fits its 256-byte slot");
+
+static void carve(GridDev& G, char* base, long long n, long long H, bool sign)
+{
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char* p = base + off;
+        off += align256(bytes);
+        return p;
+    };
+    const long long nw = (n + 63) / 64;
+    G.ctl = (DbCtl*)take(256);
+    G.hkey = (unsigned long long*)take(8 * (size_t)H);
+    G.hcount = (unsigned*)take(4 * (size_t)H);
+    G.hmin = (int*)take(4 * (size_t)H);
+    G.hstart = (int*)take(4 * (size_t)H);
+    G.hcur = (int*)take(4 * (size_t)H);
+    G.hrep = (int*)take(4 * (size_t)H);
+    G.bbox = (unsigned long long*)take(48 * (size_t)H);
+    G.cells = (int*)take(4 * (size_t)H);
+    G.pslot = (int*)take(4 * (size_t)n);
+    G.cellpts = (int*)take(4 * (size_t)n);
+    G.parent = (int*)take(4 * (size_t)n);
+    G.labroot = (int*)take(4 * (size_t)n);
+    G.cand = (int*)take(4 * (size_t)n);
+    G.minfix = sign ? (int*)take(4 * (size_t)n) : nullptr;
+    G.bitsP = (unsigned long long*)take(8 * (size_t)nw);
+    G.rankP = (int*)take(4 * (size_t)nw);
+    G.bitsN = sign ? (unsigned long long*)take(8 * (size_t)nw) : nullptr;
+    G.rankN = sign ? (int*)take(4 * (size_t)nw) : nullptr;
+    G.scan_tmp = (int*)take(4 * scan_tmp_ints(H > nw ? H : nw));
+    G.H = H;
+    G.hmask = (unsigned long long)H - 1;
+    G.n = n;
+    G.nw = nw;
+}
+
+size_t dbscan_ws_bytes(long long n, long long H, bool sign_rule) { return dbscan_ws_bytes_c(n, H, sign_rule); }
+
+long long dbscan_max_cells(long long n)
+{
+    long long H = 1024;
+    while (H < 2 * n) H <<= 1;
+    return H;
+}
+
+long long dbscan_first_cells(long long n)
+{
+    const long long H = dbscan_max_cells(n);
+    return H < (1LL << 18) ? H : (1LL << 18);
+}
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
 
 __device__ __forceinline__ unsigned long long mix64(unsigned long long z)
 {
@@ -63,39 +154,85 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long z)
     return z ^ (z >> 31);
 }
 
+__device__ __forceinline__ bool included(const GridDev& G, long long p)
+{
+    return G.include == nullptr || G.include[p] != 0;
+}
+
 __device__ __forceinline__ unsigned long long pack_cell(int dim, const long long* c)
 {
     if (dim == 3) return (unsigned long long)c[0] | ((unsigned long long)c[1] << 21) | ((unsigned long long)c[2] << 42);
     return (unsigned long long)c[0] | ((unsigned long long)c[1] << 31);
 }
 
+__device__ __forceinline__ void unpack_cell(int dim, unsigned long long key, long long* c)
+{
+    if (dim == 3) {
+        c[0] = (long long)(key & 0x1FFFFFull);
+        c[1] = (long long)((key >> 21) & 0x1FFFFFull);
+        c[2] = (long long)(key >> 42);
+    } else {
+        c[0] = (long long)(key & 0x7FFFFFFFull);
+        c[1] = (long long)(key >> 31);
+        c[2] = 0;
+    }
+}
+
 __device__ __forceinline__ void cell_coords(const GridDev& G, long long p, long long* c)
 {
-    for (int d = 0; d < G.dim; ++d) c[d] = (long long)floor((G.x[d][p] - G.lo[d]) / G.s);
+    for (int d = 0; d < G.dim; ++d) c[d] = (long long)floor((G.x[d][p] - G.ctl->lo[d]) / G.s);
+}
+
+// Order-preserving map double -> u64 (for atomicMin/atomicMax on doubles).
+__host__ __device__ __forceinline__ unsigned long long ord_enc(double v)
+{
+    unsigned long long u;
+    memcpy(&u, &v, 8);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+__host__ __device__ __forceinline__ double ord_dec(unsigned long long e)
+{
+    const unsigned long long u = (e >> 63) ? (e & 0x7fffffffffffffffull) : ~e;
+    double v;
+    memcpy(&v, &u, 8);
+    return v;
 }
 
 __device__ __forceinline__ int lookup(const GridDev& G, unsigned long long key)
 {
     unsigned long long h = mix64(key) & G.hmask;
-    while (true) {
+    for (long long probe = 0; probe < G.H; ++probe) {
         const unsigned long long k = G.hkey[h];
         if (k == key) return (int)h;
         if (k == kEmpty) return -1;
         h = (h + 1) & G.hmask;
     }
+    return -1;
 }
 
-// Order-preserving map double -> u64 (for atomicMin/atomicMax on doubles).
-__device__ __forceinline__ unsigned long long ord_enc(double v)
+// Slot of `key`, inserting it if absent.  A non-empty slot never changes, so
+// a plain load that sees a key is final; an empty one is claimed by
+// compare-and-swap.  A table more than half full is flagged (the host redoes
+// the clustering with the larger table); a full one returns -1.
+__device__ __forceinline__ int hash_insert(const GridDev& G, unsigned long long key)
 {
-    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
-}
-
-__device__ __forceinline__ double ord_dec(unsigned long long e)
-{
-    const unsigned long long u = (e >> 63) ? (e & 0x7fffffffffffffffull) : ~e;
-    return __longlong_as_double((long long)u);
+    unsigned long long h = mix64(key) & G.hmask;
+    for (long long probe = 0; probe < G.H; ++probe) {
+        const unsigned long long k = G.hkey[h];
+        if (k == key) return (int)h;
+        if (k == kEmpty) {
+            const unsigned long long prev = atomicCAS(&G.hkey[h], kEmpty, key);
+            if (prev == kEmpty) {
+                if ((long long)atomicAdd(&G.ctl->ncells, 1) + 1 > (G.H >> 1)) atomicOr(&G.ctl->flags, DB_FULL);
+                return (int)h;
+            }
+            if (prev == key) return (int)h;
+        }
+        h = (h + 1) & G.hmask;
+    }
+    atomicOr(&G.ctl->flags, DB_FULL);
+    return -1;
 }
 
 __device__ __forceinline__ bool within(const GridDev& G, long long p, long long q)
@@ -113,7 +250,7 @@ __device__ __forceinline__ double box_gap2(const GridDev& G, long long p, const 
 {
     double g2 = 0.0;
     for (int d = 0; d < G.dim; ++d) {
-        const double lo = G.lo[d] + (double)c[d] * G.s;
+        const double lo = G.ctl->lo[d] + (double)c[d] * G.s;
         const double hi = lo + G.s;
         const double v = G.x[d][p];
         const double g = v < lo ? lo - v : (v > hi ? v - hi : 0.0);
@@ -145,7 +282,7 @@ __device__ __forceinline__ int box_relation(const GridDev& G, long long p, int s
 }
 
 // Visit every cell of the 5^dim block around p's cell that may hold a point
-// within eps: f(slot, coords).  Stops early when f returns false.
+// within eps: f(slot).  Stops early when f returns false.
 template <class F>
 __device__ __forceinline__ void for_neighbour_cells(const GridDev& G, long long p, F&& f)
 {
@@ -158,12 +295,12 @@ __device__ __forceinline__ void for_neighbour_cells(const GridDev& G, long long 
                 c[0] = c0[0] + dx;
                 c[1] = c0[1] + dy;
                 c[2] = c0[2] + dz;
-                bool ok = c[0] >= 0 && c[1] >= 0 && (G.dim == 2 || c[2] >= 0);
+                const bool ok = c[0] >= 0 && c[1] >= 0 && (G.dim == 2 || c[2] >= 0);
                 if (!ok) continue;
                 if (box_gap2(G, p, c) > G.prune2) continue;
                 const int slot = lookup(G, pack_cell(G.dim, c));
                 if (slot < 0) continue;
-                if (!f(slot, c)) return;
+                if (!f(slot)) return;
             }
 }
 
@@ -193,102 +330,7 @@ __device__ __forceinline__ void uf_union(const GridDev& G, int a, int b)
     }
 }
 
-__global__ void bbox_kernel(const GridDev G, long long n, double* out)
-{
-    __shared__ double smin[3][256], smax[3][256];
-    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long long)gridDim.x * blockDim.x) {
-        if (!G.include[p]) continue;
-        for (int d = 0; d < G.dim; ++d) {
-            mn[d] = fmin(mn[d], G.x[d][p]);
-            mx[d] = fmax(mx[d], G.x[d][p]);
-        }
-    }
-    for (int d = 0; d < 3; ++d) {
-        smin[d][threadIdx.x] = mn[d];
-        smax[d][threadIdx.x] = mx[d];
-    }
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o)
-            for (int d = 0; d < 3; ++d) {
-                smin[d][threadIdx.x] = fmin(smin[d][threadIdx.x], smin[d][threadIdx.x + o]);
-                smax[d][threadIdx.x] = fmax(smax[d][threadIdx.x], smax[d][threadIdx.x + o]);
-            }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-        for (int d = 0; d < 3; ++d) {
-            out[blockIdx.x * 6 + d] = smin[d][0];
-            out[blockIdx.x * 6 + 3 + d] = smax[d][0];
-        }
-}
-
-// Slot of `key` in the open-addressing table, inserting it if absent.  A
-// non-empty slot never changes, so a plain load that sees a key is final; an
-// empty one is claimed by compare-and-swap.
-__device__ __forceinline__ int hash_insert(const GridDev& G, unsigned long long key)
-{
-    unsigned long long h = mix64(key) & G.hmask;
-    while (true) {
-        const unsigned long long k = G.hkey[h];
-        if (k == key) return (int)h;
-        if (k == kEmpty) {
-            const unsigned long long prev = atomicCAS(&G.hkey[h], kEmpty, key);
-            if (prev == kEmpty || prev == key) return (int)h;
-        }
-        h = (h + 1) & G.hmask;
-    }
-}
-
-// Wavefront groups: up to kAggGroups rounds, each taking the lowest pending
-// lane as leader and every pending lane with the leader's key as its group.
-// Lanes left pending afterwards (many distinct keys) update on their own.
-constexpr int kAggGroups = 4;
-
-// Grid of the wavefront-per-item kernels (blocks of 256): 8192 waves, a full
-// complement for 256 CUs; the waves stride over the device-side item count.
-constexpr unsigned kWaveBlocks = 2048;
-
-__device__ __forceinline__ unsigned long long lanes_below() { return (1ull << __lane_id()) - 1ull; }
-
-__global__ void insert_kernel(const GridDev G, long long n)
-{
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = p < n && G.include[p];
-    unsigned long long key = kEmpty;
-    if (valid) {
-        long long c[3] = {0, 0, 0};
-        cell_coords(G, p, c);
-        key = pack_cell(G.dim, c);
-    }
-    const int lane = __lane_id();
-    bool pending = valid;
-    int slot = -1;
-    for (int it = 0; it < kAggGroups; ++it) {
-        const unsigned long long act = __ballot(pending);
-        if (!act) break;
-        const int leader = __builtin_ctzll(act);
-        const unsigned long long lk = __shfl(key, leader);
-        const bool mine = pending && key == lk;
-        const unsigned long long grp = __ballot(mine);
-        int h = 0;
-        if (lane == leader) {
-            h = hash_insert(G, lk);
-            atomicAdd(&G.hcount[h], (unsigned long long)__popcll(grp));
-        }
-        h = __shfl(h, leader);
-        if (mine) {
-            slot = h;
-            pending = false;
-        }
-    }
-    if (pending) {
-        slot = hash_insert(G, key);
-        atomicAdd(&G.hcount[slot], 1ull);
-    }
-    if (p < n) G.pslot[p] = valid ? slot : -1;
-}
+__device__ __forceinline__ bool is_core(const GridDev& G, int p) { return uf_load(&G.parent[p]) >= 0; }
 
 __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
 {
@@ -308,86 +350,382 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
     return v;
 }
 
-// Places every included point in its cell's segment of cellpts and grows the
-// cell's bounding box.
-__global__ void fill_kernel(const GridDev G, long long n)
-{
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = p < n ? G.pslot[p] : -1;
-    const bool valid = s >= 0;
-    unsigned long long e[3] = {0, 0, 0};
-    if (valid)
-        for (int d = 0; d < G.dim; ++d) e[d] = ord_enc(G.x[d][p]);
-    const int lane = __lane_id();
-    bool pending = valid;
-    for (int it = 0; it < kAggGroups; ++it) {
-        const unsigned long long act = __ballot(pending);
-        if (!act) break;
-        const int leader = __builtin_ctzll(act);
-        const int ls = __shfl(s, leader);
-        const bool mine = pending && s == ls;
-        const unsigned long long grp = __ballot(mine);
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&G.hcur[ls], __popcll(grp));
-        base = __shfl(base, leader);
-        if (mine) G.cellpts[G.hstart[ls] + base + __popcll(grp & lanes_below())] = (int)p;
-        for (int d = 0; d < G.dim; ++d) {
-            const unsigned long long mn = wave_min_u64(mine ? e[d] : ~0ull);
-            const unsigned long long mx = wave_max_u64(mine ? e[d] : 0ull);
-            if (lane == leader) {
-                atomicMin(&G.bbox[d * G.H + ls], mn);
-                atomicMax(&G.bbox[(3 + d) * G.H + ls], mx);
-            }
-        }
-        if (mine) pending = false;
-    }
-    if (pending) {
-        G.cellpts[G.hstart[s] + atomicAdd(&G.hcur[s], 1)] = (int)p;
-        for (int d = 0; d < G.dim; ++d) {
-            atomicMin(&G.bbox[d * G.H + s], e[d]);
-            atomicMax(&G.bbox[(3 + d) * G.H + s], e[d]);
-        }
-    }
-}
-
-// Core test, dense part: a cell holding >= minPts points is all-core (its
-// diagonal is below eps).  Points of sparser cells go to a candidate list for
-// the wavefront-per-point test.
-__global__ void core_dense_kernel(const GridDev G, long long n, int minPts, int* cand, int* ncand)
-{
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = p < n ? G.pslot[p] : -1;
-    const bool dense = s >= 0 && (long long)G.hcount[s] >= minPts;
-    const bool sparse = s >= 0 && !dense;
-    if (p < n) {
-        G.core[p] = dense ? 1 : 0;
-        G.parent[p] = dense ? (int)p : -1;
-    }
-    const int lane = __lane_id();
-    const unsigned long long sb = __ballot(sparse);
-    int base = 0;
-    if (lane == 0 && sb) base = atomicAdd(ncand, __popcll(sb));
-    base = __shfl(base, 0);
-    if (sparse) cand[base + __popcll(sb & lanes_below())] = (int)p;
-    // hrep = smallest core index of the cell.  Indices rise with the lane, so
-    // a group's leader holds the group's smallest index.
-    bool pending = dense;
-    for (int it = 0; it < kAggGroups; ++it) {
-        const unsigned long long act = __ballot(pending);
-        if (!act) break;
-        const int leader = __builtin_ctzll(act);
-        const int ls = __shfl(s, leader);
-        const bool mine = pending && s == ls;
-        if (lane == leader) atomicMin(&G.hrep[ls], (int)p);
-        if (mine) pending = false;
-    }
-    if (pending) atomicMin(&G.hrep[s], (int)p);
-}
-
 __device__ __forceinline__ long long wave_sum_ll(long long v)
 {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
+}
+
+__device__ __forceinline__ unsigned long long lanes_below() { return (1ull << __lane_id()) - 1ull; }
+
+// Wavefront groups: up to kAggGroups rounds, each taking the lowest pending
+// lane as leader and every pending lane with the leader's key as its group.
+// Lanes left pending afterwards (many distinct keys) update on their own.
+constexpr int kAggGroups = 4;
+
+// Per-block chunks of the aggregated kernels: 256 threads x kPer points,
+// consecutive point indices per pass, and an LDS table of kLds cells.
+constexpr int kPer = 8;
+constexpr int kChunkPts = 256 * kPer;
+constexpr int kLds = 512;
+
+// Grid of the wavefront-per-item kernels (blocks of 256): 8192 waves, a full
+// complement for 256 CUs; the waves stride over the device-side item count.
+constexpr unsigned kWaveBlocks = 2048;
+
+// Block-wide append of the lanes with `flag` to list[*counter ...]: one
+// global atomic per block.  All threads of the block must call it.
+__device__ __forceinline__ void block_append(bool flag, int value, int* list, int* counter)
+{
+    __shared__ int s_wave[4];
+    __shared__ int s_base;
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    const unsigned long long m = __ballot(flag);
+    if (lane == 0) s_wave[wave] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int tot = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+        s_base = tot ? atomicAdd(counter, tot) : 0;
+    }
+    __syncthreads();
+    int off = s_base;
+    for (int w = 0; w < wave; ++w) off += s_wave[w];
+    if (flag) list[off + __popcll(m & lanes_below())] = value;
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+
+// Cell tables to empty; with `all`, also the control block (box from `box`
+// or empty), the fixed-point minima and the negative-root bits.
+__global__ void init_kernel(const GridDev G, const unsigned long long* box, int all)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = t; i < G.H; i += stride) {
+        G.hkey[i] = kEmpty;
+        G.hcount[i] = 0u;
+        G.hmin[i] = INT_MAX;
+        G.hcur[i] = 0;
+        for (int d = 0; d < 3; ++d) {
+            G.bbox[d * G.H + i] = ~0ull;
+            G.bbox[(3 + d) * G.H + i] = 0ull;
+        }
+    }
+    if (!all) {
+        if (t == 0) {
+            G.ctl->ncells = 0;
+            G.ctl->ncand = 0;
+            G.ctl->ncore_cells = 0;
+            G.ctl->flags = 0;
+            G.ctl->K = 0;
+        }
+        return;
+    }
+    if (G.sign) {
+        for (long long i = t; i < G.n; i += stride) G.minfix[i] = INT_MAX;
+        for (long long i = t; i < G.nw; i += stride) G.bitsN[i] = 0ull;
+    }
+    if (t == 0) {
+        DbCtl c{};
+        for (int d = 0; d < 3; ++d) {
+            c.box[d] = box ? box[d] : ~0ull;
+            c.box[3 + d] = box ? box[3 + d] : 0ull;
+        }
+        *G.ctl = c;
+    }
+}
+
+// Bounding box of the included points (encoded, atomic min / max per block).
+__global__ __launch_bounds__(256) void bbox_kernel(const GridDev G)
+{
+    __shared__ unsigned long long smin[3][4], smax[3][4];
+    unsigned long long mn[3] = {~0ull, ~0ull, ~0ull}, mx[3] = {0ull, 0ull, 0ull};
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < G.n;
+         p += (long long)gridDim.x * blockDim.x) {
+        if (!included(G, p)) continue;
+        for (int d = 0; d < G.dim; ++d) {
+            const unsigned long long e = ord_enc(G.x[d][p]);
+            mn[d] = e < mn[d] ? e : mn[d];
+            mx[d] = e > mx[d] ? e : mx[d];
+        }
+    }
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    for (int d = 0; d < 3; ++d) {
+        mn[d] = wave_min_u64(mn[d]);
+        mx[d] = wave_max_u64(mx[d]);
+        if (lane == 0) {
+            smin[d][wave] = mn[d];
+            smax[d][wave] = mx[d];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int d = threadIdx.x;
+        unsigned long long a = smin[d][0], b = smax[d][0];
+        for (int w = 1; w < 4; ++w) {
+            a = smin[d][w] < a ? smin[d][w] : a;
+            b = smax[d][w] > b ? smax[d][w] : b;
+        }
+        if (d < G.dim && a != ~0ull) {
+            atomicMin(&G.ctl->box[d], a);
+            atomicMax(&G.ctl->box[3 + d], b);
+        }
+    }
+}
+
+// Grid origin from the box; range and finiteness checks.
+__global__ void setup_kernel(const GridDev G, double cell_lim)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    DbCtl* c = G.ctl;
+    c->any = c->box[0] != ~0ull;
+    for (int d = 0; d < 3; ++d) c->lo[d] = 0.0;
+    if (!c->any) return;
+    for (int d = 0; d < G.dim; ++d) {
+        const double lo = ord_dec(c->box[d]), hi = ord_dec(c->box[3 + d]);
+        if (!isfinite(lo) || !isfinite(hi)) {
+            c->flags |= DB_NONFINITE;
+            continue;
+        }
+        c->lo[d] = lo;
+        if (floor((hi - lo) / G.s) + 1 >= cell_lim) c->flags |= DB_RANGE;
+    }
+}
+
+// Slot of `key` in the block's LDS table, claiming it if absent; -1 if full.
+__device__ __forceinline__ int lds_claim(unsigned long long* s_key, unsigned long long key)
+{
+    int h = (int)(mix64(key) & (kLds - 1));
+    for (int probe = 0; probe < kLds; ++probe) {
+        const unsigned long long k = s_key[h];
+        if (k == key) return h;
+        if (k == kEmpty) {
+            const unsigned long long prev = atomicCAS(&s_key[h], kEmpty, key);
+            if (prev == kEmpty || prev == key) return h;
+        }
+        h = (h + 1) & (kLds - 1);
+    }
+    return -1;
+}
+
+// Cells of the included points: count, smallest member index and member box
+// per cell, aggregated over the wavefront, then over the block's chunk in
+// LDS, then added to the global table once per cell and block.
+__global__ __launch_bounds__(256) void insert_kernel(const GridDev G)
+{
+    __shared__ unsigned long long s_key[kLds];
+    __shared__ unsigned long long s_box[6][kLds];
+    __shared__ unsigned s_cnt[kLds];
+    __shared__ int s_min[kLds];
+    __shared__ int s_slot[kLds];
+    const int tid = threadIdx.x, lane = __lane_id();
+    for (int i = tid; i < kLds; i += 256) {
+        s_key[i] = kEmpty;
+        s_cnt[i] = 0u;
+        s_min[i] = INT_MAX;
+        for (int d = 0; d < 3; ++d) {
+            s_box[d][i] = ~0ull;
+            s_box[3 + d][i] = 0ull;
+        }
+    }
+    __syncthreads();
+    // per point: LDS entry (>= 0), a global slot s encoded as -2 - s, or -1
+    int loc[kPer];
+    const long long base = (long long)blockIdx.x * kChunkPts;
+    for (int k = 0; k < kPer; ++k) {
+        const long long p = base + (long long)k * 256 + tid;
+        const bool valid = p < G.n && included(G, p);
+        unsigned long long key = kEmpty, e[3] = {0ull, 0ull, 0ull};
+        if (valid) {
+            long long c[3] = {0, 0, 0};
+            cell_coords(G, p, c);
+            key = pack_cell(G.dim, c);
+            for (int d = 0; d < G.dim; ++d) e[d] = ord_enc(G.x[d][p]);
+        }
+        int mine_loc = -1;
+        bool pending = valid;
+        for (int it = 0; it <= kAggGroups; ++it) {
+            const unsigned long long act = __ballot(pending);
+            if (!act) break;
+            // the last round: every lane left updates on its own
+            const bool solo = it == kAggGroups;
+            const int leader = __builtin_ctzll(act);
+            const unsigned long long lk = __shfl(key, leader);
+            const bool mine = pending && (solo || key == lk);
+            const unsigned long long grp = __ballot(mine);
+            const bool act_lane = solo ? mine : lane == leader;
+            const unsigned cnt = solo ? 1u : (unsigned)__popcll(grp);
+            unsigned long long mn[3], mx[3];
+            for (int d = 0; d < 3; ++d) {
+                mn[d] = solo ? e[d] : wave_min_u64(mine ? e[d] : ~0ull);
+                mx[d] = solo ? e[d] : wave_max_u64(mine ? e[d] : 0ull);
+            }
+            int l = -1;
+            if (act_lane) {
+                const unsigned long long kk = solo ? key : lk;
+                // the leader is the group's lowest lane: its index is the group's smallest
+                const int pmin = (int)p;
+                l = lds_claim(s_key, kk);
+                if (l >= 0) {
+                    atomicAdd(&s_cnt[l], cnt);
+                    atomicMin(&s_min[l], pmin);
+                    for (int d = 0; d < G.dim; ++d) {
+                        atomicMin(&s_box[d][l], mn[d]);
+                        atomicMax(&s_box[3 + d][l], mx[d]);
+                    }
+                } else {
+                    const int slot = hash_insert(G, kk);
+                    if (slot >= 0) {
+                        atomicAdd(&G.hcount[slot], cnt);
+                        atomicMin(&G.hmin[slot], pmin);
+                        for (int d = 0; d < G.dim; ++d) {
+                            atomicMin(&G.bbox[d * G.H + slot], mn[d]);
+                            atomicMax(&G.bbox[(3 + d) * G.H + slot], mx[d]);
+                        }
+                    }
+                    l = -2 - slot;  // -1 when the table is full
+                }
+            }
+            if (!solo) l = __shfl(l, leader);
+            if (mine) {
+                mine_loc = l;
+                pending = false;
+            }
+        }
+        loc[k] = valid ? mine_loc : -1;
+    }
+    __syncthreads();
+    for (int i = tid; i < kLds; i += 256) {
+        const unsigned long long key = s_key[i];
+        int slot = -1;
+        if (key != kEmpty) {
+            slot = hash_insert(G, key);
+            if (slot >= 0) {
+                atomicAdd(&G.hcount[slot], s_cnt[i]);
+                atomicMin(&G.hmin[slot], s_min[i]);
+                for (int d = 0; d < G.dim; ++d) {
+                    atomicMin(&G.bbox[d * G.H + slot], s_box[d][i]);
+                    atomicMax(&G.bbox[(3 + d) * G.H + slot], s_box[3 + d][i]);
+                }
+            }
+        }
+        s_slot[i] = slot;
+    }
+    __syncthreads();
+    for (int k = 0; k < kPer; ++k) {
+        const long long p = base + (long long)k * 256 + tid;
+        if (p >= G.n) break;
+        const int l = loc[k];
+        G.pslot[p] = l >= 0 ? s_slot[l] : (l <= -2 ? -2 - l : -1);
+    }
+}
+
+// Slot of an int key in an LDS table of kLds entries (-1 = empty).
+__device__ __forceinline__ int lds_claim_int(int* s_key, int key)
+{
+    int h = (int)(mix64((unsigned long long)key) & (kLds - 1));
+    for (int probe = 0; probe < kLds; ++probe) {
+        const int k = s_key[h];
+        if (k == key) return h;
+        if (k == -1) {
+            const int prev = atomicCAS(&s_key[h], -1, key);
+            if (prev == -1 || prev == key) return h;
+        }
+        h = (h + 1) & (kLds - 1);
+    }
+    return -1;
+}
+
+// Points into their cells' segments of cellpts: ranks within the block's
+// chunk per cell in LDS, one global reservation per cell and block.
+__global__ __launch_bounds__(256) void place_kernel(const GridDev G)
+{
+    __shared__ int s_slot[kLds];
+    __shared__ int s_cnt[kLds];
+    __shared__ int s_base[kLds];
+    const int tid = threadIdx.x, lane = __lane_id();
+    for (int i = tid; i < kLds; i += 256) {
+        s_slot[i] = -1;
+        s_cnt[i] = 0;
+    }
+    __syncthreads();
+    int loc[kPer], rank[kPer];  // loc: LDS entry, -2 = placed directly, -1 = none
+    const long long base = (long long)blockIdx.x * kChunkPts;
+    for (int k = 0; k < kPer; ++k) {
+        const long long p = base + (long long)k * 256 + tid;
+        const int s = p < G.n ? G.pslot[p] : -1;
+        int my_loc = -1, my_rank = 0;
+        bool pending = s >= 0;
+        for (int it = 0; it <= kAggGroups; ++it) {
+            const unsigned long long act = __ballot(pending);
+            if (!act) break;
+            const bool solo = it == kAggGroups;
+            const int leader = __builtin_ctzll(act);
+            const int ls = __shfl(s, leader);
+            const bool mine = pending && (solo || s == ls);
+            const unsigned long long grp = __ballot(mine);
+            int l = -1, r0 = 0;
+            if (solo ? mine : lane == leader) {
+                const int cnt = solo ? 1 : __popcll(grp);
+                const int key = solo ? s : ls;
+                l = lds_claim_int(s_slot, key);
+                if (l >= 0) {
+                    r0 = atomicAdd(&s_cnt[l], cnt);
+                } else {
+                    r0 = G.hstart[key] + atomicAdd(&G.hcur[key], cnt);
+                    l = -2;
+                }
+            }
+            if (!solo) {
+                l = __shfl(l, leader);
+                r0 = __shfl(r0, leader);
+            }
+            if (mine) {
+                my_loc = l;
+                my_rank = r0 + (solo ? 0 : __popcll(grp & lanes_below()));
+                pending = false;
+            }
+        }
+        loc[k] = my_loc;
+        rank[k] = my_rank;
+    }
+    __syncthreads();
+    for (int i = tid; i < kLds; i += 256) {
+        const int s = s_slot[i];
+        s_base[i] = s >= 0 ? G.hstart[s] + atomicAdd(&G.hcur[s], s_cnt[i]) : 0;
+    }
+    __syncthreads();
+    for (int k = 0; k < kPer; ++k) {
+        const long long p = base + (long long)k * 256 + tid;
+        if (p >= G.n) break;
+        const int l = loc[k];
+        if (l >= 0) G.cellpts[s_base[l] + rank[k]] = (int)p;
+        else if (l == -2) G.cellpts[rank[k]] = (int)p;
+    }
+}
+
+// Smallest core index of the dense cells (all members core: their smallest
+// member); INT_MAX elsewhere until the sparse core test.
+__global__ void cell_init_kernel(const GridDev G, int minPts)
+{
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= G.H) return;
+    G.hrep[s] = (G.hkey[s] != kEmpty && (long long)G.hcount[s] >= minPts) ? G.hmin[s] : INT_MAX;
+}
+
+// Core test, dense part: a cell holding >= minPts points is all-core (its
+// diagonal is below eps) and its members link straight to its smallest
+// member.  Points of sparser cells go to the candidate list.
+__global__ __launch_bounds__(256) void core_dense_kernel(const GridDev G, int minPts)
+{
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = p < G.n ? G.pslot[p] : -1;
+    const bool dense = s >= 0 && (long long)G.hcount[s] >= minPts;
+    if (p < G.n) G.parent[p] = dense ? G.hmin[s] : -1;
+    block_append(s >= 0 && !dense, (int)p, G.cand, &G.ctl->ncand);
 }
 
 // Neighbour cell o (0 <= o < 5^dim) of cell c0: coordinates, false if any is
@@ -400,31 +738,18 @@ __device__ __forceinline__ bool neighbour_cell(const GridDev& G, const long long
     return c[0] >= 0 && c[1] >= 0 && c[2] >= 0;
 }
 
-__device__ __forceinline__ void unpack_cell(int dim, unsigned long long key, long long* c)
-{
-    if (dim == 3) {
-        c[0] = (long long)(key & 0x1FFFFFull);
-        c[1] = (long long)((key >> 21) & 0x1FFFFFull);
-        c[2] = (long long)(key >> 42);
-    } else {
-        c[0] = (long long)(key & 0x7FFFFFFFull);
-        c[1] = (long long)(key >> 31);
-        c[2] = 0;
-    }
-}
-
 // Core test, sparse part: one wavefront per candidate point.  The lanes look
 // up the 5^dim neighbour cells, add whole cells whose box lies within eps and
 // scan the undecided ones 64 points at a time, stopping at minPts.
-__global__ void core_sparse_kernel(const GridDev G, const int* cand, const int* ncand, int minPts)
+__global__ __launch_bounds__(256) void core_sparse_kernel(const GridDev G, int minPts)
 {
     const int lane = __lane_id();
     const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
-    const int nc = *ncand;
+    const int nc = G.ctl->ncand;
     const int noff = G.dim == 3 ? 125 : 25;
     for (long long i = wave; i < nc; i += nw) {
-        const int p = cand[i];
+        const int p = G.cand[i];
         long long c0[3] = {0, 0, 0};
         cell_coords(G, p, c0);
         int sl[2] = {-1, -1}, rl[2] = {0, 0};
@@ -456,31 +781,30 @@ __global__ void core_sparse_kernel(const GridDev G, const int* cand, const int* 
             }
         }
         if (cnt >= minPts && lane == 0) {
-            G.core[p] = 1;
             G.parent[p] = p;
             atomicMin(&G.hrep[G.pslot[p]], p);
         }
     }
 }
 
-__global__ void link_cell_kernel(const GridDev G, long long n)
+// Core points of sparse cells link to their cell's smallest core index (all
+// members of a cell lie within eps of each other).
+__global__ void link_sparse_kernel(const GridDev G)
 {
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n || !G.core[p]) return;
-    G.parent[p] = G.hrep[G.pslot[p]];  // the cell's smallest core index (all-pairs within eps)
+    const int nc = G.ctl->ncand;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nc;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int p = G.cand[i];
+        if (G.parent[p] >= 0) G.parent[p] = G.hrep[G.pslot[p]];
+    }
 }
 
 // Cells holding at least one core point, compacted.
-__global__ void cell_list_kernel(const GridDev G, int* cells, int* ncells)
+__global__ __launch_bounds__(256) void cell_list_kernel(const GridDev G)
 {
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool has = s < G.H && G.hkey[s] != kEmpty && G.hrep[s] != INT_MAX;
-    const int lane = __lane_id();
-    const unsigned long long m = __ballot(has);
-    int base = 0;
-    if (lane == 0 && m) base = atomicAdd(ncells, __popcll(m));
-    base = __shfl(base, 0);
-    if (has) cells[base + __popcll(m & lanes_below())] = (int)s;
+    block_append(has, (int)s, G.cells, &G.ctl->ncore_cells);
 }
 
 // May some point of cell a lie within eps of some point of cell b?  (Exact
@@ -510,7 +834,7 @@ __device__ void link_cell_pair(const GridDev& G, int a, long long a0, long long 
         double px[3] = {0.0, 0.0, 0.0};
         if (ka + lane < a1) {
             p = G.cellpts[ka + lane];
-            if (G.core[p]) rel = box_relation(G, p, b);
+            if (is_core(G, p)) rel = box_relation(G, p, b);
         }
         const unsigned long long m2 = __ballot(rel == 2);
         if (m2) {                                  // all of b within eps of p, rb among them
@@ -527,7 +851,7 @@ __device__ void link_cell_pair(const GridDev& G, int a, long long a0, long long 
             double qx[3] = {0.0, 0.0, 0.0};
             if (kb + lane < b1) {
                 q = G.cellpts[kb + lane];
-                qok = G.core[q] != 0 && box_relation(G, q, a) != 0;
+                qok = is_core(G, q) && box_relation(G, q, a) != 0;
                 if (qok)
                     for (int d = 0; d < G.dim; ++d) qx[d] = G.x[d][q];
             }
@@ -566,15 +890,15 @@ __device__ __forceinline__ int offset_ring(const GridDev& G, int o)
 // skipping pairs already connected or whose boxes are farther apart than eps.
 // The rings run as three launches, nearest first: in a dense region the face
 // links connect the cells, and the farther pairs then pass the root check.
-__global__ void link_pairs_kernel(const GridDev G, const int* cells, const int* ncells, int ring)
+__global__ __launch_bounds__(256) void link_pairs_kernel(const GridDev G, int ring)
 {
     const int lane = __lane_id();
     const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
-    const int nc = *ncells;
+    const int nc = G.ctl->ncore_cells;
     const int noff = G.dim == 3 ? 125 : 25;
     for (long long i = wave; i < nc; i += nw) {
-        const int a = cells[i];
+        const int a = G.cells[i];
         const unsigned long long ka = G.hkey[a];
         const int ra = G.hrep[a];
         long long c0[3];
@@ -606,299 +930,403 @@ __global__ void link_pairs_kernel(const GridDev G, const int* cells, const int* 
     }
 }
 
-__global__ void flatten_kernel(const GridDev G, long long n, int* root)
+// Label root of every point (core: its component root; border: the root of
+// its smallest-index core neighbour; else -1), the positive-root bit words
+// (one wavefront = one 64-point word) and, under the sign rule, each
+// component's smallest fixed-point core index.
+__global__ __launch_bounds__(256) void border_kernel(const GridDev G)
 {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    root[p] = G.core[p] ? uf_find(G, (int)p) : -1;
-}
-
-__global__ void border_kernel(const GridDev G, long long n, const int* root, int* labroot)
-{
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    if (G.pslot[p] < 0) {
-        labroot[p] = -1;
-        return;
-    }
-    if (G.core[p]) {
-        labroot[p] = root[p];
-        return;
-    }
-    int best = INT_MAX;
-    for_neighbour_cells(G, p, [&](int slot, const long long*) {
-        const int rel = box_relation(G, p, slot);
-        if (rel == 0) return true;
-        if (rel == 2) {                                // smallest core index of the cell
-            best = min(best, G.hrep[slot]);
-            return true;
+    int root = -1;
+    bool isroot = false;
+    if (p < G.n) {
+        const int s = G.pslot[p];
+        if (s >= 0) {
+            const int par = G.parent[p];
+            if (par >= 0) {
+                root = uf_find(G, (int)p);
+                isroot = par == (int)p;
+                if (G.sign && G.include && G.include[p] == -1) atomicMin(&G.minfix[root], (int)p);
+            } else {
+                int best = INT_MAX;
+                for_neighbour_cells(G, p, [&](int slot) {
+                    const int rel = box_relation(G, p, slot);
+                    if (rel == 0) return true;
+                    if (rel == 2) {  // smallest core index of the cell
+                        best = min(best, G.hrep[slot]);
+                        return true;
+                    }
+                    const long long b = G.hstart[slot], e = b + (long long)G.hcount[slot];
+                    for (long long k = b; k < e; ++k) {
+                        const int q = G.cellpts[k];
+                        if (q < best && G.parent[q] >= 0 && within(G, p, q)) best = q;
+                    }
+                    return true;
+                });
+                if (best != INT_MAX) root = uf_find(G, best);
+            }
         }
-        const long long b = G.hstart[slot], e = b + (long long)G.hcount[slot];
-        for (long long k = b; k < e; ++k) {
-            const int q = G.cellpts[k];
-            if (q < best && G.core[q] && within(G, p, q)) best = q;
-        }
-        return true;
-    });
-    labroot[p] = best == INT_MAX ? -1 : root[best];
-}
-
-__global__ void mark_roots_kernel(const int* root, long long n, const int* include, bool sign_rule, int* minfix,
-                                  long long* markP, long long* markN)
-{
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    markP[p] = 0;
-    if (markN) markN[p] = 0;
-    if (sign_rule && root[p] >= 0 && include[p] == -1) atomicMin(&minfix[root[p]], (int)p);
-}
-
-__global__ void mark_roots2_kernel(const int* root, long long n, bool sign_rule, const int* minfix, long long* markP,
-                                   long long* markN)
-{
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    if (root[p] != (int)p) return;
-    if (sign_rule && minfix[p] != INT_MAX) markN[minfix[p]] = 1;
-    else markP[p] = 1;
-}
-
-__global__ void label_kernel(const int* labroot, long long n, bool sign_rule, const int* minfix,
-                             const long long* scanP, const long long* scanN, int* labels)
-{
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const int r = labroot[p];
-    if (r < 0) {
-        labels[p] = 0;
-        return;
+        G.labroot[p] = root;
     }
-    if (sign_rule && minfix[r] != INT_MAX) labels[p] = -(int)(scanN[minfix[r]] + 1);
-    else labels[p] = (int)(scanP[r] + 1);
+    const unsigned long long bits = __ballot(isroot);
+    if (__lane_id() == 0 && (p >> 6) < G.nw) G.bitsP[p >> 6] = bits;
 }
 
-__global__ void fill_int_kernel(int* a, long long n, int v)
+// Sign rule: a root whose component holds a fixed point moves from the
+// positive numbering to the negative one, keyed by that fixed point.
+__global__ void sign_move_kernel(const GridDev G)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = v;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= G.n) return;
+    const unsigned long long bit = 1ull << (p & 63);
+    if (!(G.bitsP[p >> 6] & bit)) return;
+    const int f = G.minfix[p];
+    if (f == INT_MAX) return;
+    atomicAnd(&G.bitsP[p >> 6], ~bit);
+    atomicOr(&G.bitsN[f >> 6], 1ull << (f & 63));
 }
 
-__global__ void fill_u64_kernel(unsigned long long* a, long long n, unsigned long long v)
+__global__ void word_count_kernel(const GridDev G)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = v;
+    const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= G.nw) return;
+    G.rankP[w] = __popcll(G.bitsP[w]);
+    if (G.sign) G.rankN[w] = __popcll(G.bitsN[w]);
+}
+
+__device__ __forceinline__ int bit_rank(const unsigned long long* bits, const int* rank, int x)
+{
+    return rank[x >> 6] + __popcll(bits[x >> 6] & ((1ull << (x & 63)) - 1ull));
+}
+
+__global__ void label_kernel(const GridDev G, int* labels)
+{
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= G.n) return;
+    const int r = G.labroot[p];
+    int lab = 0;
+    if (r >= 0) {
+        const int f = G.sign ? G.minfix[r] : INT_MAX;
+        lab = f != INT_MAX ? -(bit_rank(G.bitsN, G.rankN, f) + 1) : bit_rank(G.bitsP, G.rankP, r) + 1;
+    }
+    labels[p] = lab;
+}
+
+__global__ void finish_kernel(const GridDev G)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const long long w = G.nw - 1;
+    G.ctl->K = w >= 0 ? G.rankP[w] + __popcll(G.bitsP[w]) : 0;
+}
+
+// Exclusive prefix sums of int32 (in place allowed): tiles of 1024 scanned in
+// LDS, tile sums scanned recursively in tmp.
+constexpr int kScanThreads = 256;
+constexpr int kScanPer = 4;
+constexpr int kScanTile = kScanThreads * kScanPer;
+
+__global__ __launch_bounds__(kScanThreads) void scan_tile_kernel(const int* in, int* out, int* tile_sums,
+                                                                 long long n)
+{
+    __shared__ int s[kScanThreads];
+    const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanPer;
+    int v[kScanPer];
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : 0;
+        sum += v[k];
+    }
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        const int t = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    int run = s[threadIdx.x] - sum;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == kScanThreads - 1) tile_sums[blockIdx.x] = s[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_add_kernel(int* out, const int* tile_off, long long n)
+{
+    const long long i = (long long)blockIdx.x * kScanThreads + threadIdx.x;
+    if (i < n) out[i] += tile_off[i / kScanTile];
+}
+
+static void scan_i32(hipStream_t st, const int* in, int* out, long long n, int* tmp)
+{
+    if (n <= 0) return;
+    const long long tiles = (n + kScanTile - 1) / kScanTile;
+    int* sums = tmp;
+    hipLaunchKernelGGL(scan_tile_kernel, dim3((unsigned)tiles), dim3(kScanThreads), 0, st, in, out, sums, n);
+    if (tiles > 1) {
+        int* offs = tmp + tiles;
+        scan_i32(st, sums, offs, tiles, tmp + 2 * tiles);
+        hipLaunchKernelGGL(scan_add_kernel, dim3(ceil_div(n, kScanThreads)), dim3(kScanThreads), 0, st, out, offs, n);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------
+
+// One pass of the pipeline on a carved workspace; returns the control block.
+static DbCtl run_pipeline(CallCtx& ctx, GridDev& G, const DbscanParams& prm, int* labels,
+                          const unsigned long long* box)
+{
+    hipStream_t st = ctx.stream();
+    const unsigned T = 256;
+    const long long n = G.n;
+    const unsigned gn = ceil_div(n, T);
+    const unsigned gh = ceil_div(G.H, T);
+    const unsigned gi = std::min<unsigned>(std::max(gh, gn), 4096u);
+    const double cell_lim = prm.dim == 3 ? (double)(1 << 21) - 4 : 2147483000.0;
+    hipLaunchKernelGGL(init_kernel, dim3(gi), dim3(T), 0, st, G, box, 1);
+    if (!box) hipLaunchKernelGGL(bbox_kernel, dim3(std::min<unsigned>(gn, 1024u)), dim3(T), 0, st, G);
+    hipLaunchKernelGGL(setup_kernel, dim3(1), dim3(64), 0, st, G, cell_lim);
+    const unsigned gc = ceil_div(n, kChunkPts);
+    hipLaunchKernelGGL(insert_kernel, dim3(gc), dim3(T), 0, st, G);
+    scan_i32(st, reinterpret_cast<const int*>(G.hcount), G.hstart, G.H, G.scan_tmp);
+    hipLaunchKernelGGL(place_kernel, dim3(gc), dim3(T), 0, st, G);
+    hipLaunchKernelGGL(cell_init_kernel, dim3(gh), dim3(T), 0, st, G, prm.minPts);
+    hipLaunchKernelGGL(core_dense_kernel, dim3(gn), dim3(T), 0, st, G, prm.minPts);
+    hipLaunchKernelGGL(core_sparse_kernel, dim3(kWaveBlocks), dim3(T), 0, st, G, prm.minPts);
+    hipLaunchKernelGGL(link_sparse_kernel, dim3(1024), dim3(T), 0, st, G);
+    hipLaunchKernelGGL(cell_list_kernel, dim3(gh), dim3(T), 0, st, G);
+    for (int ring = 0; ring < 3; ++ring)
+        hipLaunchKernelGGL(link_pairs_kernel, dim3(kWaveBlocks), dim3(T), 0, st, G, ring);
+    hipLaunchKernelGGL(border_kernel, dim3(gn), dim3(T), 0, st, G);
+    if (G.sign) hipLaunchKernelGGL(sign_move_kernel, dim3(gn), dim3(T), 0, st, G);
+    hipLaunchKernelGGL(word_count_kernel, dim3(ceil_div(G.nw, T)), dim3(T), 0, st, G);
+    scan_i32(st, G.rankP, G.rankP, G.nw, G.scan_tmp);
+    if (G.sign) scan_i32(st, G.rankN, G.rankN, G.nw, G.scan_tmp);
+    hipLaunchKernelGGL(label_kernel, dim3(gn), dim3(T), 0, st, G, labels);
+    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(64), 0, st, G);
+    CHAOS_HIP(hipGetLastError());
+    ctx.add_launch(22);
+    DbCtl c{};
+    CHAOS_HIP(hipMemcpyAsync(&c, G.ctl, sizeof(DbCtl), hipMemcpyDeviceToHost, st));
+    CHAOS_HIP(hipStreamSynchronize(st));
+    return c;
 }
 
 int dbscan_device(CallCtx& ctx, const double* const* pts, const int* include, long long n, const DbscanParams& prm,
-                  int* labels)
+                  int* labels, void* ws, size_t ws_bytes, const unsigned long long* box)
 {
     CHAOS_REQUIRE(prm.dim == 2 || prm.dim == 3, "dbscan dim must be 2 or 3");
     CHAOS_REQUIRE(prm.eps > 0 && std::isfinite(prm.eps), "eps must be positive");
     CHAOS_REQUIRE(prm.minPts >= 1, "minPts must be >= 1");
-    hipStream_t st = ctx.stream();
-    const unsigned T = 256;
-    const unsigned gn = ceil_div(n, T);
+    CHAOS_REQUIRE(n >= 1 && n < INT_MAX, "dbscan point count out of range");
     GridDev G{};
     for (int d = 0; d < 3; ++d) G.x[d] = d < prm.dim ? pts[d] : pts[0];
-    G.dim = prm.dim;
-    G.eps = prm.eps;
     G.include = include;
-
-    // bounding box of the included points
-    const unsigned nb = std::min<unsigned>(gn, 2048);
-    DevBuf<double> bb((size_t)nb * 6);
-    hipLaunchKernelGGL(bbox_kernel, dim3(nb), dim3(256), 0, st, G, n, bb.get());
-    CHAOS_HIP(hipGetLastError());
-    std::vector<double> hb((size_t)nb * 6);
-    CHAOS_HIP(hipMemcpyAsync(hb.data(), bb.get(), hb.size() * 8, hipMemcpyDeviceToHost, st));
-    CHAOS_HIP(hipStreamSynchronize(st));
-    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (unsigned b = 0; b < nb; ++b)
-        for (int d = 0; d < 3; ++d) {
-            lo[d] = std::fmin(lo[d], hb[b * 6 + d]);
-            hi[d] = std::fmax(hi[d], hb[b * 6 + 3 + d]);
-        }
-    if (!(lo[0] <= hi[0])) {  // nothing included
-        hipLaunchKernelGGL(fill_int_kernel, dim3(gn), dim3(T), 0, st, labels, n, 0);
-        CHAOS_HIP(hipGetLastError());
-        return 0;
-    }
+    G.dim = prm.dim;
+    G.sign = prm.sign_rule ? 1 : 0;
+    G.eps = prm.eps;
     // cell side eps/sqrt(dim), shrunk so that rounding can never put two
     // points of one cell farther apart than eps
     G.s = prm.eps / std::sqrt((double)prm.dim) * (1.0 - 1e-7);
     G.prune2 = (prm.eps * (1 + 1e-7)) * (prm.eps * (1 + 1e-7));
-    const double cell_lim = prm.dim == 3 ? (double)(1 << 21) - 4 : 2147483000.0;
-    for (int d = 0; d < prm.dim; ++d) {
-        CHAOS_REQUIRE(std::isfinite(lo[d]) && std::isfinite(hi[d]), "non-finite feature values");
-        G.lo[d] = lo[d];
-        const double cells = std::floor((hi[d] - lo[d]) / G.s) + 1;
-        if (cells >= cell_lim)
-            fail(CHAOS_E_UNSUPPORTED, "feature range %.3g too wide for eps %.3g in dim %d", hi[d] - lo[d], prm.eps, d);
+    long long H = dbscan_first_cells(n);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        const size_t need = dbscan_ws_bytes(n, H, prm.sign_rule);
+        DevBuf<char> own;
+        char* base = static_cast<char*>(ws);
+        if (!ws) {
+            own.alloc(need);
+            base = own.get();
+        } else if (need > ws_bytes) {
+            fail(CHAOS_E_CAPACITY, "clustering needs %zu bytes of scratch (%lld cells), %zu available", need, H,
+                 ws_bytes);
+        }
+        carve(G, base, n, H, prm.sign_rule);
+        const DbCtl c = run_pipeline(ctx, G, prm, labels, box);
+        if (c.flags & DB_NONFINITE) fail(CHAOS_E_INVALID, "non-finite feature values");
+        if (c.flags & DB_RANGE) fail(CHAOS_E_UNSUPPORTED, "feature range too wide for eps %.3g", prm.eps);
+        if (!(c.flags & DB_FULL)) return c.K;
+        if (H >= dbscan_max_cells(n)) fail(CHAOS_E_HIP, "cell table full at %lld cells", H);
+        H = dbscan_max_cells(n);
     }
-    unsigned long long H = 1024;
-    while (H < 2ull * (unsigned long long)n) H <<= 1;
-    G.hmask = H - 1;
-    DevBuf<unsigned long long> hkey(H), hcount(H);
-    DevBuf<long long> hstart(H);
-    DevBuf<int> hcur(H), hrep(H), pslot(n), cellpts(n), core(n), parent(n), root(n), labroot(n);
-    DevBuf<unsigned long long> bbox(6 * H);
-    G.bbox = bbox.get();
-    G.H = (long long)H;
-    G.hkey = hkey.get();
-    G.hcount = hcount.get();
-    G.hstart = hstart.get();
-    G.hcur = hcur.get();
-    G.hrep = hrep.get();
-    G.pslot = pslot.get();
-    G.cellpts = cellpts.get();
-    G.core = core.get();
-    G.parent = parent.get();
-    const unsigned gh = ceil_div(H, T);
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(gh), dim3(T), 0, st, G.hkey, (long long)H, kEmpty);
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(gh), dim3(T), 0, st, G.hcount, (long long)H, 0ull);
-    hipLaunchKernelGGL(fill_int_kernel, dim3(gh), dim3(T), 0, st, G.hcur, (long long)H, 0);
-    hipLaunchKernelGGL(fill_int_kernel, dim3(gh), dim3(T), 0, st, G.hrep, (long long)H, INT_MAX);
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(3 * gh), dim3(T), 0, st, G.bbox, 3 * (long long)H, ~0ull);
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(3 * gh), dim3(T), 0, st, G.bbox + 3 * H, 3 * (long long)H, 0ull);
-    hipLaunchKernelGGL(insert_kernel, dim3(gn), dim3(T), 0, st, G, n);
-    CHAOS_HIP(hipGetLastError());
-    exclusive_scan(st, reinterpret_cast<const long long*>(G.hcount), G.hstart, (long long)H);
-    hipLaunchKernelGGL(fill_kernel, dim3(gn), dim3(T), 0, st, G, n);
-    DevBuf<int> cand(n), cells(n), counters(2);  // counters: candidates, cells
-    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, st, counters.get(), 2LL, 0);
-    hipLaunchKernelGGL(core_dense_kernel, dim3(gn), dim3(T), 0, st, G, n, prm.minPts, cand.get(), counters.get());
-    hipLaunchKernelGGL(core_sparse_kernel, dim3(kWaveBlocks), dim3(T), 0, st, G, cand.get(), counters.get(),
-                       prm.minPts);
-    hipLaunchKernelGGL(link_cell_kernel, dim3(gn), dim3(T), 0, st, G, n);
-    hipLaunchKernelGGL(cell_list_kernel, dim3(gh), dim3(T), 0, st, G, cells.get(), counters.get() + 1);
-    for (int ring = 0; ring < 3; ++ring)
-        hipLaunchKernelGGL(link_pairs_kernel, dim3(kWaveBlocks), dim3(T), 0, st, G, cells.get(), counters.get() + 1,
-                           ring);
-    hipLaunchKernelGGL(flatten_kernel, dim3(gn), dim3(T), 0, st, G, n, root.get());
-    hipLaunchKernelGGL(border_kernel, dim3(gn), dim3(T), 0, st, G, n, root.get(), labroot.get());
-    CHAOS_HIP(hipGetLastError());
-
-    DevBuf<int> minfix(n);
-    DevBuf<long long> markP(n), scanP(n), markN, scanN;
-    if (prm.sign_rule) {
-        markN.alloc(n);
-        scanN.alloc(n);
-    }
-    hipLaunchKernelGGL(fill_int_kernel, dim3(gn), dim3(T), 0, st, minfix.get(), n, INT_MAX);
-    hipLaunchKernelGGL(mark_roots_kernel, dim3(gn), dim3(T), 0, st, root.get(), n, include, prm.sign_rule,
-                       minfix.get(), markP.get(), markN.get());
-    hipLaunchKernelGGL(mark_roots2_kernel, dim3(gn), dim3(T), 0, st, root.get(), n, prm.sign_rule, minfix.get(),
-                       markP.get(), markN.get());
-    CHAOS_HIP(hipGetLastError());
-    const long long K = exclusive_scan(st, markP.get(), scanP.get(), n);
-    if (prm.sign_rule) exclusive_scan(st, markN.get(), scanN.get(), n);
-    hipLaunchKernelGGL(label_kernel, dim3(gn), dim3(T), 0, st, labroot.get(), n, prm.sign_rule, minfix.get(),
-                       scanP.get(), scanN.get(), labels);
-    CHAOS_HIP(hipGetLastError());
-    CHAOS_HIP(hipStreamSynchronize(st));
-    ctx.add_launch(21);
-    return (int)K;
+    fail(CHAOS_E_HIP, "clustering did not settle");
 }
 
 // ---------------------------------------------------------------------------
-// deterministic reductions and z-score
+// deterministic z-score
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(512) void det_block_sum_kernel(const double* __restrict__ x, const int* __restrict__ mask,
-                                                            long long n, bool sq, double mean,
-                                                            double* __restrict__ out)
+// Block b sums its 1024 points with the fixed tree (s[t] = v[t] + v[t+512],
+// then strides 256 .. 1) for every feature; pass 0 sums the values and counts
+// the masked points, pass 1 the squared deviations from the pass-0 mean.
+__global__ __launch_bounds__(512) void moment_block_kernel(const double* x0, const double* x1, const double* x2,
+                                                           const int* __restrict__ mask, long long n, int k, int pass,
+                                                           const double* __restrict__ mom, double* __restrict__ part,
+                                                           int* __restrict__ cnt)
 {
     __shared__ double s[512];
+    const double* xs[3] = {x0, x1, x2};
     const long long base = (long long)blockIdx.x * 1024;
-    auto val = [&](long long i) -> double {
-        if (i >= n || (mask && mask[i] == 0)) return 0.0;
-        if (!sq) return x[i];
-        const double d = x[i] - mean;
-        return d * d;
-    };
     const int t = threadIdx.x;
-    s[t] = val(base + t) + val(base + t + 512);  // stride-512 level
-    __syncthreads();
-    for (int stride = 256; stride > 0; stride >>= 1) {
-        if (t < stride) s[t] = s[t] + s[t + stride];
+    const long long nb = gridDim.x;
+    for (int j = 0; j < k; ++j) {
+        const double* x = xs[j];
+        const double mean = pass == 1 ? mom[2 * j] : 0.0;
+        auto val = [&](long long i) -> double {
+            if (i >= n || (mask && mask[i] == 0)) return 0.0;
+            if (pass == 0) return x[i];
+            const double d = x[i] - mean;
+            return d * d;
+        };
+        s[t] = val(base + t) + val(base + t + 512);
+        __syncthreads();
+        for (int stride = 256; stride > 0; stride >>= 1) {
+            if (t < stride) s[t] = s[t] + s[t + stride];
+            __syncthreads();
+        }
+        if (t == 0) part[j * nb + blockIdx.x] = s[0];
         __syncthreads();
     }
-    if (t == 0) out[blockIdx.x] = s[0];
+    if (pass == 0) {
+        __shared__ int sc[8];
+        const long long i0 = base + t, i1 = base + t + 512;
+        const bool m0 = i0 < n && (!mask || mask[i0] != 0), m1 = i1 < n && (!mask || mask[i1] != 0);
+        const int c = __popcll(__ballot(m0)) + __popcll(__ballot(m1));
+        if (__lane_id() == 0) sc[t >> 6] = c;
+        __syncthreads();
+        if (t == 0) {
+            int tot = 0;
+            for (int w = 0; w < 8; ++w) tot += sc[w];
+            cnt[blockIdx.x] = tot;
+        }
+    }
 }
 
-double det_sum_device(CallCtx& ctx, const double* x, const int* mask, long long n, bool sq, double mean)
+// Block sums added in block order: mom[2j] = mean (pass 0) or mom[2j+1] = sd
+// (pass 1); mom[2k] = number of masked points.
+__global__ void moment_final_kernel(const double* __restrict__ part, const int* __restrict__ cnt, long long nb, int k,
+                                    int pass, double* __restrict__ mom)
 {
-    if (n <= 0) return 0.0;
-    const long long nb = (n + 1023) / 1024;
-    DevBuf<double> bs((size_t)nb);
-    hipLaunchKernelGGL(det_block_sum_kernel, dim3((unsigned)nb), dim3(512), 0, ctx.stream(), x, mask, n, sq, mean,
-                       bs.get());
-    CHAOS_HIP(hipGetLastError());
-    std::vector<double> h((size_t)nb);
-    CHAOS_HIP(hipMemcpyAsync(h.data(), bs.get(), nb * 8, hipMemcpyDeviceToHost, ctx.stream()));
-    CHAOS_HIP(hipStreamSynchronize(ctx.stream()));
+    const int j = threadIdx.x;
+    if (j > k) return;
+    if (j == k) {
+        if (pass == 0) {
+            long long N = 0;
+            for (long long b = 0; b < nb; ++b) N += cnt[b];
+            mom[2 * k] = (double)N;
+        }
+        return;
+    }
     double tot = 0.0;
-    for (long long b = 0; b < nb; ++b) tot = tot + h[b];  // block sums in order
-    return tot;
+    for (long long b = 0; b < nb; ++b) tot = tot + part[j * nb + b];
+    if (pass == 0) {
+        long long N = 0;
+        for (long long b = 0; b < nb; ++b) N += cnt[b];
+        mom[2 * j] = N > 0 ? tot / (double)N : 0.0;
+    } else {
+        const double N = mom[2 * k];
+        mom[2 * j + 1] = N > 0 ? sqrt(tot / N) : 0.0;
+    }
 }
 
-__global__ void count_mask_kernel(const int* mask, long long n, unsigned long long* cnt)
+// dst = z-scored src on the masked points, src elsewhere (and everywhere when
+// no point is masked); box: encoded bounding box of dst over masked points.
+__global__ __launch_bounds__(256) void normalize_kernel(const double* x0, const double* x1, const double* x2, double* y0,
+                                                       double* y1, double* y2, const int* __restrict__ mask,
+                                                       long long n, int k, const double* __restrict__ mom,
+                                                       unsigned long long* box)
 {
+    __shared__ unsigned long long smin[3][4], smax[3][4];
+    const double* xs[3] = {x0, x1, x2};
+    double* ys[3] = {y0, y1, y2};
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool m = i < n && mask[i] != 0;
-    const unsigned long long b = __ballot(m);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(cnt, (unsigned long long)__popcll(b));
+    const bool any = mom[2 * k] > 0;
+    const bool in = i < n && (!mask || mask[i] != 0);
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    for (int j = 0; j < k; ++j) {
+        unsigned long long e = 0;
+        if (i < n) {
+            const double v = xs[j][i];
+            const double mean = mom[2 * j], sd = mom[2 * j + 1];
+            const double z = (!any || !in) ? v : (sd > 0 ? (v - mean) / sd : 0.0);
+            ys[j][i] = z;
+            e = ord_enc(z);
+        }
+        if (box) {
+            const unsigned long long mn = wave_min_u64(in ? e : ~0ull), mx = wave_max_u64(in ? e : 0ull);
+            if (lane == 0) {
+                smin[j][wave] = mn;
+                smax[j][wave] = mx;
+            }
+        }
+    }
+    if (!box) return;
+    __syncthreads();
+    if (threadIdx.x < k) {
+        const int j = threadIdx.x;
+        unsigned long long a = smin[j][0], b = smax[j][0];
+        for (int w = 1; w < 4; ++w) {
+            a = smin[j][w] < a ? smin[j][w] : a;
+            b = smax[j][w] > b ? smax[j][w] : b;
+        }
+        if (a != ~0ull) {
+            atomicMin(&box[j], a);
+            atomicMax(&box[3 + j], b);
+        }
+    }
 }
 
-// y = z-scored x on the masked points, x elsewhere (y may be x).
-__global__ void normalize_kernel(const double* x, double* y, const int* mask, long long n, double mean, double sd)
+__global__ void box_init_kernel(unsigned long long* box)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double v = x[i];
-    y[i] = mask[i] == 0 ? v : (sd > 0 ? (v - mean) / sd : 0.0);
+    if (threadIdx.x < 6) box[threadIdx.x] = threadIdx.x < 3 ? ~0ull : 0ull;
+}
+
+size_t zscore_scratch_bytes(long long n, int k)
+{
+    const long long nb = (n + 1023) / 1024;
+    return align256(8 * (size_t)k * nb) + align256(8 * (size_t)(2 * k + 1)) + align256(4 * (size_t)nb) + 256;
 }
 
 void zscore_device(CallCtx& ctx, const double* const* src, double* const* dst, int k, const int* mask, long long n,
-                   double* moments)
+                   double* moments, unsigned long long* box, void* scratch)
 {
-    // the block sums of all k features come back in one copy per moment
-    // (two round trips in all), summed on the host in block order
+    CHAOS_REQUIRE(k >= 1 && k <= 3, "z-score of 1 to 3 features");
+    if (n <= 0) return;
+    hipStream_t st = ctx.stream();
     const long long nb = (n + 1023) / 1024;
-    DevBuf<unsigned long long> c(1);
-    DevBuf<double> bs((size_t)(k * nb));
-    std::vector<double> h((size_t)(k * nb));
-    unsigned long long N = 0;
-    CHAOS_HIP(hipMemsetAsync(c.get(), 0, 8, ctx.stream()));
-    hipLaunchKernelGGL(count_mask_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx.stream(), mask, n, c.get());
-    std::vector<double> mean(k), sd(k);
-    for (int pass = 0; pass < 2; ++pass) {
-        for (int j = 0; j < k; ++j)
-            hipLaunchKernelGGL(det_block_sum_kernel, dim3((unsigned)nb), dim3(512), 0, ctx.stream(), src[j], mask, n,
-                               pass == 1, pass == 1 ? mean[j] : 0.0, bs.get() + j * nb);
-        CHAOS_HIP(hipGetLastError());
-        if (pass == 0) CHAOS_HIP(hipMemcpyAsync(&N, c.get(), 8, hipMemcpyDeviceToHost, ctx.stream()));
-        CHAOS_HIP(hipMemcpyAsync(h.data(), bs.get(), k * nb * 8, hipMemcpyDeviceToHost, ctx.stream()));
-        CHAOS_HIP(hipStreamSynchronize(ctx.stream()));
-        if (N == 0) break;
-        for (int j = 0; j < k; ++j) {
-            double tot = 0.0;
-            for (long long b = 0; b < nb; ++b) tot = tot + h[j * nb + b];
-            if (pass == 0) mean[j] = tot / (double)N;
-            else sd[j] = std::sqrt(tot / (double)N);
-        }
+    DevBuf<char> own;
+    char* base = static_cast<char*>(scratch);
+    if (!base) {
+        own.alloc(zscore_scratch_bytes(n, k));
+        base = own.get();
     }
-    for (int j = 0; j < k; ++j) {
-        if (N > 0 && moments) {
-            moments[2 * j] = mean[j];
-            moments[2 * j + 1] = sd[j];
-        }
-        if (N > 0)
-            hipLaunchKernelGGL(normalize_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx.stream(), src[j], dst[j],
-                               mask, n, mean[j], sd[j]);
-        else if (dst[j] != src[j])
-            CHAOS_HIP(hipMemcpyAsync(dst[j], src[j], n * 8, hipMemcpyDeviceToDevice, ctx.stream()));
-        CHAOS_HIP(hipGetLastError());
+    double* part = (double*)base;
+    double* mom = (double*)(base + align256(8 * (size_t)k * nb));
+    int* cnt = (int*)((char*)mom + align256(8 * (size_t)(2 * k + 1)));
+    const double* s[3] = {src[0], k > 1 ? src[1] : src[0], k > 2 ? src[2] : src[0]};
+    double* d[3] = {dst[0], k > 1 ? dst[1] : dst[0], k > 2 ? dst[2] : dst[0]};
+    for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL(moment_block_kernel, dim3((unsigned)nb), dim3(512), 0, st, s[0], s[1], s[2], mask, n, k,
+                           pass, mom, part, cnt);
+        hipLaunchKernelGGL(moment_final_kernel, dim3(1), dim3(64), 0, st, part, cnt, nb, k, pass, mom);
+    }
+    if (box) hipLaunchKernelGGL(box_init_kernel, dim3(1), dim3(64), 0, st, box);
+    hipLaunchKernelGGL(normalize_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, s[0], s[1], s[2], d[0], d[1], d[2],
+                       mask, n, k, mom, box);
+    CHAOS_HIP(hipGetLastError());
+    ctx.add_launch(5 + (box ? 1 : 0));
+    if (moments) {
+        double h[7] = {};
+        CHAOS_HIP(hipMemcpyAsync(h, mom, (2 * k + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+        CHAOS_HIP(hipStreamSynchronize(st));
+        for (int j = 0; j < 2 * k; ++j) moments[j] = h[j];
+    } else if (!scratch) {
+        CHAOS_HIP(hipStreamSynchronize(st));  // own scratch is released on return
     }
 }
 
@@ -936,12 +1364,11 @@ extern "C" int chaos_dbscan(const chaos_options* opt, const double* x, const dou
             }
             d_lab.alloc(n);
             lab = d_lab.get();
-        }
-        if (!inc || !dev) {
-            d_inc.alloc(n);
-            if (inc) CHAOS_HIP(hipMemcpyAsync(d_inc.get(), inc, n * sizeof(int), hipMemcpyHostToDevice, st));
-            else hipLaunchKernelGGL(fill_int_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, d_inc.get(), n, 1);
-            inc = d_inc.get();
+            if (inc) {
+                d_inc.alloc(n);
+                CHAOS_HIP(hipMemcpyAsync(d_inc.get(), inc, n * sizeof(int), hipMemcpyHostToDevice, st));
+                inc = d_inc.get();
+            }
         }
         ctx.begin();
         DbscanParams prm{dim, eps, sign_rule ? 1 : min_pts, sign_rule != 0};

@@ -1,5 +1,8 @@
-// dbscan.hpp -- grid DBSCAN over 2-D / 3-D feature points on the device.
+// dbscan.hpp -- grid DBSCAN over 2-D / 3-D feature points on the device, and
+// the deterministic z-score that feeds it.
 #pragma once
+
+#include <cstddef>
 
 #include "runtime.hpp"
 
@@ -12,22 +15,64 @@ struct DbscanParams {
     bool sign_rule;   // Basins REFERENCE: components holding a flag -1 point are negative
 };
 
-// pts: dim device arrays of n doubles; include: device flags (point used iff
-// include[i] != 0; with sign_rule, include[i] == -1 marks a fixed point).
+// Scratch of one clustering call: everything the kernels use besides the
+// caller's points, include flags and labels, carved from one device buffer
+// in 256-byte aligned pieces: a control block; per cell (H slots, a power of
+// two) key, count, smallest member, segment start and cursor, smallest core,
+// member box and the core-cell list; per point slot, cell list, parent,
+// label root, candidate list (and smallest fixed point under the sign rule);
+// per 64-point word root bits and their prefix sums; scan temporaries.
+constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+constexpr size_t scan_tmp_ints(long long n)
+{
+    return n <= 1024 ? 1 : 2 * (size_t)((n + 1023) / 1024) + scan_tmp_ints((n + 1023) / 1024);
+}
+
+constexpr size_t dbscan_ws_bytes_c(long long n, long long H, bool sign)
+{
+    const long long nw = (n + 63) / 64;
+    size_t t = 256;  // control block
+    t += align256(8 * (size_t)H) + 5 * align256(4 * (size_t)H) + align256(48 * (size_t)H) + align256(4 * (size_t)H);
+    t += 5 * align256(4 * (size_t)n) + (sign ? align256(4 * (size_t)n) : 0);
+    t += align256(8 * (size_t)nw) + align256(4 * (size_t)nw);
+    if (sign) t += align256(8 * (size_t)nw) + align256(4 * (size_t)nw);
+    t += align256(4 * scan_tmp_ints(H > nw ? H : nw));
+    return t;
+}
+
+size_t dbscan_ws_bytes(long long n, long long H, bool sign_rule);
+// Cell-table sizes for n points: the first attempt is sized for the few
+// distinct cells clustered features occupy (a table more than half full is
+// redone at the second size, which holds any n points).
+long long dbscan_first_cells(long long n);
+long long dbscan_max_cells(long long n);
+
+// pts: dim device arrays of n doubles; include: device flags or NULL (all
+// points used; with sign_rule, include[i] == -1 marks a fixed point).
 // Writes labels[n] (device).  Deterministic: clusters are numbered 1..K by
 // smallest core index (sign_rule: negatives -1..-K by smallest fixed-point
 // index); a border point joins the cluster of its smallest-index core
 // neighbour; noise and excluded points get 0.  Returns K (positives).
+// ws / ws_bytes: caller scratch (a call that needs more than ws_bytes fails
+// with CHAOS_E_CAPACITY); NULL takes the scratch from the device allocator.
+// box: device [6] order-preserving encodings of the included points'
+// bounding box (min x, y, z, max x, y, z), or NULL to compute it.
+// Synchronises the stream once (twice when the first cell table fills).
 int dbscan_device(CallCtx& ctx, const double* const* pts, const int* include, long long n,
-                  const DbscanParams& prm, int* labels);
+                  const DbscanParams& prm, int* labels, void* ws = nullptr, size_t ws_bytes = 0,
+                  const unsigned long long* box = nullptr);
 
-// Deterministic fixed-order sum (blocks of 1024 reduced by the pairwise tree
-// stride 512..1, block sums added in order) of mask ? f(x) : 0 where f is x
-// (sq == false) or (x - mean)^2 (sq == true).
-double det_sum_device(CallCtx& ctx, const double* x, const int* mask, long long n, bool sq, double mean);
-
-// In-place z-score of the k feature arrays over points with mask != 0.
+// Z-score of the k feature arrays over points with mask != 0, out of place,
+// with no host round trip: two moment passes (fixed-order block sums, added
+// in block order on the device), then one pass that writes dst and, when box
+// is not NULL, the encoded bounding box of the normalised included points
+// (device [6], the form dbscan_device takes).  moments (host, may be NULL)
+// receives mean_j, sd_j after a synchronisation.  scratch: device memory of
+// zscore_scratch_bytes(n, k) that stays valid until the stream has run the
+// call's kernels, or NULL (then the call allocates and synchronises).
+size_t zscore_scratch_bytes(long long n, int k);
 void zscore_device(CallCtx& ctx, const double* const* src, double* const* dst, int k, const int* mask, long long n,
-                   double* moments);
+                   double* moments, unsigned long long* box = nullptr, void* scratch = nullptr);
 
 }  // namespace chaos

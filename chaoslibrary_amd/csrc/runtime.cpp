@@ -218,7 +218,7 @@ const hipDeviceProp_t& device_props(int device)
     return g_props[device];
 }
 
-CallCtx::CallCtx(const chaos_options& o) : opt_(o), device_(o.device)
+CallCtx::CallCtx(const chaos_options& o, bool library_stream_ok) : opt_(o), device_(o.device)
 {
     t0_ = wall_ms();
     int count = 0;
@@ -227,9 +227,7 @@ CallCtx::CallCtx(const chaos_options& o) : opt_(o), device_(o.device)
     if (device_ < 0 || device_ >= count) fail(CHAOS_E_INVALID, "device %d out of range (%d devices)", device_, count);
     CHAOS_HIP(hipGetDevice(&prev_device_));
     CHAOS_HIP(hipSetDevice(device_));
-    stream_ = o.stream ? static_cast<hipStream_t>(o.stream) : library_stream(device_);
-    CHAOS_HIP(hipEventCreate(&ev0_));
-    CHAOS_HIP(hipEventCreate(&ev1_));
+    stream_ = o.stream ? static_cast<hipStream_t>(o.stream) : (library_stream_ok ? library_stream(device_) : nullptr);
 }
 
 CallCtx::~CallCtx()
@@ -241,7 +239,12 @@ CallCtx::~CallCtx()
 
 const hipDeviceProp_t& CallCtx::props() const { return device_props(device_); }
 
-void CallCtx::begin() { CHAOS_HIP(hipEventRecord(ev0_, stream_)); }
+void CallCtx::begin()
+{
+    if (!ev0_) CHAOS_HIP(hipEventCreate(&ev0_));
+    if (!ev1_) CHAOS_HIP(hipEventCreate(&ev1_));
+    CHAOS_HIP(hipEventRecord(ev0_, stream_));
+}
 
 double CallCtx::end()
 {

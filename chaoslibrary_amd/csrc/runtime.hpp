@@ -115,7 +115,10 @@ private:
 // records HIP events for the timing report.
 class CallCtx {
 public:
-    explicit CallCtx(const chaos_options& o);
+    // library_stream_ok false: with no caller stream the call runs on the
+    // null stream instead of creating the library stream.  Timing events are
+    // created at the first begin().
+    explicit CallCtx(const chaos_options& o, bool library_stream_ok = true);
     ~CallCtx();
     CallCtx(const CallCtx&) = delete;
     CallCtx& operator=(const CallCtx&) = delete;
