@@ -243,6 +243,23 @@ int chaos_dbscan(const chaos_options* opt, const double* x, const double* y,
                  int min_pts, int sign_rule, const int* include, int* labels,
                  int* clusters);
 
+/* No counterpart in the reference.  chaos_dbscan for callers that cannot
+ * allocate (code under hipGraph capture, tight serving loops): the same
+ * kernels and results, with no hipMalloc, hipFree or other allocation -- the
+ * clustering scratch is a static device array of the library (about 47 MB
+ * per GPU, held until the process ends).  x, y, z, include and labels are
+ * device pointers whatever opt->device_buffers says; the call runs on
+ * opt->stream (NULL: the null stream) and fills no opt->stats.  At most
+ * CHAOS_DBSCAN_NOALLOC_MAX_POINTS points, and the points may occupy at most
+ * 2^17 distinct grid cells (cells of side eps/sqrt(dim)); beyond either it
+ * returns CHAOS_E_CAPACITY.  Calls on one GPU share its scratch: calls that
+ * overlap on one GPU race. */
+#define CHAOS_DBSCAN_NOALLOC_MAX_POINTS (1LL << 20)
+int chaos_dbscan_noalloc(const chaos_options* opt, const double* x,
+                         const double* y, const double* z, long long n,
+                         int dim, double eps, int min_pts, int sign_rule,
+                         const int* include, int* labels, int* clusters);
+
 /* ---------------- Legacy entry points (debri/, namespace old_library) ---- */
 /* The legacy library sweeps the same integrator with its own LLE kernels
  * (debri/cudaLibrary.cu:1897-2137), the Lyapunov spectrum
