@@ -963,6 +963,270 @@ int orc_dbscan_bruteforce(const double* pts, int n, int dim, double eps,
     return K;
 }
 
+/*
+ * The same clustering as orc_dbscan_bruteforce in O(n) neighbour work, for
+ * full-size label checks (millions of points): points binned into cells of
+ * side s = eps/sqrt(dim) * (1 - 1e-7) (two points of one cell lie within
+ * eps), a point's eps-neighbours all lie in the 5^dim cells around its own,
+ * and a neighbour cell is taken whole or skipped when its members' exact box
+ * lies wholly within / beyond eps (the gap and far distances are formed like
+ * the point distance, so the verdicts agree with it point by point).  Same
+ * rules and numbering as the brute force.  Returns K, -2 when a coordinate
+ * range is too wide for the cell keys, -3 on allocation failure.
+ * Pinned by: equality with orc_dbscan_bruteforce (tests/test_oracle_cluster.py).
+ */
+typedef struct { unsigned long long key; int idx; } orc_ki;
+
+static int orc_cmp_ki(const void* a, const void* b)
+{
+    const orc_ki* x = (const orc_ki*)a;
+    const orc_ki* y = (const orc_ki*)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return (x->idx > y->idx) - (x->idx < y->idx);
+}
+
+typedef struct {
+    const double* pts;
+    int dim;
+    double eps;
+    int ncell;
+    const unsigned long long* ckey;
+    const int* cstart;   /* [ncell + 1] into ord */
+    const orc_ki* ord;
+    const double* cmin;  /* [ncell][3] */
+    const double* cmax;
+} orc_grid;
+
+static int orc_within(const orc_grid* g, int p, int q)
+{
+    double d2 = 0.0;
+    for (int k = 0; k < g->dim; ++k) {
+        double df = g->pts[(size_t)p * g->dim + k] - g->pts[(size_t)q * g->dim + k];
+        d2 = d2 + df * df;
+    }
+    return sqrt(d2) <= g->eps;
+}
+
+/* 0: no member within eps of p; 2: every member within eps; 1: undecided. */
+static int orc_box_relation(const orc_grid* g, int p, int c)
+{
+    double g2 = 0.0, f2 = 0.0;
+    for (int k = 0; k < g->dim; ++k) {
+        double lo = g->cmin[3 * c + k], hi = g->cmax[3 * c + k];
+        double v = g->pts[(size_t)p * g->dim + k];
+        double gap = v < lo ? lo - v : (v > hi ? v - hi : 0.0);
+        double far = fmax(v - lo, hi - v);
+        g2 = g2 + gap * gap;
+        f2 = f2 + far * far;
+    }
+    if (sqrt(g2) > g->eps) return 0;
+    if (sqrt(f2) <= g->eps) return 2;
+    return 1;
+}
+
+static int orc_find_cell(const orc_grid* g, unsigned long long key)
+{
+    int a = 0, b = g->ncell;
+    while (a < b) {
+        int m = (a + b) / 2;
+        if (g->ckey[m] < key) a = m + 1;
+        else b = m;
+    }
+    return a < g->ncell && g->ckey[a] == key ? a : -1;
+}
+
+static unsigned long long orc_pack(int dim, const long long* c)
+{
+    if (dim == 3) return (unsigned long long)c[0] | ((unsigned long long)c[1] << 21) | ((unsigned long long)c[2] << 42);
+    return (unsigned long long)c[0] | ((unsigned long long)c[1] << 31);
+}
+
+/* Cells of the 5^dim block around cell `c0` coordinates: out[] indices (-1
+ * for none); returns the count written. */
+static int orc_neighbours(const orc_grid* g, const long long* c0, int* out)
+{
+    int m = 0;
+    int rz = g->dim == 3 ? 2 : 0;
+    for (int dz = -rz; dz <= rz; ++dz)
+        for (int dy = -2; dy <= 2; ++dy)
+            for (int dx = -2; dx <= 2; ++dx) {
+                long long c[3] = {c0[0] + dx, c0[1] + dy, c0[2] + dz};
+                if (c[0] < 0 || c[1] < 0 || c[2] < 0) continue;
+                int k = orc_find_cell(g, orc_pack(g->dim, c));
+                if (k >= 0) out[m++] = k;
+            }
+    return m;
+}
+
+int orc_dbscan_grid(const double* pts, int n, int dim, double eps, int minPts,
+                    const int* include, int* labels)
+{
+    if (dim != 2 && dim != 3) return -1;
+    double s = eps / sqrt((double)dim) * (1.0 - 1e-7);
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    int ninc = 0;
+    for (int i = 0; i < n; ++i) {
+        labels[i] = 0;
+        if (include && !include[i]) continue;
+        ++ninc;
+        for (int k = 0; k < dim; ++k) {
+            double v = pts[(size_t)i * dim + k];
+            lo[k] = fmin(lo[k], v);
+            hi[k] = fmax(hi[k], v);
+        }
+    }
+    if (ninc == 0) return 0;
+    double lim = dim == 3 ? (double)(1 << 21) - 4 : 2147483000.0;
+    for (int k = 0; k < dim; ++k)
+        if (floor((hi[k] - lo[k]) / s) + 1 >= lim) return -2;
+    orc_ki* ord = (orc_ki*)malloc(sizeof(orc_ki) * (size_t)ninc);
+    int* cell_of = (int*)malloc(sizeof(int) * (size_t)n);
+    int* core = (int*)calloc((size_t)n, sizeof(int));
+    int* par = (int*)malloc(sizeof(int) * (size_t)n);
+    int* num = (int*)calloc((size_t)n, sizeof(int));
+    if (!ord || !cell_of || !core || !par || !num) {
+        free(ord); free(cell_of); free(core); free(par); free(num);
+        return -3;
+    }
+    int m = 0;
+    for (int i = 0; i < n; ++i) {
+        cell_of[i] = -1;
+        if (include && !include[i]) continue;
+        long long c[3] = {0, 0, 0};
+        for (int k = 0; k < dim; ++k) c[k] = (long long)floor((pts[(size_t)i * dim + k] - lo[k]) / s);
+        ord[m].key = orc_pack(dim, c);
+        ord[m].idx = i;
+        ++m;
+    }
+    qsort(ord, (size_t)ninc, sizeof(orc_ki), orc_cmp_ki);
+    int ncell = 0;
+    for (int t = 0; t < ninc; ++t)
+        if (t == 0 || ord[t].key != ord[t - 1].key) ++ncell;
+    unsigned long long* ckey = (unsigned long long*)malloc(sizeof(unsigned long long) * (size_t)ncell);
+    int* cstart = (int*)malloc(sizeof(int) * (size_t)(ncell + 1));
+    double* cmin = (double*)malloc(sizeof(double) * 3 * (size_t)ncell);
+    double* cmax = (double*)malloc(sizeof(double) * 3 * (size_t)ncell);
+    int* cmincore = (int*)malloc(sizeof(int) * (size_t)ncell);
+    if (!ckey || !cstart || !cmin || !cmax || !cmincore) {
+        free(ord); free(cell_of); free(core); free(par); free(num);
+        free(ckey); free(cstart); free(cmin); free(cmax); free(cmincore);
+        return -3;
+    }
+    int c = -1;
+    for (int t = 0; t < ninc; ++t) {
+        if (t == 0 || ord[t].key != ord[t - 1].key) {
+            ++c;
+            ckey[c] = ord[t].key;
+            cstart[c] = t;
+            for (int k = 0; k < 3; ++k) { cmin[3 * c + k] = INFINITY; cmax[3 * c + k] = -INFINITY; }
+        }
+        int p = ord[t].idx;
+        cell_of[p] = c;
+        for (int k = 0; k < dim; ++k) {
+            double v = pts[(size_t)p * dim + k];
+            cmin[3 * c + k] = fmin(cmin[3 * c + k], v);
+            cmax[3 * c + k] = fmax(cmax[3 * c + k], v);
+        }
+    }
+    cstart[ncell] = ninc;
+    orc_grid g = {pts, dim, eps, ncell, ckey, cstart, ord, cmin, cmax};
+
+    /* core points: whole dense cells, counted neighbourhoods elsewhere */
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int t = 0; t < ninc; ++t) {
+        int p = ord[t].idx, cp = cell_of[p];
+        if (cstart[cp + 1] - cstart[cp] >= minPts) { core[p] = 1; continue; }
+        long long c0[3] = {0, 0, 0};
+        for (int k = 0; k < dim; ++k) c0[k] = (long long)floor((pts[(size_t)p * dim + k] - lo[k]) / s);
+        int nb[125];
+        int nn = orc_neighbours(&g, c0, nb);
+        long long cnt = 0;
+        for (int j = 0; j < nn && cnt < minPts; ++j) {
+            int rel = orc_box_relation(&g, p, nb[j]);
+            if (rel == 2) cnt += cstart[nb[j] + 1] - cstart[nb[j]];
+            else if (rel == 1)
+                for (int u = cstart[nb[j]]; u < cstart[nb[j] + 1] && cnt < minPts; ++u)
+                    cnt += orc_within(&g, p, ord[u].idx);
+        }
+        core[p] = cnt >= minPts;
+    }
+    /* smallest core index per cell (runs are sorted by index) */
+    for (int k = 0; k < ncell; ++k) {
+        cmincore[k] = -1;
+        for (int u = cstart[k]; u < cstart[k + 1]; ++u)
+            if (core[ord[u].idx]) { cmincore[k] = ord[u].idx; break; }
+    }
+    for (int i = 0; i < n; ++i) par[i] = i;
+    /* cores of one cell are mutually within eps */
+    for (int k = 0; k < ncell; ++k) {
+        if (cmincore[k] < 0) continue;
+        for (int u = cstart[k]; u < cstart[k + 1]; ++u)
+            if (core[ord[u].idx]) uf_union(par, ord[u].idx, cmincore[k]);
+    }
+    /* one witness pair per pair of neighbouring core cells */
+    for (int a = 0; a < ncell; ++a) {
+        if (cmincore[a] < 0) continue;
+        int pa = ord[cstart[a]].idx;
+        long long c0[3] = {0, 0, 0};
+        for (int k = 0; k < dim; ++k) c0[k] = (long long)floor((pts[(size_t)pa * dim + k] - lo[k]) / s);
+        int nb[125];
+        int nn = orc_neighbours(&g, c0, nb);
+        for (int j = 0; j < nn; ++j) {
+            int b = nb[j];
+            if (b <= a || cmincore[b] < 0) continue;
+            if (uf_find(par, cmincore[a]) == uf_find(par, cmincore[b])) continue;
+            int done = 0;
+            for (int u = cstart[a]; u < cstart[a + 1] && !done; ++u) {
+                int p = ord[u].idx;
+                if (!core[p]) continue;
+                int rel = orc_box_relation(&g, p, b);
+                if (rel == 0) continue;
+                if (rel == 2) { uf_union(par, p, cmincore[b]); done = 1; break; }
+                for (int w = cstart[b]; w < cstart[b + 1]; ++w) {
+                    int q = ord[w].idx;
+                    if (core[q] && orc_within(&g, p, q)) { uf_union(par, p, q); done = 1; break; }
+                }
+            }
+        }
+    }
+    /* clusters numbered by smallest core index */
+    int K = 0;
+    for (int i = 0; i < n; ++i)
+        if (core[i]) {
+            int r = uf_find(par, i);
+            if (num[r] == 0) num[r] = ++K;
+        }
+    for (int i = 0; i < n; ++i)
+        if (core[i]) labels[i] = num[uf_find(par, i)];
+    /* border points: the cluster of the smallest-index core neighbour */
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int t = 0; t < ninc; ++t) {
+        int p = ord[t].idx;
+        if (core[p]) continue;
+        long long c0[3] = {0, 0, 0};
+        for (int k = 0; k < dim; ++k) c0[k] = (long long)floor((pts[(size_t)p * dim + k] - lo[k]) / s);
+        int nb[125];
+        int nn = orc_neighbours(&g, c0, nb);
+        int best = n;
+        for (int j = 0; j < nn; ++j) {
+            int b = nb[j];
+            if (cmincore[b] < 0) continue;
+            int rel = orc_box_relation(&g, p, b);
+            if (rel == 0) continue;
+            if (rel == 2) { if (cmincore[b] < best) best = cmincore[b]; continue; }
+            for (int w = cstart[b]; w < cstart[b + 1]; ++w) {
+                int q = ord[w].idx;
+                if (q >= best) break;
+                if (core[q] && orc_within(&g, p, q)) { best = q; break; }
+            }
+        }
+        labels[p] = best < n ? labels[best] : 0;
+    }
+    free(ord); free(cell_of); free(core); free(par); free(num);
+    free(ckey); free(cstart); free(cmin); free(cmax); free(cmincore);
+    return K;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Legacy entry points (debri/, namespace old_library)                      */
 /* ------------------------------------------------------------------------ */

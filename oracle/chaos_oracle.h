@@ -110,6 +110,10 @@ int orc_zscore(double* f0, double* f1, double* f2, const int* flags, long long n
                double* moments /* [6] mean0,std0,mean1,std1,mean2,std2 */);
 int orc_dbscan_bruteforce(const double* pts, int n, int dim, double eps,
                           int minPts, const int* include, int* labels);
+/* Same result as orc_dbscan_bruteforce with grid-binned neighbour searches
+ * (for millions of points). */
+int orc_dbscan_grid(const double* pts, int n, int dim, double eps,
+                    int minPts, const int* include, int* labels);
 
 /* Legacy entry points (debri/, namespace old_library). */
 double orc_legacy_lle_point(int sys, int sem, int form, double tMax, double NT, double h,

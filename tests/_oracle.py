@@ -61,6 +61,7 @@ def _load():
         "orc_det_sum": (C.c_double, [_D, _I, C.c_longlong]),
         "orc_zscore": (C.c_int, [_D, _D, _D, _I, C.c_longlong, _D]),
         "orc_dbscan_bruteforce": (C.c_int, [_D, C.c_int, C.c_int, C.c_double, C.c_int, _I, _I]),
+        "orc_dbscan_grid": (C.c_int, [_D, C.c_int, C.c_int, C.c_double, C.c_int, _I, _I]),
         "orc_legacy_lle_point": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                               C.c_double, C.c_double, _D, C.c_int, _D, C.c_double,
                                               C.c_double, C.c_uint64, C.c_longlong]),
@@ -256,6 +257,17 @@ def dbscan_bruteforce(pts, eps, min_pts, include=None):
     labels = np.zeros(n, dtype=np.int32)
     k = lib.orc_dbscan_bruteforce(dptr(pts), n, dim, eps, min_pts, iptr(inc), iptr(labels))
     assert k >= 0
+    return labels
+
+
+def dbscan_grid(pts, eps, min_pts, include=None):
+    """orc_dbscan_grid: the brute force's clustering in O(n) neighbour work."""
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    n, dim = pts.shape
+    inc = None if include is None else np.ascontiguousarray(include, dtype=np.int32)
+    labels = np.zeros(n, dtype=np.int32)
+    k = lib.orc_dbscan_grid(dptr(pts), n, dim, eps, min_pts, iptr(inc), iptr(labels))
+    assert k >= 0, k
     return labels
 
 

@@ -85,10 +85,32 @@ def test_basinsOfAttraction_2_files(chaos, tmp_path):
     assert (tmp_path / "basins.csv_3.csv").read_text() == text(flags, lambda v: "%d" % v)
 
 
+def _perm_agreement(a, b):
+    """Fraction of points whose label agrees after mapping each label of a to
+    the label of b it shares most points with (labels equal up to
+    permutation); 0 (noise / divergent) maps to 0 only."""
+    a = np.asarray(a).ravel()
+    b = np.asarray(b).ravel()
+    pairs, counts = np.unique(np.stack([a, b]), axis=1, return_counts=True)
+    best = {}
+    for (la, lb), c in zip(pairs.T, counts):
+        if la not in best or c > best[la][1]:
+            best[la] = (lb, c)
+    agree = sum(c for la, (lb, c) in best.items() if (la == 0) == (lb == 0))
+    return agree / a.size
+
+
 def test_basins_full_config(chaos, oracle):
     """configs: Lorenz rho=24.5 basins on a 2048^2 IC grid, 300 + 300 t.u.,
-    DBSCAN(0.5, 20) on z-scored features.  Sampled raw features bit-exact vs
-    the oracle; the three attractors give three dominant labels."""
+    DBSCAN(0.5, 20) on z-scored features.
+    * 256 sampled points: raw features and flags bit-exact vs the oracle's
+      per-point restatement;
+    * all 4,194,304 labels: the GPU features z-scored by the oracle and
+      clustered by its grid restatement of the DBSCAN (pinned against the
+      brute force), compared up to permutation: the agreement fraction is
+      reported against the 99 % tolerance for basin labels, and the labels
+      must in fact be identical (same numbering rule on identical features).
+    """
     n = 2048
     ranges = [-20.0, 20.0, -20.0, 20.0]
     st = chaos.Stats()
@@ -97,14 +119,24 @@ def test_basins_full_config(chaos, oracle):
                                       eps=0.5, system="lorenz_rk4", semantics=1, stats=st)
     print(f"kernel {st.kernel_ms:.1f} ms analysis {st.analysis_ms:.1f} ms", flush=True)
     rng = np.random.default_rng(5)
-    for _ in range(12):
-        p = int(rng.integers(0, n * n))
+    pts = rng.choice(n * n, 256, replace=False)
+    for p in pts:
+        p = int(p)
         f0, f1, f2, fl = oracle.basins_features(oracle.SYS["lorenz_rk4"], 1, 300.0, n, 0.01, [0.0, 0.0, 25.0],
                                                 ranges, [0, 1], 0, 0.0, 300.0, LOR, 1, point_begin=p,
                                                 point_end=p + 1)
         j, i = divmod(p, n)
         assert flags[j, i] == fl[0]
         assert feat[0, j, i] == f0[0] and feat[1, j, i] == f1[0] and feat[2, j, i] == f2[0]
+    print("  256 sampled feature points bit-exact", flush=True)
+    fl = flags.ravel().astype(np.int32)
+    z0, z1, z2, _ = oracle.zscore(feat[0].ravel(), feat[1].ravel(), feat[2].ravel(), fl)
+    want = oracle.dbscan_grid(np.stack([z0, z1, z2], 1), 0.5, 20, fl)
+    frac = _perm_agreement(lab.ravel(), want)
+    print(f"  labels: {100 * frac:.4f} % agree with the oracle up to permutation; "
+          f"{int((lab.ravel() == want).sum())} of {n * n} identical", flush=True)
+    assert frac >= 0.99
+    assert np.array_equal(lab.ravel(), want)
     vals, counts = np.unique(lab, return_counts=True)
     top = np.sort(counts)[::-1]
     assert top[:3].sum() > 0.9 * n * n, dict(zip(vals.tolist(), counts.tolist()))

@@ -1,0 +1,82 @@
+"""REFERENCE semantics on the reference's own inputs: the parameter sets its
+timing harness passes (src/main.cu) at their full horizons on small grids,
+compared with the oracle at the stated tolerances.
+
+* bifurcation2D on the reference's active model, Chameleon
+  (systems/systems.cuh:9-39), src/main.cu:449-485: params {0.5,3,0,2,1,18,1},
+  IC (3,3,0), ranges [0,5] x [0,20] on params 3 and 5, writableVar 0,
+  maxValue 1e4, CT 400, TT 2000, h 0.01, eps 0.001 -- maps equal on at least
+  99.5 % of the points;
+* LLE2D on Chameleon, src/main.cu:219-246: same set, NT 0.5, eps 1e-6 --
+  exponents within 1e-3;
+* basinsOfAttraction_2 on the BASINS system, src/main.cu:26-73: params
+  {0.5,0.1,1.4,15.552,2}, IC (0,0,0), x0, y0 in [-6,6], writableVar 1,
+  maxValue 1e8, tMax 500, TT 4000, eps 0.05 -- labels equal up to
+  permutation on at least 99 % of the points.
+Chameleon and BASINS call cos / sin, whose device (ocml) and host (glibc)
+results may differ in the last bit; chaotic orbits amplify that, which is
+what the tolerances allow for.  The agreement is printed.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CHAM = dict(values=[0.5, 3, 0, 2, 1, 18, 1], ic=[3.0, 3.0, 0.0], ranges=[0.0, 5.0, 0.0, 20.0], idx=[3, 5])
+
+
+def test_bifurcation2D_chameleon_reference_params(chaos, oracle):
+    n = 24
+    c = CHAM
+    print(f"bif2d chameleon reference params {n}x{n}, CT 400, TT 2000", flush=True)
+    got = chaos.bif2d(400.0, n, 0.01, c["ic"], c["ranges"], c["idx"], 0, 10000.0, 2000.0, c["values"], 1, 0.001,
+                      system="chameleon", semantics=0)
+    want = oracle.bif2d(oracle.SYS["chameleon"], 0, 400.0, n, 0.01, c["ic"], c["ranges"], c["idx"], 0, 10000.0,
+                        2000.0, c["values"], 1, 0.001)
+    frac = float((got == want).mean())
+    print(f"  map: {100 * frac:.2f} % of {n * n} points equal; values {sorted(set(want.ravel().tolist()))[:12]}",
+          flush=True)
+    assert frac >= 0.995
+    assert len(np.unique(want)) > 2
+
+
+def test_LLE2D_chameleon_reference_params(chaos, oracle):
+    n = 16
+    c = CHAM
+    print(f"LLE2D chameleon reference params {n}x{n}, CT 400, TT 2000", flush=True)
+    got = chaos.lle2d(400.0, 0.5, n, 0.01, 1e-6, c["ic"], c["ranges"], c["idx"], 2000.0, c["values"],
+                      system="chameleon", semantics=0, seed=12345)
+    want = oracle.lle2d(oracle.SYS["chameleon"], 0, 400.0, 0.5, n, 0.01, 1e-6, c["ic"], c["ranges"], c["idx"], 2000.0,
+                        c["values"], 12345)
+    err = np.abs(got - want)
+    print(f"  LLE: max |diff| {err.max():.3g}, median {np.median(err):.3g}, range of exponents "
+          f"[{want.min():.3f}, {want.max():.3f}]", flush=True)
+    assert err.max() <= 1e-3
+
+
+def _perm_agreement(a, b):
+    a = np.asarray(a).ravel()
+    b = np.asarray(b).ravel()
+    pairs, counts = np.unique(np.stack([a, b]), axis=1, return_counts=True)
+    best = {}
+    for (la, lb), c in zip(pairs.T, counts):
+        if la not in best or c > best[la][1]:
+            best[la] = (lb, c)
+    return sum(c for la, (lb, c) in best.items() if (la == 0) == (lb == 0)) / a.size
+
+
+def test_basinsOfAttraction_2_basins_reference_params(chaos, oracle):
+    n = 24
+    vals = [0.5, 0.1, 1.4, 15.552, 2]
+    ranges = [-6.0, 6.0, -6.0, 6.0]
+    print(f"basins BASINS reference params {n}x{n}, tMax 500, TT 4000", flush=True)
+    lab, feat, flags = chaos.basins2d(500.0, n, 0.01, [0.0, 0.0, 0.0], ranges, [0, 1], 1, 1e8, 4000.0, vals, 1,
+                                      eps=0.05, system="basins", semantics=0)
+    f0, f1, _, fl = oracle.basins_features(oracle.SYS["basins"], 0, 500.0, n, 0.01, [0.0, 0.0, 0.0], ranges, [0, 1],
+                                           1, 1e8, 4000.0, vals, 1)
+    want = oracle.basins_cluster_ref(f0, f1, fl, 0.05)
+    frac = _perm_agreement(lab.ravel(), want)
+    flag_eq = float((flags.ravel() == fl).mean())
+    print(f"  labels {100 * frac:.2f} % equal up to permutation, flags {100 * flag_eq:.2f} % equal, "
+          f"{len(np.unique(want))} label values", flush=True)
+    assert frac >= 0.99

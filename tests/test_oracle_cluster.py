@@ -171,3 +171,34 @@ def test_zscore():
         assert abs(z[inc].mean()) < 1e-12 and abs(z[inc].std() - 1) < 1e-12
         assert np.array_equal(z[~inc], orig[~inc])
         assert abs(mom[2 * k] - orig[inc].mean()) < 1e-12
+
+
+def _grid_cases():
+    rng = np.random.default_rng(17)
+    cases = []
+    for dim in (2, 3):
+        # blobs + background noise, several densities around minPts
+        for n, eps, mp in ((3000, 0.3, 5), (8000, 0.15, 12), (20000, 0.1, 20)):
+            centres = rng.uniform(-3, 3, (5, dim))
+            k = rng.integers(0, 6, n)
+            pts = np.where((k < 5)[:, None], centres[np.minimum(k, 4)] + rng.normal(0, 0.3, (n, dim)),
+                           rng.uniform(-5, 5, (n, dim)))
+            inc = (rng.random(n) > 0.1).astype(np.int32)
+            cases.append((pts, eps, mp, inc))
+        # a converged basin: thousands of identical points beside a sparse cloud
+        pts = np.concatenate([np.tile(rng.uniform(-1, 1, dim), (4000, 1)), rng.uniform(-2, 2, (3000, dim))])
+        cases.append((pts, 0.25, 20, None))
+        # a lattice at spacing exactly eps (ties sit on the eps boundary)
+        g = np.arange(0, 30) * 0.5
+        mesh = np.stack(np.meshgrid(*([g] * dim), indexing="ij"), -1).reshape(-1, dim)[:20000]
+        cases.append((mesh, 0.5, 2 * dim + 1, None))
+    return cases
+
+
+@pytest.mark.parametrize("case", range(10))
+def test_grid_dbscan_equals_bruteforce(case):
+    pts, eps, mp, inc = _grid_cases()[case]
+    want = O.dbscan_bruteforce(pts, eps, mp, inc)
+    got = O.dbscan_grid(pts, eps, mp, inc)
+    assert np.array_equal(got, want)
+    assert want.max() >= 1
