@@ -70,7 +70,110 @@ struct Occupancy<6> {
     static constexpr int sweep_min = 7;
 };
 
-// systems/systems.cuh:11-39 -- Chameleon, two-stage composition with cos().
+// ---------------------------------------------------------------------------
+// Portable sin / cos / exp / atan for the reference systems that call them
+// (systems/systems.cuh:28,72-78,93,126-163).  The host and device math
+// libraries may round these differently in the last bit, and a chaotic orbit
+// turns that into a different orbit (and, where attractors coexist, into a
+// different attractor); with these routines a step gives the same bits on
+// the GPU and in the CPU oracle, which evaluates the same operations in the
+// same order.  Methods (from the mathematics; accuracy within 2 ulp of the C
+// library for sin / cos / exp, 4 ulp for atan):
+//   sin / cos: x = k pi/2 + r by Cody-Waite reduction with a three-part split
+//     of pi/2 (33 + 33 + 53 bits), Taylor series of sin r to r^19 and cos r
+//     to r^18 on |r| <= pi/4 (Horner in r^2 with fma), quadrant k mod 4;
+//   exp: x = k ln2 + r (two-part split of ln2), Taylor series to r^13, 2^k;
+//   atan: |x| > 1 through pi/2 - atan(1/|x|), two half-angle steps
+//     atan t = 2 atan(t / (1 + sqrt(1 + t^2))), Taylor series to t^25.
+// Coefficients: the series' rationals rounded to double.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline double pm_trig(double x, int cosine)
+{
+    if (!(x - x == 0.0)) return x - x;  // inf or NaN -> NaN
+    const double k = rint(x * 0x1.45f306dc9c883p-1);
+    double r = fma(-k, 0x1.921fb54400000p+0, x);
+    r = fma(-k, 0x1.0b4611a600000p-34, r);
+    r = fma(-k, 0x1.3198a2e037073p-69, r);
+    const double z = r * r;
+    double s = -0x1.2f49b46814157p-57;
+    s = fma(s, z, 0x1.952c77030ad4ap-49);
+    s = fma(s, z, -0x1.ae7f3e733b81fp-41);
+    s = fma(s, z, 0x1.6124613a86d09p-33);
+    s = fma(s, z, -0x1.ae64567f544e4p-26);
+    s = fma(s, z, 0x1.71de3a556c734p-19);
+    s = fma(s, z, -0x1.a01a01a01a01ap-13);
+    s = fma(s, z, 0x1.1111111111111p-7);
+    s = fma(s, z, -0x1.5555555555555p-3);
+    double c = -0x1.6827863b97d97p-53;
+    c = fma(c, z, 0x1.ae7f3e733b81fp-45);
+    c = fma(c, z, -0x1.93974a8c07c9dp-37);
+    c = fma(c, z, 0x1.1eed8eff8d898p-29);
+    c = fma(c, z, -0x1.27e4fb7789f5cp-22);
+    c = fma(c, z, 0x1.a01a01a01a01ap-16);
+    c = fma(c, z, -0x1.6c16c16c16c17p-10);
+    c = fma(c, z, 0x1.5555555555555p-5);
+    c = fma(c, z, -0x1.0000000000000p-1);
+    const double sr = fma(r * z, s, r);
+    const double cr = fma(z, c, 1.0);
+    const int q = ((int)(k - 4.0 * floor(k * 0.25)) + cosine) & 3;  // k mod 4, exact
+    return q == 0 ? sr : q == 1 ? cr : q == 2 ? -sr : -cr;
+}
+
+__host__ __device__ inline double pm_sin(double x) { return pm_trig(x, 0); }
+__host__ __device__ inline double pm_cos(double x) { return pm_trig(x, 1); }
+
+__host__ __device__ inline double pm_exp(double x)
+{
+    if (x != x) return x;
+    if (x > 710.0) return INFINITY;
+    if (x < -746.0) return 0.0;
+    const double k = rint(x * 0x1.71547652b82fep+0);
+    double r = fma(-k, 0x1.62e42fee00000p-1, x);
+    r = fma(-k, 0x1.a39ef35793c76p-33, r);
+    double p = 0x1.6124613a86d09p-33;
+    p = fma(p, r, 0x1.1eed8eff8d898p-29);
+    p = fma(p, r, 0x1.ae64567f544e4p-26);
+    p = fma(p, r, 0x1.27e4fb7789f5cp-22);
+    p = fma(p, r, 0x1.71de3a556c734p-19);
+    p = fma(p, r, 0x1.a01a01a01a01ap-16);
+    p = fma(p, r, 0x1.a01a01a01a01ap-13);
+    p = fma(p, r, 0x1.6c16c16c16c17p-10);
+    p = fma(p, r, 0x1.1111111111111p-7);
+    p = fma(p, r, 0x1.5555555555555p-5);
+    p = fma(p, r, 0x1.5555555555555p-3);
+    p = fma(p, r, 0x1.0000000000000p-1);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(p, (int)k);
+}
+
+__host__ __device__ inline double pm_atan(double x)
+{
+    if (x != x) return x;
+    double a = fabs(x);
+    const bool inv = a > 1.0;
+    if (inv) a = 1.0 / a;
+    double t = a / (1.0 + sqrt(fma(a, a, 1.0)));
+    t = t / (1.0 + sqrt(fma(t, t, 1.0)));
+    const double z = t * t;
+    double p = 0x1.47ae147ae147bp-5;
+    p = fma(p, z, -0x1.642c8590b2164p-5);
+    p = fma(p, z, 0x1.8618618618618p-5);
+    p = fma(p, z, -0x1.af286bca1af28p-5);
+    p = fma(p, z, 0x1.e1e1e1e1e1e1ep-5);
+    p = fma(p, z, -0x1.1111111111111p-4);
+    p = fma(p, z, 0x1.3b13b13b13b14p-4);
+    p = fma(p, z, -0x1.745d1745d1746p-4);
+    p = fma(p, z, 0x1.c71c71c71c71cp-4);
+    p = fma(p, z, -0x1.2492492492492p-3);
+    p = fma(p, z, 0x1.999999999999ap-3);
+    p = fma(p, z, -0x1.5555555555555p-2);
+    double v = 4.0 * fma(t * z, p, t);
+    if (inv) v = (0x1.921fb54442d18p+0 - v) + 0x1.1a62633145c07p-54;
+    return x < 0 ? -v : v;
+}
+
+// systems/systems.cuh:11-39 -- Chameleon, two-stage composition with pm_cos().
 template <>
 struct System<SYS_CHAMELEON> {
     static constexpr int NX = 3, NA = 7;
@@ -80,7 +183,7 @@ struct System<SYS_CHAMELEON> {
         const double h2 = (1 - a[0]) * c.h;
         X[0] = fma(h1, -a[6] * X[1], X[0]);
         X[1] = fma(h1, a[6] * X[0] + a[1] * X[2], X[1]);
-        const double cos_term = cos(a[5] * X[1]);
+        const double cos_term = pm_cos(a[5] * X[1]);
         X[2] = fma(h1, a[2] - a[3] * X[2] + a[4] * cos_term, X[2]);
         X[2] = fma(h2, (a[2] + a[4] * cos_term), X[2]) / (1 + a[3] * h2);
         X[1] = fma(h2, (a[6] * X[0] + a[1] * X[2]), X[1]);
@@ -116,12 +219,12 @@ struct System<SYS_BASINS> {
     {
         const double h = c.h;
         const double h2 = (double)(float)(h * (1 - a[0]));
-        X[0] = X[0] + h * (sin(X[1]) - a[1] * X[0]);
-        X[1] = X[1] + h * (sin(X[2]) - a[1] * X[1]);
-        X[2] = X[2] + h * (sin(X[0]) - a[1] * X[2]);
-        X[2] = (X[2] + h2 * sin(X[0])) / (1 + h2 * a[1]);
-        X[1] = (X[1] + h2 * sin(X[2])) / (1 + h2 * a[1]);
-        X[0] = (X[0] + h2 * sin(X[1])) / (1 + h2 * a[1]);
+        X[0] = X[0] + h * (pm_sin(X[1]) - a[1] * X[0]);
+        X[1] = X[1] + h * (pm_sin(X[2]) - a[1] * X[1]);
+        X[2] = X[2] + h * (pm_sin(X[0]) - a[1] * X[2]);
+        X[2] = (X[2] + h2 * pm_sin(X[0])) / (1 + h2 * a[1]);
+        X[1] = (X[1] + h2 * pm_sin(X[2])) / (1 + h2 * a[1]);
+        X[0] = (X[0] + h2 * pm_sin(X[1])) / (1 + h2 * a[1]);
     }
 };
 
@@ -134,9 +237,9 @@ struct System<SYS_BASINS_2> {
         const double h1 = (double)(float)(c.h * a[0]);
         const double h2 = (double)(float)(c.h * (1 - a[0]));
         X[0] = X[0] + h1 * (-X[1]);
-        X[1] = X[1] + h1 * (a[1] * X[0] + sin(X[1]));
+        X[1] = X[1] + h1 * (a[1] * X[0] + pm_sin(X[1]));
         const double z = (double)(float)X[1];
-        X[1] = z + h2 * (a[1] * X[0] + sin(X[1]));
+        X[1] = z + h2 * (a[1] * X[0] + pm_sin(X[1]));
         X[0] = X[0] + h2 * (-X[1]);
     }
 };
@@ -155,9 +258,9 @@ struct System<SYS_DAMIR> {
         const double Iv = -3e-6, D = 26.0, E = 0.14;
         const double Vd = Vc + U2;
         const double Vm = Vc + U1;
-        const double idiode = Is * (exp(Vd / Vt) - exp(-Vd / Vt));
-        const double itunnel = Ip / Vp * Vd * exp(-(Vd - Vp) / Vp);
-        const double iex = Iv * (atan(D * (Vd - E)) + atan(D * (Vd + E)));
+        const double idiode = Is * (pm_exp(Vd / Vt) - pm_exp(-Vd / Vt));
+        const double itunnel = Ip / Vp * Vd * pm_exp(-(Vd - Vp) / Vp);
+        const double iex = Iv * (pm_atan(D * (Vd - E)) + pm_atan(D * (Vd + E)));
         const double Id = idiode + itunnel + iex;
         const double Ron = 1434.0, Roff = 1e6, Von1 = 0.28, Voff1 = 0.14;
         const double Von2 = -0.12, Voff2 = -0.006, TAU = 1e-7, T = 2.0;
@@ -165,11 +268,11 @@ struct System<SYS_DAMIR> {
         const double g = XSV / Ron + (1.0 - XSV) / Roff;
         const double arg1 = -1.0 / (T * boltz / echarge) * (Vm - Von1) * (Vm - Von2);
         const double arg2 = -1.0 / (T * boltz / echarge) * (Vm - Voff2) * (Vm - Voff1);
-        const double f1 = 1.0 / (1.0 + exp(arg1));
-        const double f2 = 1.0 / (1.0 + exp(arg2));
+        const double f1 = 1.0 / (1.0 + pm_exp(arg1));
+        const double f2 = 1.0 / (1.0 + pm_exp(arg2));
         const double Ix = (1.0 / TAU) * (f1 * (1.0 - XSV) - (1.0 - f2) * XSV);
         const double Imem = Vm * g;
-        const double Iin = a[0] * sin(a[1] * t);
+        const double Iin = a[0] * pm_sin(a[1] * t);
         const double dVc = (Iin - Imem - Id) / C1;
         X[0] = Vc + h * dVc;
         X[1] = XSV + h * Ix;

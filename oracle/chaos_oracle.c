@@ -21,6 +21,116 @@
 #define ORC_MAX_NX 12
 
 /* ------------------------------------------------------------------------ */
+/* Portable transcendentals                                                  */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * sin, cos, exp and atan for the reference systems that call them
+ * (systems/systems.cuh:28,72-78,93,126-163).  The kernels evaluate the same
+ * operations in the same order (chaoslibrary_amd/csrc/systems.hpp), so a
+ * step of these systems gives the same bits on the host and the device,
+ * where the platforms' own libraries may differ in the last bit (and chaotic
+ * orbits amplify that).  Methods, all from the mathematics:
+ *   sin / cos: Cody-Waite reduction x = k pi/2 + r with a three-part split
+ *     of pi/2 (33 + 33 + 53 bits, each k * part exact for |k| < 2^20), then
+ *     the Taylor series of sin r (to r^19) and cos r (to r^18) on
+ *     |r| <= pi/4, Horner in r^2 with fma, quadrant by k mod 4;
+ *   exp: x = k ln2 + r (two-part split of ln2), Taylor series to r^13 on
+ *     |r| <= ln2/2, scaled by 2^k;
+ *   atan: |x| > 1 through pi/2 - atan(1/|x|), two half-angle steps
+ *     atan t = 2 atan(t / (1 + sqrt(1 + t^2))) to |t| <= tan(pi/16), then the
+ *     Taylor series to t^25.
+ * Coefficients are the series' rationals rounded to double.  Pinned by:
+ * tests/test_oracle_math.py (within 2 ulp of the C library for sin / cos /
+ * exp, 4 ulp for atan, over the arguments the systems meet).
+ */
+static double pm_trig(double x, int cosine)
+{
+    if (!(x - x == 0.0)) return x - x;                   /* inf or NaN -> NaN */
+    const double k = rint(x * 0x1.45f306dc9c883p-1);     /* nearest multiple of pi/2 */
+    double r = fma(-k, 0x1.921fb54400000p+0, x);
+    r = fma(-k, 0x1.0b4611a600000p-34, r);
+    r = fma(-k, 0x1.3198a2e037073p-69, r);
+    const double z = r * r;
+    double s = -0x1.2f49b46814157p-57;                    /* -1/19! */
+    s = fma(s, z, 0x1.952c77030ad4ap-49);                 /*  1/17! */
+    s = fma(s, z, -0x1.ae7f3e733b81fp-41);                /* -1/15! */
+    s = fma(s, z, 0x1.6124613a86d09p-33);                 /*  1/13! */
+    s = fma(s, z, -0x1.ae64567f544e4p-26);                /* -1/11! */
+    s = fma(s, z, 0x1.71de3a556c734p-19);                 /*  1/9!  */
+    s = fma(s, z, -0x1.a01a01a01a01ap-13);                /* -1/7!  */
+    s = fma(s, z, 0x1.1111111111111p-7);                  /*  1/5!  */
+    s = fma(s, z, -0x1.5555555555555p-3);                 /* -1/3!  */
+    double c = -0x1.6827863b97d97p-53;                    /* -1/18! */
+    c = fma(c, z, 0x1.ae7f3e733b81fp-45);                 /*  1/16! */
+    c = fma(c, z, -0x1.93974a8c07c9dp-37);                /* -1/14! */
+    c = fma(c, z, 0x1.1eed8eff8d898p-29);                 /*  1/12! */
+    c = fma(c, z, -0x1.27e4fb7789f5cp-22);                /* -1/10! */
+    c = fma(c, z, 0x1.a01a01a01a01ap-16);                 /*  1/8!  */
+    c = fma(c, z, -0x1.6c16c16c16c17p-10);                /* -1/6!  */
+    c = fma(c, z, 0x1.5555555555555p-5);                  /*  1/4!  */
+    c = fma(c, z, -0x1.0000000000000p-1);                 /* -1/2!  */
+    const double sr = fma(r * z, s, r);
+    const double cr = fma(z, c, 1.0);
+    const int q = ((int)(k - 4.0 * floor(k * 0.25)) + cosine) & 3;  /* k mod 4, exact */
+    return q == 0 ? sr : q == 1 ? cr : q == 2 ? -sr : -cr;
+}
+
+double orc_sin(double x) { return pm_trig(x, 0); }
+double orc_cos(double x) { return pm_trig(x, 1); }
+
+double orc_exp(double x)
+{
+    if (x != x) return x;
+    if (x > 710.0) return INFINITY;
+    if (x < -746.0) return 0.0;
+    const double k = rint(x * 0x1.71547652b82fep+0);     /* nearest multiple of ln2 */
+    double r = fma(-k, 0x1.62e42fee00000p-1, x);
+    r = fma(-k, 0x1.a39ef35793c76p-33, r);
+    double p = 0x1.6124613a86d09p-33;                     /* 1/13! */
+    p = fma(p, r, 0x1.1eed8eff8d898p-29);                 /* 1/12! */
+    p = fma(p, r, 0x1.ae64567f544e4p-26);                 /* 1/11! */
+    p = fma(p, r, 0x1.27e4fb7789f5cp-22);                 /* 1/10! */
+    p = fma(p, r, 0x1.71de3a556c734p-19);                 /* 1/9!  */
+    p = fma(p, r, 0x1.a01a01a01a01ap-16);                 /* 1/8!  */
+    p = fma(p, r, 0x1.a01a01a01a01ap-13);                 /* 1/7!  */
+    p = fma(p, r, 0x1.6c16c16c16c17p-10);                 /* 1/6!  */
+    p = fma(p, r, 0x1.1111111111111p-7);                  /* 1/5!  */
+    p = fma(p, r, 0x1.5555555555555p-5);                  /* 1/4!  */
+    p = fma(p, r, 0x1.5555555555555p-3);                  /* 1/3!  */
+    p = fma(p, r, 0x1.0000000000000p-1);                  /* 1/2!  */
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(p, (int)k);
+}
+
+double orc_atan(double x)
+{
+    if (x != x) return x;
+    double a = fabs(x);
+    const int inv = a > 1.0;
+    if (inv) a = 1.0 / a;
+    double t = a / (1.0 + sqrt(fma(a, a, 1.0)));
+    t = t / (1.0 + sqrt(fma(t, t, 1.0)));
+    const double z = t * t;
+    double p = 0x1.47ae147ae147bp-5;                      /*  1/25 */
+    p = fma(p, z, -0x1.642c8590b2164p-5);                 /* -1/23 */
+    p = fma(p, z, 0x1.8618618618618p-5);                  /*  1/21 */
+    p = fma(p, z, -0x1.af286bca1af28p-5);                 /* -1/19 */
+    p = fma(p, z, 0x1.e1e1e1e1e1e1ep-5);                  /*  1/17 */
+    p = fma(p, z, -0x1.1111111111111p-4);                 /* -1/15 */
+    p = fma(p, z, 0x1.3b13b13b13b14p-4);                  /*  1/13 */
+    p = fma(p, z, -0x1.745d1745d1746p-4);                 /* -1/11 */
+    p = fma(p, z, 0x1.c71c71c71c71cp-4);                  /*  1/9  */
+    p = fma(p, z, -0x1.2492492492492p-3);                 /* -1/7  */
+    p = fma(p, z, 0x1.999999999999ap-3);                  /*  1/5  */
+    p = fma(p, z, -0x1.5555555555555p-2);                 /* -1/3  */
+    double v = 4.0 * fma(t * z, p, t);
+    if (inv) v = (0x1.921fb54442d18p+0 - v) + 0x1.1a62633145c07p-54;  /* pi/2 - v */
+    return x < 0 ? -v : v;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Systems (L1 plug-in)                                                      */
 /* ------------------------------------------------------------------------ */
 
@@ -42,7 +152,7 @@ static void step_chameleon(double* X, const double* a, double h)
     double h2 = (1 - a[0]) * h;
     X[0] = fma(h1, -a[6] * X[1], X[0]);
     X[1] = fma(h1, a[6] * X[0] + a[1] * X[2], X[1]);
-    double cos_term = cos(a[5] * X[1]);
+    double cos_term = orc_cos(a[5] * X[1]);
     X[2] = fma(h1, a[2] - a[3] * X[2] + a[4] * cos_term, X[2]);
     X[2] = fma(h2, (a[2] + a[4] * cos_term), X[2]) / (1 + a[3] * h2);
     X[1] = fma(h2, (a[6] * X[0] + a[1] * X[2]), X[1]);
@@ -71,12 +181,12 @@ static void step_rossler_cd(double* x, const double* a, double h)
 static void step_basins(double* X, const double* a, double h)
 {
     float h2 = (float)(h * (1 - a[0]));
-    X[0] = X[0] + h * (sin(X[1]) - a[1] * X[0]);
-    X[1] = X[1] + h * (sin(X[2]) - a[1] * X[1]);
-    X[2] = X[2] + h * (sin(X[0]) - a[1] * X[2]);
-    X[2] = (X[2] + (double)h2 * sin(X[0])) / (1 + (double)h2 * a[1]);
-    X[1] = (X[1] + (double)h2 * sin(X[2])) / (1 + (double)h2 * a[1]);
-    X[0] = (X[0] + (double)h2 * sin(X[1])) / (1 + (double)h2 * a[1]);
+    X[0] = X[0] + h * (orc_sin(X[1]) - a[1] * X[0]);
+    X[1] = X[1] + h * (orc_sin(X[2]) - a[1] * X[1]);
+    X[2] = X[2] + h * (orc_sin(X[0]) - a[1] * X[2]);
+    X[2] = (X[2] + (double)h2 * orc_sin(X[0])) / (1 + (double)h2 * a[1]);
+    X[1] = (X[1] + (double)h2 * orc_sin(X[2])) / (1 + (double)h2 * a[1]);
+    X[0] = (X[0] + (double)h2 * orc_sin(X[1])) / (1 + (double)h2 * a[1]);
 }
 
 /* systems/systems.cuh:85-101 (USE_SYSTEM_FOR_BASINS_2); h1, h2, z are floats.
@@ -86,9 +196,9 @@ static void step_basins2(double* X, const double* a, double h)
     float h1 = (float)(h * a[0]);
     float h2 = (float)(h * (1 - a[0]));
     X[0] = X[0] + (double)h1 * (-X[1]);
-    X[1] = X[1] + (double)h1 * (a[1] * X[0] + sin(X[1]));
+    X[1] = X[1] + (double)h1 * (a[1] * X[0] + orc_sin(X[1]));
     float z = (float)X[1];
-    X[1] = (double)z + (double)h2 * (a[1] * X[0] + sin(X[1]));
+    X[1] = (double)z + (double)h2 * (a[1] * X[0] + orc_sin(X[1]));
     X[0] = X[0] + (double)h2 * (-X[1]);
 }
 
@@ -103,9 +213,9 @@ static void step_damir(double* X, const double* a, double h)
     double Iv = -3e-6, D = 26.0, E = 0.14;
     double Vd = Vc + U2;
     double Vm = Vc + U1;
-    double idiode = Is * (exp(Vd / Vt) - exp(-Vd / Vt));
-    double itunnel = Ip / Vp * Vd * exp(-(Vd - Vp) / Vp);
-    double iex = Iv * (atan(D * (Vd - E)) + atan(D * (Vd + E)));
+    double idiode = Is * (orc_exp(Vd / Vt) - orc_exp(-Vd / Vt));
+    double itunnel = Ip / Vp * Vd * orc_exp(-(Vd - Vp) / Vp);
+    double iex = Iv * (orc_atan(D * (Vd - E)) + orc_atan(D * (Vd + E)));
     double Id = idiode + itunnel + iex;
     double Ron = 1434.0, Roff = 1e6, Von1 = 0.28, Voff1 = 0.14;
     double Von2 = -0.12, Voff2 = -0.006, TAU = 1e-7, T = 2.0;
@@ -113,13 +223,13 @@ static void step_damir(double* X, const double* a, double h)
     double g = XSV / Ron + (1.0 - XSV) / Roff;
     double arg1 = -1.0 / (T * boltz / echarge) * (Vm - Von1) * (Vm - Von2);
     double arg2 = -1.0 / (T * boltz / echarge) * (Vm - Voff2) * (Vm - Voff1);
-    double exp1 = exp(arg1);
-    double exp2 = exp(arg2);
+    double exp1 = orc_exp(arg1);
+    double exp2 = orc_exp(arg2);
     double f1 = 1.0 / (1.0 + exp1);
     double f2 = 1.0 / (1.0 + exp2);
     double Ix = (1.0 / TAU) * (f1 * (1.0 - XSV) - (1.0 - f2) * XSV);
     double Imem = Vm * g;
-    double Iin = a[0] * sin(a[1] * t);
+    double Iin = a[0] * orc_sin(a[1] * t);
     double dVc = (Iin - Imem - Id) / C1;
     double dXSV = Ix;
     X[0] = Vc + h * dVc;

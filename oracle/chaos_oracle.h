@@ -42,6 +42,13 @@ enum { ORC_SEM_REFERENCE = 0, ORC_SEM_CONFIG = 1 };
 /* Divergence / fixed-point check profiles (see orc_loop). */
 enum { ORC_CHK_BIF_REF = 0, ORC_CHK_BASINS_REF = 1, ORC_CHK_CONFIG = 2 };
 
+/* Portable sin / cos / exp / atan the reference systems' steps use (the
+ * kernels evaluate the same operations; see chaos_oracle.c). */
+double orc_sin(double x);
+double orc_cos(double x);
+double orc_exp(double x);
+double orc_atan(double x);
+
 int  orc_dims(int sys, int* nx, int* na);
 void orc_step(int sys, double* X, const double* a, double h);
 void orc_steps(int sys, double* X, const double* a, double h, long long n);

@@ -26,6 +26,10 @@ def _load():
     lib = C.CDLL(LIB_PATH)
     sig = {
         "orc_dims": (C.c_int, [C.c_int, _I, _I]),
+        "orc_sin": (C.c_double, [C.c_double]),
+        "orc_cos": (C.c_double, [C.c_double]),
+        "orc_exp": (C.c_double, [C.c_double]),
+        "orc_atan": (C.c_double, [C.c_double]),
         "orc_step": (None, [C.c_int, _D, _D, C.c_double]),
         "orc_steps": (None, [C.c_int, _D, _D, C.c_double, C.c_longlong]),
         "orc_integrate": (None, [C.c_int, C.c_int, _D, _D, C.c_longlong, C.c_double]),

@@ -1,7 +1,7 @@
 """GPU parity of the per-step maps through the C-ABI (chaos_integrate)
-against the oracle: bit-exact over 10^4 steps for the transcendental-free
-maps (RK4 Rossler/Lorenz, Rossler-CD); within tolerance for the maps that
-call cos/sin/exp/atan (device libm vs host libm)."""
+against the oracle: bit-exact over 10^4 steps for every system, including
+the reference maps that call cos / sin / exp / atan (both sides evaluate the
+library's portable routines, operation for operation)."""
 import numpy as np
 import pytest
 
@@ -37,13 +37,13 @@ def test_trajectories_bit_exact_1e4_steps(chaos, oracle, system, fixed, ranges, 
     assert same.all(), f"{(~same.all(1)).sum()} of {n} trajectories differ"
 
 
-@pytest.mark.parametrize("system,params,ic,h,steps,rtol", [
-    ("chameleon", (0.5, 3.0, 0.0, 2.0, 1.0, 18.0, 1.0), (3.0, 3.0, 0.0), 0.01, 200, 1e-9),
-    ("basins", (0.5, 0.1, 1.4, 15.552, 2.0), (0.3, -0.2, 0.1), 0.01, 200, 1e-9),
-    ("basins_2", (0.5, 0.1), (0.3, -0.2), 0.01, 200, 1e-6),
-    ("damir", (1e-5, 1e5), (0.1, 0.5, 0.0), 1e-10, 200, 1e-9),
+@pytest.mark.parametrize("system,params,ic,h", [
+    ("chameleon", (0.5, 3.0, 0.0, 2.0, 1.0, 18.0, 1.0), (3.0, 3.0, 0.0), 0.01),
+    ("basins", (0.5, 0.1, 1.4, 15.552, 2.0), (0.3, -0.2, 0.1), 0.01),
+    ("basins_2", (0.5, 0.1), (0.3, -0.2), 0.01),
+    ("damir", (1e-5, 1e5), (0.1, 0.5, 0.0), 1e-10),
 ])
-def test_transcendental_maps_close(chaos, oracle, system, params, ic, h, steps, rtol):
+def test_transcendental_maps_bit_exact_1e4_steps(chaos, oracle, system, params, ic, h):
     sid = oracle.SYS[system]
     n = 256
     rng = np.random.default_rng(5)
@@ -51,10 +51,11 @@ def test_transcendental_maps_close(chaos, oracle, system, params, ic, h, steps, 
     X = np.asarray(ic, dtype=float) + rng.uniform(-0.05, 0.05, (n, len(ic)))
     if system == "damir":
         X[:, 2] = 0.0
-    print(f"integrate {system} n={n} steps={steps}", flush=True)
-    got = chaos.integrate(system, P, X, steps, h)
-    want = oracle.integrate(sid, P, X, steps, h)
-    np.testing.assert_allclose(got, want, rtol=rtol, atol=1e-12)
+    print(f"integrate {system} n={n} steps=10000", flush=True)
+    got = chaos.integrate(system, P, X, 10000, h)
+    want = oracle.integrate(sid, P, X, 10000, h)
+    same = (got.view(np.uint64) == want.view(np.uint64))
+    assert same.all(), f"{(~same.all(1)).sum()} of {n} trajectories differ"
 
 
 def test_zero_steps_and_empty(chaos):
