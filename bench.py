@@ -218,6 +218,20 @@ def run_lle(args):
     return res
 
 
+def engine_params(args):
+    """Engine parameters: the library defaults, then --spl, then --engine
+    "key=value,..." (block, waves_per_simd, steps_per_launch,
+    points_per_tile, use_graph, max_jobs, max_tiles)."""
+    import chaoslibrary_amd as ch
+    prm = ch.engine_params()
+    if args.spl:
+        prm.steps_per_launch = args.spl
+    for kv in filter(None, args.engine.split(",")):
+        k, v = kv.split("=")
+        setattr(prm, k.strip(), int(v))
+    return prm
+
+
 def stream_supplement(args):
     """The other half of the metric: the served mixed-job stream (Poisson
     arrivals at 200 jobs/s, the configurations' kinds and size ranges) served
@@ -226,7 +240,7 @@ def stream_supplement(args):
     full horizons offers ~5.5x the GPU's FP64 peak and only measures the
     queue."""
     import chaoslibrary_amd as ch
-    prm = ch.engine_params()
+    prm = engine_params(args)
     out = {}
     for mode, name in ((0, "batched"), (1, "per_job")):
         cfg = ch.stream_config(lambda_=200.0, duration_s=args.stream_duration, horizon_scale=args.stream_scale,
@@ -380,7 +394,7 @@ def run_ls(args):
 def run_stream(args):
     import numpy as np
     import chaoslibrary_amd as ch
-    prm = ch.engine_params(steps_per_launch=args.spl) if args.spl else ch.engine_params()
+    prm = engine_params(args)
     if args.tune:
         prm, rate = ch.autotune()
         log(f"autotune: block {prm.block} waves {prm.waves_per_simd} steps {prm.steps_per_launch} ({rate:.3e} steps/s)")
@@ -404,7 +418,9 @@ def run_stream(args):
            "config": {"workload": "served_stream", "lambda": args.lam, "duration_s": args.duration,
                       "horizon_scale": args.scale, "engine": {"block": prm.block,
                                                               "waves_per_simd": prm.waves_per_simd,
-                                                              "steps_per_launch": prm.steps_per_launch}},
+                                                              "steps_per_launch": prm.steps_per_launch,
+                                                              "points_per_tile": prm.points_per_tile,
+                                                              "use_graph": prm.use_graph}},
            "batched": out.get("batched"), "per_job": out.get("per_job"),
            "throughput_ratio": out["batched"]["steps_per_s"] / out["per_job"]["steps_per_s"] if both else None,
            "p99_ratio": out["batched"]["p99_ms"] / out["per_job"]["p99_ms"] if both else None}
@@ -625,6 +641,7 @@ def main():
     ap.add_argument("--duration", type=float, default=60.0, help="stream arrival window (s)")
     ap.add_argument("--scale", type=float, default=1.0, help="stream horizon scale")
     ap.add_argument("--spl", type=int, default=0, help="engine steps per launch (0 = default)")
+    ap.add_argument("--engine", default="", help='engine parameters "key=value,..." over the defaults')
     ap.add_argument("--tune", action="store_true", help="autotune the engine before the stream")
     ap.add_argument("--mode", default="both", choices=["both", "batched", "per_job"], help="stream modes to run")
     ap.add_argument("--bytes", type=int, default=1 << 30)

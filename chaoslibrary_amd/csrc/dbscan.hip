@@ -25,7 +25,8 @@
 // serve the allocating entry point and chaos_dbscan_noalloc.
 #include <climits>
 #include <cmath>
-#include <mutex>
+#include <cstdio>
+#include <cstdlib>
 
 #include "dbscan.hpp"
 
@@ -1150,6 +1151,10 @@ int dbscan_device(CallCtx& ctx, const double* const* pts, const int* include, lo
         const DbCtl c = run_pipeline(ctx, G, prm, labels, box);
         if (c.flags & DB_NONFINITE) fail(CHAOS_E_INVALID, "non-finite feature values");
         if (c.flags & DB_RANGE) fail(CHAOS_E_UNSUPPORTED, "feature range too wide for eps %.3g", prm.eps);
+        if (std::getenv("CHAOS_DBSCAN_TRACE"))
+            std::fprintf(stderr, "dbscan: %lld points, %d cells of %lld slots, %d sparse-cell points, "
+                         "%d core cells, %d clusters, flags %d\n", n, c.ncells, H, c.ncand, c.ncore_cells, c.K,
+                         c.flags);
         if (!(c.flags & DB_FULL)) return c.K;
         if (H >= dbscan_max_cells(n)) fail(CHAOS_E_HIP, "cell table full at %lld cells", H);
         H = dbscan_max_cells(n);
@@ -1212,21 +1217,35 @@ __global__ __launch_bounds__(512) void moment_block_kernel(const double* x0, con
 __global__ void moment_final_kernel(const double* __restrict__ part, const int* __restrict__ cnt, long long nb, int k,
                                     int pass, double* __restrict__ mom)
 {
+    // the count of masked points (an integer sum: any order), by the block
+    __shared__ long long s_n[256];
     const int j = threadIdx.x;
-    if (j > k) return;
-    if (j == k) {
-        if (pass == 0) {
-            long long N = 0;
-            for (long long b = 0; b < nb; ++b) N += cnt[b];
-            mom[2 * k] = (double)N;
-        }
-        return;
+    long long c = 0;
+    if (pass == 0)
+        for (long long b = j; b < nb; b += blockDim.x) c += cnt[b];
+    s_n[j] = c;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (j < o) s_n[j] += s_n[j + o];
+        __syncthreads();
     }
+    if (pass == 0 && j == 0) mom[2 * k] = (double)s_n[0];
+    if (j >= k) return;
+    // block sums added strictly in block order; the loads of 16 blocks are
+    // issued together so the dependent adds do not wait on each load
     double tot = 0.0;
-    for (long long b = 0; b < nb; ++b) tot = tot + part[j * nb + b];
+    const double* pj = part + j * nb;
+    long long b = 0;
+    for (; b + 16 <= nb; b += 16) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = pj[b + u];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) tot = tot + v[u];
+    }
+    for (; b < nb; ++b) tot = tot + pj[b];
     if (pass == 0) {
-        long long N = 0;
-        for (long long b = 0; b < nb; ++b) N += cnt[b];
+        const long long N = s_n[0];
         mom[2 * j] = N > 0 ? tot / (double)N : 0.0;
     } else {
         const double N = mom[2 * k];
@@ -1241,31 +1260,37 @@ __global__ __launch_bounds__(256) void normalize_kernel(const double* x0, const 
                                                        long long n, int k, const double* __restrict__ mom,
                                                        unsigned long long* box)
 {
+    // grid-stride: a few thousand blocks, so the box takes a few thousand
+    // atomics per address rather than one per 256 points
     __shared__ unsigned long long smin[3][4], smax[3][4];
     const double* xs[3] = {x0, x1, x2};
     double* ys[3] = {y0, y1, y2};
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool any = mom[2 * k] > 0;
-    const bool in = i < n && (!mask || mask[i] != 0);
     const int lane = __lane_id(), wave = threadIdx.x >> 6;
-    for (int j = 0; j < k; ++j) {
-        unsigned long long e = 0;
-        if (i < n) {
+    unsigned long long mn[3] = {~0ull, ~0ull, ~0ull}, mx[3] = {0ull, 0ull, 0ull};
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const bool in = !mask || mask[i] != 0;
+        for (int j = 0; j < k; ++j) {
             const double v = xs[j][i];
             const double mean = mom[2 * j], sd = mom[2 * j + 1];
             const double z = (!any || !in) ? v : (sd > 0 ? (v - mean) / sd : 0.0);
             ys[j][i] = z;
-            e = ord_enc(z);
-        }
-        if (box) {
-            const unsigned long long mn = wave_min_u64(in ? e : ~0ull), mx = wave_max_u64(in ? e : 0ull);
-            if (lane == 0) {
-                smin[j][wave] = mn;
-                smax[j][wave] = mx;
+            if (in) {
+                const unsigned long long e = ord_enc(z);
+                mn[j] = e < mn[j] ? e : mn[j];
+                mx[j] = e > mx[j] ? e : mx[j];
             }
         }
     }
     if (!box) return;
+    for (int j = 0; j < k; ++j) {
+        const unsigned long long a = wave_min_u64(mn[j]), b = wave_max_u64(mx[j]);
+        if (lane == 0) {
+            smin[j][wave] = a;
+            smax[j][wave] = b;
+        }
+    }
     __syncthreads();
     if (threadIdx.x < k) {
         const int j = threadIdx.x;
@@ -1313,10 +1338,11 @@ void zscore_device(CallCtx& ctx, const double* const* src, double* const* dst, i
     for (int pass = 0; pass < 2; ++pass) {
         hipLaunchKernelGGL(moment_block_kernel, dim3((unsigned)nb), dim3(512), 0, st, s[0], s[1], s[2], mask, n, k,
                            pass, mom, part, cnt);
-        hipLaunchKernelGGL(moment_final_kernel, dim3(1), dim3(64), 0, st, part, cnt, nb, k, pass, mom);
+        hipLaunchKernelGGL(moment_final_kernel, dim3(1), dim3(256), 0, st, part, cnt, nb, k, pass, mom);
     }
     if (box) hipLaunchKernelGGL(box_init_kernel, dim3(1), dim3(64), 0, st, box);
-    hipLaunchKernelGGL(normalize_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, s[0], s[1], s[2], d[0], d[1], d[2],
+    hipLaunchKernelGGL(normalize_kernel, dim3(std::min<unsigned>(ceil_div(n, 256), 2048u)), dim3(256), 0, st, s[0],
+                       s[1], s[2], d[0], d[1], d[2],
                        mask, n, k, mom, box);
     CHAOS_HIP(hipGetLastError());
     ctx.add_launch(5 + (box ? 1 : 0));
