@@ -218,12 +218,22 @@ def run_lle(args):
     return res
 
 
+ENGINE_TUNED = ("block", "waves_per_simd", "steps_per_launch", "points_per_tile", "use_graph")
+
+
 def engine_params(args):
-    """Engine parameters: the library defaults, then --spl, then --engine
+    """Engine parameters: the library defaults, overridden by the autotuner's
+    choice recorded in profiles/engine_autotune_calibration_mix.md (lines
+    `tuned_<field>: value`; --untuned skips them), then --spl, then --engine
     "key=value,..." (block, waves_per_simd, steps_per_launch,
     points_per_tile, use_graph, max_jobs, max_tiles)."""
     import chaoslibrary_amd as ch
     prm = ch.engine_params()
+    if not getattr(args, "untuned", False):
+        for k in ENGINE_TUNED:
+            v = profile_figure("engine_autotune_calibration_mix", "tuned_" + k)
+            if v is not None:
+                setattr(prm, k, int(v))
     if args.spl:
         prm.steps_per_launch = args.spl
     for kv in filter(None, args.engine.split(",")):
@@ -642,6 +652,7 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="stream horizon scale")
     ap.add_argument("--spl", type=int, default=0, help="engine steps per launch (0 = default)")
     ap.add_argument("--engine", default="", help='engine parameters "key=value,..." over the defaults')
+    ap.add_argument("--untuned", action="store_true", help="engine: library defaults, not the recorded autotune")
     ap.add_argument("--tune", action="store_true", help="autotune the engine before the stream")
     ap.add_argument("--mode", default="both", choices=["both", "batched", "per_job"], help="stream modes to run")
     ap.add_argument("--bytes", type=int, default=1 << 30)
