@@ -282,29 +282,6 @@ __device__ __forceinline__ int box_relation(const GridDev& G, long long p, int s
     return 1;
 }
 
-// Visit every cell of the 5^dim block around p's cell that may hold a point
-// within eps: f(slot).  Stops early when f returns false.
-template <class F>
-__device__ __forceinline__ void for_neighbour_cells(const GridDev& G, long long p, F&& f)
-{
-    long long c0[3] = {0, 0, 0}, c[3] = {0, 0, 0};
-    cell_coords(G, p, c0);
-    const int r2 = G.dim == 3 ? 2 : 0;
-    for (int dz = -r2; dz <= r2; ++dz)
-        for (int dy = -2; dy <= 2; ++dy)
-            for (int dx = -2; dx <= 2; ++dx) {
-                c[0] = c0[0] + dx;
-                c[1] = c0[1] + dy;
-                c[2] = c0[2] + dz;
-                const bool ok = c[0] >= 0 && c[1] >= 0 && (G.dim == 2 || c[2] >= 0);
-                if (!ok) continue;
-                if (box_gap2(G, p, c) > G.prune2) continue;
-                const int slot = lookup(G, pack_cell(G.dim, c));
-                if (slot < 0) continue;
-                if (!f(slot)) return;
-            }
-}
-
 __device__ __forceinline__ int uf_load(const int* a) { return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
 __device__ __forceinline__ int uf_find(const GridDev& G, int x)
@@ -348,6 +325,12 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
         const unsigned long long w = __shfl_xor(v, o);
         v = w > v ? w : v;
     }
+    return v;
+}
+
+__device__ __forceinline__ int wave_min_int(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
     return v;
 }
 
@@ -739,31 +722,47 @@ __device__ __forceinline__ bool neighbour_cell(const GridDev& G, const long long
     return c[0] >= 0 && c[1] >= 0 && c[2] >= 0;
 }
 
+// Members per lane and pass of the wavefront cell scans: kScanU member
+// indices are loaded, then their coordinates gathered, before any test, so a
+// wavefront keeps 64 x kScanU gathers in flight (a cell of 10^5 members is
+// about 200 passes).
+constexpr int kScanU = 8;
+
+// The up-to-2 x 64 neighbour cells of candidate p that may hold a point within
+// eps, one per lane and j: slot and box relation (0 = none / no cell).
+__device__ __forceinline__ void candidate_cells(const GridDev& G, int p, int* sl, int* rl)
+{
+    const int lane = __lane_id();
+    const int noff = G.dim == 3 ? 125 : 25;
+    long long c0[3] = {0, 0, 0};
+    cell_coords(G, p, c0);
+    for (int j = 0; j < 2; ++j) {
+        sl[j] = -1;
+        rl[j] = 0;
+        const int o = lane + 64 * j;
+        long long c[3];
+        if (o >= noff || !neighbour_cell(G, c0, o, c)) continue;
+        if (box_gap2(G, p, c) > G.prune2) continue;
+        const int slot = lookup(G, pack_cell(G.dim, c));
+        if (slot < 0) continue;
+        sl[j] = slot;
+        rl[j] = box_relation(G, p, slot);
+    }
+}
+
 // Core test, sparse part: one wavefront per candidate point.  The lanes look
 // up the 5^dim neighbour cells, add whole cells whose box lies within eps and
-// scan the undecided ones 64 points at a time, stopping at minPts.
+// scan the undecided ones 64 x kScanU points a pass, stopping at minPts.
 __global__ __launch_bounds__(256) void core_sparse_kernel(const GridDev G, int minPts)
 {
     const int lane = __lane_id();
     const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
     const int nc = G.ctl->ncand;
-    const int noff = G.dim == 3 ? 125 : 25;
     for (long long i = wave; i < nc; i += nw) {
         const int p = G.cand[i];
-        long long c0[3] = {0, 0, 0};
-        cell_coords(G, p, c0);
-        int sl[2] = {-1, -1}, rl[2] = {0, 0};
-        for (int j = 0; j < 2; ++j) {
-            const int o = lane + 64 * j;
-            long long c[3];
-            if (o >= noff || !neighbour_cell(G, c0, o, c)) continue;
-            if (box_gap2(G, p, c) > G.prune2) continue;
-            const int slot = lookup(G, pack_cell(G.dim, c));
-            if (slot < 0) continue;
-            sl[j] = slot;
-            rl[j] = box_relation(G, p, slot);
-        }
+        int sl[2], rl[2];
+        candidate_cells(G, p, sl, rl);
         long long whole = 0;
         for (int j = 0; j < 2; ++j)
             if (rl[j] == 2) whole += (long long)G.hcount[sl[j]];
@@ -775,9 +774,15 @@ __global__ __launch_bounds__(256) void core_sparse_kernel(const GridDev G, int m
                 m &= m - 1;
                 const int slot = __shfl(sl[j], ld);
                 const long long b = G.hstart[slot], e = b + (long long)G.hcount[slot];
-                for (long long k = b; k < e && cnt < minPts; k += 64) {
-                    const bool hit = k + lane < e && within(G, p, G.cellpts[k + lane]);
-                    cnt += __popcll(__ballot(hit));
+                for (long long k = b; k < e && cnt < minPts; k += 64 * kScanU) {
+                    int q[kScanU];
+#pragma unroll
+                    for (int u = 0; u < kScanU; ++u) {
+                        const long long kk = k + 64 * u + lane;
+                        q[u] = kk < e ? G.cellpts[kk] : -1;
+                    }
+#pragma unroll
+                    for (int u = 0; u < kScanU; ++u) cnt += __popcll(__ballot(q[u] >= 0 && within(G, p, q[u])));
                 }
             }
         }
@@ -785,6 +790,49 @@ __global__ __launch_bounds__(256) void core_sparse_kernel(const GridDev G, int m
             G.parent[p] = p;
             atomicMin(&G.hrep[G.pslot[p]], p);
         }
+    }
+}
+
+// Border search of the non-core candidates (every non-core point of a cell
+// lies in a sparse cell, so the candidate list holds them all): the smallest
+// core index within eps, one wavefront per point, and its component root.
+// Runs after border_kernel, which leaves -1 for these points.
+__global__ __launch_bounds__(256) void border_sparse_kernel(const GridDev G)
+{
+    const int lane = __lane_id();
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+    const int nc = G.ctl->ncand;
+    for (long long i = wave; i < nc; i += nw) {
+        const int p = G.cand[i];
+        if (G.parent[p] >= 0) continue;  // core (uniform over the wavefront)
+        int sl[2], rl[2];
+        candidate_cells(G, p, sl, rl);
+        int best = INT_MAX;
+        for (int j = 0; j < 2; ++j)
+            if (rl[j] == 2) best = min(best, G.hrep[sl[j]]);
+        for (int j = 0; j < 2; ++j) {
+            unsigned long long m = __ballot(rl[j] == 1);
+            while (m) {
+                const int ld = __builtin_ctzll(m);
+                m &= m - 1;
+                const int slot = __shfl(sl[j], ld);
+                const long long b = G.hstart[slot], e = b + (long long)G.hcount[slot];
+                for (long long k = b; k < e; k += 64 * kScanU) {
+                    int q[kScanU];
+#pragma unroll
+                    for (int u = 0; u < kScanU; ++u) {
+                        const long long kk = k + 64 * u + lane;
+                        q[u] = kk < e ? G.cellpts[kk] : INT_MAX;
+                    }
+#pragma unroll
+                    for (int u = 0; u < kScanU; ++u)
+                        if (q[u] < best && G.parent[q[u]] >= 0 && within(G, p, q[u])) best = q[u];
+                }
+            }
+        }
+        best = wave_min_int(best);
+        if (lane == 0 && best != INT_MAX) G.labroot[p] = uf_find(G, best);
     }
 }
 
@@ -931,8 +979,8 @@ __global__ __launch_bounds__(256) void link_pairs_kernel(const GridDev G, int ri
     }
 }
 
-// Label root of every point (core: its component root; border: the root of
-// its smallest-index core neighbour; else -1), the positive-root bit words
+// Label root of every core point (its component root; -1 for the others,
+// whose border search border_sparse_kernel does next), the positive-root bit words
 // (one wavefront = one 64-point word) and, under the sign rule, each
 // component's smallest fixed-point core index.
 __global__ __launch_bounds__(256) void border_kernel(const GridDev G)
@@ -948,24 +996,7 @@ __global__ __launch_bounds__(256) void border_kernel(const GridDev G)
                 root = uf_find(G, (int)p);
                 isroot = par == (int)p;
                 if (G.sign && G.include && G.include[p] == -1) atomicMin(&G.minfix[root], (int)p);
-            } else {
-                int best = INT_MAX;
-                for_neighbour_cells(G, p, [&](int slot) {
-                    const int rel = box_relation(G, p, slot);
-                    if (rel == 0) return true;
-                    if (rel == 2) {  // smallest core index of the cell
-                        best = min(best, G.hrep[slot]);
-                        return true;
-                    }
-                    const long long b = G.hstart[slot], e = b + (long long)G.hcount[slot];
-                    for (long long k = b; k < e; ++k) {
-                        const int q = G.cellpts[k];
-                        if (q < best && G.parent[q] >= 0 && within(G, p, q)) best = q;
-                    }
-                    return true;
-                });
-                if (best != INT_MAX) root = uf_find(G, best);
-            }
+            }  // non-core: border_sparse_kernel
         }
         G.labroot[p] = root;
     }
@@ -1104,6 +1135,7 @@ static DbCtl run_pipeline(CallCtx& ctx, GridDev& G, const DbscanParams& prm, int
     for (int ring = 0; ring < 3; ++ring)
         hipLaunchKernelGGL(link_pairs_kernel, dim3(kWaveBlocks), dim3(T), 0, st, G, ring);
     hipLaunchKernelGGL(border_kernel, dim3(gn), dim3(T), 0, st, G);
+    hipLaunchKernelGGL(border_sparse_kernel, dim3(kWaveBlocks), dim3(T), 0, st, G);
     if (G.sign) hipLaunchKernelGGL(sign_move_kernel, dim3(gn), dim3(T), 0, st, G);
     hipLaunchKernelGGL(word_count_kernel, dim3(ceil_div(G.nw, T)), dim3(T), 0, st, G);
     scan_i32(st, G.rankP, G.rankP, G.nw, G.scan_tmp);
@@ -1111,7 +1143,7 @@ static DbCtl run_pipeline(CallCtx& ctx, GridDev& G, const DbscanParams& prm, int
     hipLaunchKernelGGL(label_kernel, dim3(gn), dim3(T), 0, st, G, labels);
     hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(64), 0, st, G);
     CHAOS_HIP(hipGetLastError());
-    ctx.add_launch(22);
+    ctx.add_launch(23);
     DbCtl c{};
     CHAOS_HIP(hipMemcpyAsync(&c, G.ctl, sizeof(DbCtl), hipMemcpyDeviceToHost, st));
     CHAOS_HIP(hipStreamSynchronize(st));
