@@ -59,28 +59,42 @@ SweepArgs make_basins_args(const chaos_options& o, double tMax, int nPts, double
     return A;
 }
 
+// CONFIG clustering buffer: the encoded box (256 bytes) and the z-score
+// scratch.
+size_t basins_zbuf_bytes(long long n) { return 256 + zscore_scratch_bytes(n, 3); }
+
+void basins_sweep_partials(SweepArgs& A, void* zbuf)
+{
+    const ZPartials z = zscore_sweep_partials(static_cast<char*>(zbuf) + 256, A.n_points, 3);
+    A.zsum = z.sum;
+    A.zcnt = z.cnt;
+    A.zmm = z.mm;
+    A.zb = z.nb;
+}
+
 void basins_cluster(CallCtx& ctx, bool ref, long long n, const double* f0, const double* f1, const double* f2,
-                    const int* flags, double eps, int minPts, int* labels)
+                    const int* flags, double eps, int minPts, int* labels, void* zbuf)
 {
     if (ref) {
         const double* pts[2] = {f0, f1};
         DbscanParams prm{2, eps, 1, true};
         dbscan_device(ctx, pts, flags, n, prm, labels);
     } else {
-        // z-scored into a separate buffer (the caller keeps the raw
-        // features); the z-score leaves the clustering's bounding box behind,
-        // and its scratch shares that buffer (no synchronisation between)
-        const size_t zb = zscore_scratch_bytes(n, 3);
-        DevBuf<char> buf(align256(3 * (size_t)n * sizeof(double)) + zb + 256);
-        double* z = (double*)buf.get();
-        unsigned long long* box = (unsigned long long*)(buf.get() + align256(3 * (size_t)n * sizeof(double)));
-        void* zs = buf.get() + align256(3 * (size_t)n * sizeof(double)) + 256;
+        // clustered by their z-scores, formed on each read from the raw
+        // features and the device moments (no normalised copy); the moment
+        // passes leave the clustering's bounding box, and the clustering's
+        // one synchronisation comes before the buffer is released
+        DevBuf<char> own;
+        char* buf = static_cast<char*>(zbuf);
+        if (!buf) {
+            own.alloc(basins_zbuf_bytes(n));
+            buf = own.get();
+        }
+        unsigned long long* box = (unsigned long long*)buf;
         const double* src[3] = {f0, f1, f2};
-        double* f[3] = {z, z + n, z + 2 * n};
-        zscore_device(ctx, src, f, 3, flags, n, nullptr, box, zs);
-        const double* pts[3] = {f[0], f[1], f[2]};
+        const double* mom = zscore_moments_device(ctx, src, 3, flags, n, box, buf + 256, zbuf != nullptr);
         DbscanParams prm{3, eps, minPts, false};
-        dbscan_device(ctx, pts, flags, n, prm, labels, nullptr, 0, box);
+        dbscan_device(ctx, src, flags, n, prm, labels, nullptr, 0, box, mom);
     }
 }
 
@@ -103,6 +117,12 @@ extern "C" int chaos_basins2d(const chaos_options* opt, double tMax, int nPts, d
         CallCtx ctx(o);
         DevBuf<int> d_flags(n), d_labels;
         DevBuf<double> d_f(3 * (size_t)n);
+        DevBuf<char> zbuf;
+        if (!ref) {
+            // the sweep forms the z-score's first moments in its epilogue
+            zbuf.alloc(basins_zbuf_bytes(n));
+            basins_sweep_partials(A, zbuf.get());
+        }
         A.flags = d_flags.get();
         A.f0 = d_f.get();
         A.f1 = d_f.get() + n;
@@ -117,7 +137,7 @@ extern "C" int chaos_basins2d(const chaos_options* opt, double tMax, int nPts, d
         ctx.add_kernel_ms(ctx.end());
         ctx.add_steps((double)n * ((double)A.S + (double)A.B * A.pre));
         ctx.begin();
-        basins_cluster(ctx, ref, n, A.f0, A.f1, A.f2, A.flags, eps, o.min_pts, out_labels);
+        basins_cluster(ctx, ref, n, A.f0, A.f1, A.f2, A.flags, eps, o.min_pts, out_labels, ref ? nullptr : zbuf.get());
         ctx.add_analysis_ms(ctx.end());
         const hipMemcpyKind k = o.device_buffers ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
         if (!o.device_buffers)

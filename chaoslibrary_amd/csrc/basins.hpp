@@ -15,9 +15,15 @@ SweepArgs make_basins_args(const chaos_options& o, double tMax, int nPts, double
                            double transientTime, const double* values, int na, int preScaller, double eps);
 
 // Labels from the per-IC features: REFERENCE single linkage on (f0, f1) with
-// the fixed-point sign rule; CONFIG z-score (on a copy) + 3-D DBSCAN.
+// the fixed-point sign rule; CONFIG 3-D DBSCAN of the z-scored features
+// (formed on each read; the raw features stay as they are).
+// zbuf (CONFIG only): NULL, or device memory of basins_zbuf_bytes(n) into
+// which the sweep wrote its blocks' first moments (basins_sweep_partials).
 void basins_cluster(CallCtx& ctx, bool ref, long long n, const double* f0, const double* f1, const double* f2,
-                    const int* flags, double eps, int minPts, int* labels);
+                    const int* flags, double eps, int minPts, int* labels, void* zbuf = nullptr);
+size_t basins_zbuf_bytes(long long n);
+// Points the CONFIG sweep's moment outputs (zsum, zcnt, zmm, zb) into zbuf.
+void basins_sweep_partials(SweepArgs& A, void* zbuf);
 
 // The files basinsOfAttraction_2 writes: OUT (labels), OUT_1.csv / OUT_2.csv
 // (features 0, 1), OUT_3.csv (flags) and, for CONFIG, OUT_4.csv (feature 2).

@@ -29,6 +29,7 @@
 #include <cstdlib>
 
 #include "dbscan.hpp"
+#include "moments.hpp"
 
 namespace chaos {
 
@@ -50,10 +51,12 @@ struct DbCtl {
     int flags;                  // DbFlags
     int K;                      // positive clusters
     int any;                    // some point is included
+    int npieces;                // pieces of split core tests claimed
 };
 
 struct GridDev {
     const double* x[3];
+    const double* zmom;  // device [mean_0, sd_0, mean_1, sd_1, ...] to z-score x on access, or NULL
     const int* include;  // NULL: every point included
     int dim;
     int sign;            // fixed-point sign rule
@@ -79,6 +82,8 @@ struct GridDev {
     int* labroot;  // component root of the point's label, -1 for none
     int* cand;     // points of sparse cells
     int* minfix;   // sign rule: smallest fixed-point core index per root
+    int4* pieces;  // split core tests: {point, first member, end member, 0} ranges of cellpts
+    long long piece_cap;
     // numbering [nw]
     unsigned long long* bitsP;  // roots of positive clusters
     unsigned long long* bitsN;  // smallest fixed point of each negative cluster
@@ -87,6 +92,17 @@ struct GridDev {
     int* scan_tmp;
     DbCtl* ctl;
 };
+
+// Coordinate d of point p: the stored value, or its z-score when the points
+// are raw features with their moments in zmom -- the same operations as the
+// z-score of the host restatement, so the same bits as a normalised copy.
+__device__ __forceinline__ double coord(const GridDev& G, int d, long long p)
+{
+    const double v = G.x[d][p];
+    if (!G.zmom) return v;
+    const double sd = G.zmom[2 * d + 1];
+    return sd > 0 ? (v - G.zmom[2 * d]) / sd : 0.0;
+}
 
 // ---------------------------------------------------------------------------
 // workspace layout
@@ -118,6 +134,8 @@ static void carve(GridDev& G, char* base, long long n, long long H, bool sign)
     G.labroot = (int*)take(4 * (size_t)n);
     G.cand = (int*)take(4 * (size_t)n);
     G.minfix = sign ? (int*)take(4 * (size_t)n) : nullptr;
+    G.piece_cap = core_piece_cap(n);
+    G.pieces = (int4*)take(16 * (size_t)G.piece_cap);
     G.bitsP = (unsigned long long*)take(8 * (size_t)nw);
     G.rankP = (int*)take(4 * (size_t)nw);
     G.bitsN = sign ? (unsigned long long*)take(8 * (size_t)nw) : nullptr;
@@ -181,24 +199,9 @@ __device__ __forceinline__ void unpack_cell(int dim, unsigned long long key, lon
 
 __device__ __forceinline__ void cell_coords(const GridDev& G, long long p, long long* c)
 {
-    for (int d = 0; d < G.dim; ++d) c[d] = (long long)floor((G.x[d][p] - G.ctl->lo[d]) / G.s);
+    for (int d = 0; d < G.dim; ++d) c[d] = (long long)floor((coord(G, d, p) - G.ctl->lo[d]) / G.s);
 }
 
-// Order-preserving map double -> u64 (for atomicMin/atomicMax on doubles).
-__host__ __device__ __forceinline__ unsigned long long ord_enc(double v)
-{
-    unsigned long long u;
-    memcpy(&u, &v, 8);
-    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
-}
-
-__host__ __device__ __forceinline__ double ord_dec(unsigned long long e)
-{
-    const unsigned long long u = (e >> 63) ? (e & 0x7fffffffffffffffull) : ~e;
-    double v;
-    memcpy(&v, &u, 8);
-    return v;
-}
 
 __device__ __forceinline__ int lookup(const GridDev& G, unsigned long long key)
 {
@@ -240,7 +243,7 @@ __device__ __forceinline__ bool within(const GridDev& G, long long p, long long 
 {
     double d2 = 0.0;
     for (int d = 0; d < G.dim; ++d) {
-        const double df = G.x[d][p] - G.x[d][q];
+        const double df = coord(G, d, p) - coord(G, d, q);
         d2 = d2 + df * df;
     }
     return sqrt(d2) <= G.eps;
@@ -253,7 +256,7 @@ __device__ __forceinline__ double box_gap2(const GridDev& G, long long p, const 
     for (int d = 0; d < G.dim; ++d) {
         const double lo = G.ctl->lo[d] + (double)c[d] * G.s;
         const double hi = lo + G.s;
-        const double v = G.x[d][p];
+        const double v = coord(G, d, p);
         const double g = v < lo ? lo - v : (v > hi ? v - hi : 0.0);
         g2 += g * g;
     }
@@ -271,7 +274,7 @@ __device__ __forceinline__ int box_relation(const GridDev& G, long long p, int s
     for (int d = 0; d < G.dim; ++d) {
         const double lo = ord_dec(G.bbox[d * G.H + slot]);
         const double hi = ord_dec(G.bbox[(3 + d) * G.H + slot]);
-        const double v = G.x[d][p];
+        const double v = coord(G, d, p);
         const double g = v < lo ? lo - v : (v > hi ? v - hi : 0.0);
         const double f = fmax(v - lo, hi - v);
         g2 = g2 + g * g;
@@ -405,6 +408,7 @@ __global__ void init_kernel(const GridDev G, const unsigned long long* box, int 
             G.ctl->ncore_cells = 0;
             G.ctl->flags = 0;
             G.ctl->K = 0;
+            G.ctl->npieces = 0;
         }
         return;
     }
@@ -431,7 +435,7 @@ __global__ __launch_bounds__(256) void bbox_kernel(const GridDev G)
          p += (long long)gridDim.x * blockDim.x) {
         if (!included(G, p)) continue;
         for (int d = 0; d < G.dim; ++d) {
-            const unsigned long long e = ord_enc(G.x[d][p]);
+            const unsigned long long e = ord_enc(coord(G, d, p));
             mn[d] = e < mn[d] ? e : mn[d];
             mx[d] = e > mx[d] ? e : mx[d];
         }
@@ -527,7 +531,7 @@ __global__ __launch_bounds__(256) void insert_kernel(const GridDev G)
             long long c[3] = {0, 0, 0};
             cell_coords(G, p, c);
             key = pack_cell(G.dim, c);
-            for (int d = 0; d < G.dim; ++d) e[d] = ord_enc(G.x[d][p]);
+            for (int d = 0; d < G.dim; ++d) e[d] = ord_enc(coord(G, d, p));
         }
         int mine_loc = -1;
         bool pending = valid;
@@ -624,8 +628,11 @@ __device__ __forceinline__ int lds_claim_int(int* s_key, int key)
 }
 
 // Points into their cells' segments of cellpts: ranks within the block's
-// chunk per cell in LDS, one global reservation per cell and block.
-__global__ __launch_bounds__(256) void place_kernel(const GridDev G)
+// chunk per cell in LDS, one global reservation per cell and block.  Also the
+// dense part of the core test: a cell holding >= minPts points is all-core
+// (its diagonal is below eps) and its members link straight to its smallest
+// member; points of sparser cells go to the candidate list.
+__global__ __launch_bounds__(256) void place_kernel(const GridDev G, int minPts)
 {
     __shared__ int s_slot[kLds];
     __shared__ int s_cnt[kLds];
@@ -636,11 +643,12 @@ __global__ __launch_bounds__(256) void place_kernel(const GridDev G)
         s_cnt[i] = 0;
     }
     __syncthreads();
-    int loc[kPer], rank[kPer];  // loc: LDS entry, -2 = placed directly, -1 = none
+    int loc[kPer], rank[kPer], sl[kPer];  // loc: LDS entry, -2 = placed directly, -1 = none
     const long long base = (long long)blockIdx.x * kChunkPts;
     for (int k = 0; k < kPer; ++k) {
         const long long p = base + (long long)k * 256 + tid;
         const int s = p < G.n ? G.pslot[p] : -1;
+        sl[k] = s;
         int my_loc = -1, my_rank = 0;
         bool pending = s >= 0;
         for (int it = 0; it <= kAggGroups; ++it) {
@@ -684,10 +692,13 @@ __global__ __launch_bounds__(256) void place_kernel(const GridDev G)
     __syncthreads();
     for (int k = 0; k < kPer; ++k) {
         const long long p = base + (long long)k * 256 + tid;
-        if (p >= G.n) break;
         const int l = loc[k];
         if (l >= 0) G.cellpts[s_base[l] + rank[k]] = (int)p;
         else if (l == -2) G.cellpts[rank[k]] = (int)p;
+        const int s = sl[k];
+        const bool dense = s >= 0 && (long long)G.hcount[s] >= minPts;
+        if (p < G.n) G.parent[p] = dense ? G.hmin[s] : -1;
+        block_append(s >= 0 && !dense, (int)p, G.cand, &G.ctl->ncand);  // every thread of the block
     }
 }
 
@@ -698,18 +709,6 @@ __global__ void cell_init_kernel(const GridDev G, int minPts)
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= G.H) return;
     G.hrep[s] = (G.hkey[s] != kEmpty && (long long)G.hcount[s] >= minPts) ? G.hmin[s] : INT_MAX;
-}
-
-// Core test, dense part: a cell holding >= minPts points is all-core (its
-// diagonal is below eps) and its members link straight to its smallest
-// member.  Points of sparser cells go to the candidate list.
-__global__ __launch_bounds__(256) void core_dense_kernel(const GridDev G, int minPts)
-{
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = p < G.n ? G.pslot[p] : -1;
-    const bool dense = s >= 0 && (long long)G.hcount[s] >= minPts;
-    if (p < G.n) G.parent[p] = dense ? G.hmin[s] : -1;
-    block_append(s >= 0 && !dense, (int)p, G.cand, &G.ctl->ncand);
 }
 
 // Neighbour cell o (0 <= o < 5^dim) of cell c0: coordinates, false if any is
@@ -750,9 +749,65 @@ __device__ __forceinline__ void candidate_cells(const GridDev& G, int p, int* sl
     }
 }
 
+// Exclusive prefix sum over the lanes of a wavefront.
+__device__ __forceinline__ int wave_excl_sum(int v)
+{
+    const int lane = __lane_id();
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    return x - v;
+}
+
+// Members of one piece of a split core test (8 passes of a wavefront); a
+// candidate whose undecided cells hold more members than this is split.
+constexpr int kPieceMembers = 64 * kScanU * 8;
+// Members of each undecided cell a split candidate scans itself first (one
+// pass): a core candidate usually reaches minPts there.
+constexpr int kHeadMembers = 64 * kScanU;
+
+// Hits of point p among members [b + first, min(e, b + last)) of every
+// undecided cell (rl == 1) of the lane's slots, added to cnt, stopping once
+// cnt reaches minPts; wavefront-uniform.
+__device__ __forceinline__ long long scan_undecided(const GridDev& G, int p, const int* sl, const int* rl,
+                                                    long long first, long long last, long long cnt, int minPts)
+{
+    const int lane = __lane_id();
+    for (int j = 0; j < 2 && cnt < minPts; ++j) {
+        unsigned long long m = __ballot(rl[j] == 1);
+        while (m && cnt < minPts) {
+            const int ld = __builtin_ctzll(m);
+            m &= m - 1;
+            const int slot = __shfl(sl[j], ld);
+            const long long c0 = G.hstart[slot], c1 = c0 + (long long)G.hcount[slot];
+            const long long b = c0 + first, e = c1 - c0 > last ? c0 + last : c1;
+            for (long long k = b; k < e && cnt < minPts; k += 64 * kScanU) {
+                int q[kScanU];
+#pragma unroll
+                for (int u = 0; u < kScanU; ++u) {
+                    const long long kk = k + 64 * u + lane;
+                    q[u] = kk < e ? G.cellpts[kk] : -1;
+                }
+#pragma unroll
+                for (int u = 0; u < kScanU; ++u) cnt += __popcll(__ballot(q[u] >= 0 && within(G, p, q[u])));
+            }
+        }
+    }
+    return cnt;
+}
+
 // Core test, sparse part: one wavefront per candidate point.  The lanes look
 // up the 5^dim neighbour cells, add whole cells whose box lies within eps and
-// scan the undecided ones 64 x kScanU points a pass, stopping at minPts.
+// scan the undecided ones 64 x kScanU points a pass, stopping at minPts.  A
+// candidate whose undecided cells hold more than kPieceMembers members (a
+// point beside a cell of a converged basin: 10^5..10^6 members) scans the
+// first kHeadMembers of each itself; if that does not decide it, the rest
+// is split: its count so far goes to labroot[p] (free until the border
+// pass) and the remaining members go to the piece list for
+// core_piece_kernel.  labroot[p] = -1 marks a candidate decided here.  A
+// full list falls back to the scan in this wavefront.
 __global__ __launch_bounds__(256) void core_sparse_kernel(const GridDev G, int minPts)
 {
     const int lane = __lane_id();
@@ -763,30 +818,97 @@ __global__ __launch_bounds__(256) void core_sparse_kernel(const GridDev G, int m
         const int p = G.cand[i];
         int sl[2], rl[2];
         candidate_cells(G, p, sl, rl);
-        long long whole = 0;
-        for (int j = 0; j < 2; ++j)
+        long long whole = 0, und = 0;
+        for (int j = 0; j < 2; ++j) {
             if (rl[j] == 2) whole += (long long)G.hcount[sl[j]];
+            if (rl[j] == 1) und += (long long)G.hcount[sl[j]];
+        }
         long long cnt = wave_sum_ll(whole);
-        for (int j = 0; j < 2 && cnt < minPts; ++j) {
-            unsigned long long m = __ballot(rl[j] == 1);
-            while (m && cnt < minPts) {
-                const int ld = __builtin_ctzll(m);
-                m &= m - 1;
-                const int slot = __shfl(sl[j], ld);
-                const long long b = G.hstart[slot], e = b + (long long)G.hcount[slot];
-                for (long long k = b; k < e && cnt < minPts; k += 64 * kScanU) {
-                    int q[kScanU];
-#pragma unroll
-                    for (int u = 0; u < kScanU; ++u) {
-                        const long long kk = k + 64 * u + lane;
-                        q[u] = kk < e ? G.cellpts[kk] : -1;
+        const bool heavy = cnt < minPts && wave_sum_ll(und) > kPieceMembers;
+        cnt = scan_undecided(G, p, sl, rl, 0, heavy ? kHeadMembers : LLONG_MAX, cnt, minPts);
+        if (heavy && cnt < minPts) {
+            int npc = 0;
+            for (int j = 0; j < 2; ++j)
+                if (rl[j] == 1) {
+                    const long long rest = (long long)G.hcount[sl[j]] - kHeadMembers;
+                    if (rest > 0) npc += (int)((rest + kPieceMembers - 1) / kPieceMembers);
+                }
+            const int off = wave_excl_sum(npc);
+            const int tot = __shfl(off + npc, 63);
+            // claim tot slots only while they fit, so that every slot below
+            // the counter is written
+            int base = -1;
+            if (lane == 0) {
+                int cur = __hip_atomic_load(&G.ctl->npieces, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                while ((long long)cur + tot <= G.piece_cap) {
+                    const int prev = atomicCAS(&G.ctl->npieces, cur, cur + tot);
+                    if (prev == cur) {
+                        base = cur;
+                        break;
                     }
-#pragma unroll
-                    for (int u = 0; u < kScanU; ++u) cnt += __popcll(__ballot(q[u] >= 0 && within(G, p, q[u])));
+                    cur = prev;
                 }
             }
+            base = __shfl(base, 0);
+            if (base >= 0) {
+                int w = base + off;
+                for (int j = 0; j < 2; ++j) {
+                    if (rl[j] != 1) continue;
+                    const int c0 = G.hstart[sl[j]], e = c0 + (int)G.hcount[sl[j]];
+                    for (int k = c0 + kHeadMembers; k < e; k += kPieceMembers)
+                        G.pieces[w++] = make_int4(p, k, min(k + kPieceMembers, e), 0);
+                }
+                if (lane == 0) G.labroot[p] = (int)cnt;
+                continue;
+            }
+            cnt = scan_undecided(G, p, sl, rl, kHeadMembers, LLONG_MAX, cnt, minPts);
         }
-        if (cnt >= minPts && lane == 0) {
+        if (lane == 0) {
+            G.labroot[p] = -1;
+            if (cnt >= minPts) {
+                G.parent[p] = p;
+                atomicMin(&G.hrep[G.pslot[p]], p);
+            }
+        }
+    }
+}
+
+// Pieces of the split core tests, one wavefront per piece: the hits among
+// the piece's members, added to the candidate's count after every pass; the
+// piece stops once the candidate has reached minPts.
+__global__ __launch_bounds__(256) void core_piece_kernel(const GridDev G, int minPts)
+{
+    const int lane = __lane_id();
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+    const long long np = G.ctl->npieces;  // <= piece_cap, every slot below it written
+    for (long long i = wave; i < np; i += nw) {
+        const int4 pc = G.pieces[i];
+        const int p = pc.x;
+        for (int k = pc.y; k < pc.z; k += 64 * kScanU) {
+            if (__builtin_amdgcn_readfirstlane(uf_load(&G.labroot[p])) >= minPts) break;
+            int q[kScanU];
+#pragma unroll
+            for (int u = 0; u < kScanU; ++u) {
+                const int kk = k + 64 * u + lane;
+                q[u] = kk < pc.z ? G.cellpts[kk] : -1;
+            }
+            int hits = 0;
+#pragma unroll
+            for (int u = 0; u < kScanU; ++u) hits += __popcll(__ballot(q[u] >= 0 && within(G, p, q[u])));
+            if (lane == 0 && hits) atomicAdd(&G.labroot[p], hits);
+        }
+    }
+}
+
+// Verdicts of the split core tests (labroot[p] >= 0 marks a split candidate).
+__global__ void core_split_kernel(const GridDev G, int minPts)
+{
+    const int nc = G.ctl->ncand;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nc;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int p = G.cand[i];
+        if (G.labroot[p] >= minPts) {
             G.parent[p] = p;
             atomicMin(&G.hrep[G.pslot[p]], p);
         }
@@ -893,7 +1015,7 @@ __device__ void link_cell_pair(const GridDev& G, int a, long long a0, long long 
         const bool cand = rel == 1;
         if (!__ballot(cand)) continue;
         if (cand)
-            for (int d = 0; d < G.dim; ++d) px[d] = G.x[d][p];
+            for (int d = 0; d < G.dim; ++d) px[d] = coord(G, d, p);
         for (long long kb = b0; kb < b1; kb += 64) {
             int q = -1;
             bool qok = false;
@@ -902,7 +1024,7 @@ __device__ void link_cell_pair(const GridDev& G, int a, long long a0, long long 
                 q = G.cellpts[kb + lane];
                 qok = is_core(G, q) && box_relation(G, q, a) != 0;
                 if (qok)
-                    for (int d = 0; d < G.dim; ++d) qx[d] = G.x[d][q];
+                    for (int d = 0; d < G.dim; ++d) qx[d] = coord(G, d, q);
             }
             unsigned long long mq = __ballot(qok);
             while (mq) {
@@ -1126,10 +1248,11 @@ static DbCtl run_pipeline(CallCtx& ctx, GridDev& G, const DbscanParams& prm, int
     const unsigned gc = ceil_div(n, kChunkPts);
     hipLaunchKernelGGL(insert_kernel, dim3(gc), dim3(T), 0, st, G);
     scan_i32(st, reinterpret_cast<const int*>(G.hcount), G.hstart, G.H, G.scan_tmp);
-    hipLaunchKernelGGL(place_kernel, dim3(gc), dim3(T), 0, st, G);
+    hipLaunchKernelGGL(place_kernel, dim3(gc), dim3(T), 0, st, G, prm.minPts);
     hipLaunchKernelGGL(cell_init_kernel, dim3(gh), dim3(T), 0, st, G, prm.minPts);
-    hipLaunchKernelGGL(core_dense_kernel, dim3(gn), dim3(T), 0, st, G, prm.minPts);
     hipLaunchKernelGGL(core_sparse_kernel, dim3(kWaveBlocks), dim3(T), 0, st, G, prm.minPts);
+    hipLaunchKernelGGL(core_piece_kernel, dim3(kWaveBlocks), dim3(T), 0, st, G, prm.minPts);
+    hipLaunchKernelGGL(core_split_kernel, dim3(1024), dim3(T), 0, st, G, prm.minPts);
     hipLaunchKernelGGL(link_sparse_kernel, dim3(1024), dim3(T), 0, st, G);
     hipLaunchKernelGGL(cell_list_kernel, dim3(gh), dim3(T), 0, st, G);
     for (int ring = 0; ring < 3; ++ring)
@@ -1143,7 +1266,7 @@ static DbCtl run_pipeline(CallCtx& ctx, GridDev& G, const DbscanParams& prm, int
     hipLaunchKernelGGL(label_kernel, dim3(gn), dim3(T), 0, st, G, labels);
     hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(64), 0, st, G);
     CHAOS_HIP(hipGetLastError());
-    ctx.add_launch(23);
+    ctx.add_launch(24);
     DbCtl c{};
     CHAOS_HIP(hipMemcpyAsync(&c, G.ctl, sizeof(DbCtl), hipMemcpyDeviceToHost, st));
     CHAOS_HIP(hipStreamSynchronize(st));
@@ -1151,7 +1274,7 @@ static DbCtl run_pipeline(CallCtx& ctx, GridDev& G, const DbscanParams& prm, int
 }
 
 int dbscan_device(CallCtx& ctx, const double* const* pts, const int* include, long long n, const DbscanParams& prm,
-                  int* labels, void* ws, size_t ws_bytes, const unsigned long long* box)
+                  int* labels, void* ws, size_t ws_bytes, const unsigned long long* box, const double* zmom)
 {
     CHAOS_REQUIRE(prm.dim == 2 || prm.dim == 3, "dbscan dim must be 2 or 3");
     CHAOS_REQUIRE(prm.eps > 0 && std::isfinite(prm.eps), "eps must be positive");
@@ -1159,6 +1282,7 @@ int dbscan_device(CallCtx& ctx, const double* const* pts, const int* include, lo
     CHAOS_REQUIRE(n >= 1 && n < INT_MAX, "dbscan point count out of range");
     GridDev G{};
     for (int d = 0; d < 3; ++d) G.x[d] = d < prm.dim ? pts[d] : pts[0];
+    G.zmom = zmom;
     G.include = include;
     G.dim = prm.dim;
     G.sign = prm.sign_rule ? 1 : 0;
@@ -1198,194 +1322,291 @@ int dbscan_device(CallCtx& ctx, const double* const* pts, const int* include, lo
 // deterministic z-score
 // ---------------------------------------------------------------------------
 
-// Block b sums its 1024 points with the fixed tree (s[t] = v[t] + v[t+512],
-// then strides 256 .. 1) for every feature; pass 0 sums the values and counts
-// the masked points, pass 1 the squared deviations from the pass-0 mean.
+// Block b sums its 1024 points by the pairwise tree (thread t adds points
+// 2t and 2t+1, the wavefronts' butterflies give the 128-point runs, then the
+// 8 runs pairwise); pass 0 sums the values, counts the masked points and
+// takes their encoded minimum and maximum, pass 1 sums the squared
+// deviations from the pass-0 mean.  Masked-out points add +0.
 __global__ __launch_bounds__(512) void moment_block_kernel(const double* x0, const double* x1, const double* x2,
                                                            const int* __restrict__ mask, long long n, int k, int pass,
                                                            const double* __restrict__ mom, double* __restrict__ part,
-                                                           int* __restrict__ cnt)
+                                                           int* __restrict__ cnt, unsigned long long* __restrict__ pmm)
 {
-    __shared__ double s[512];
+    __shared__ double s_w[3][8];
+    __shared__ unsigned long long s_mm[6][8];
+    __shared__ int s_c[8];
     const double* xs[3] = {x0, x1, x2};
-    const long long base = (long long)blockIdx.x * 1024;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, lane = __lane_id(), wave = t >> 6;
     const long long nb = gridDim.x;
+    const long long i0 = (long long)blockIdx.x * 1024 + 2 * t, i1 = i0 + 1;
+    const bool m0 = i0 < n && (!mask || mask[i0] != 0), m1 = i1 < n && (!mask || mask[i1] != 0);
     for (int j = 0; j < k; ++j) {
         const double* x = xs[j];
-        const double mean = pass == 1 ? mom[2 * j] : 0.0;
-        auto val = [&](long long i) -> double {
-            if (i >= n || (mask && mask[i] == 0)) return 0.0;
-            if (pass == 0) return x[i];
-            const double d = x[i] - mean;
-            return d * d;
-        };
-        s[t] = val(base + t) + val(base + t + 512);
-        __syncthreads();
-        for (int stride = 256; stride > 0; stride >>= 1) {
-            if (t < stride) s[t] = s[t] + s[t + stride];
-            __syncthreads();
+        const double v0 = m0 ? x[i0] : 0.0, v1 = m1 ? x[i1] : 0.0;
+        double a0 = v0, a1 = v1;
+        if (pass == 1) {
+            const double mean = mom[2 * j];
+            const double d0 = v0 - mean, d1 = v1 - mean;
+            a0 = m0 ? d0 * d0 : 0.0;
+            a1 = m1 ? d1 * d1 : 0.0;
         }
-        if (t == 0) part[j * nb + blockIdx.x] = s[0];
-        __syncthreads();
-    }
-    if (pass == 0) {
-        __shared__ int sc[8];
-        const long long i0 = base + t, i1 = base + t + 512;
-        const bool m0 = i0 < n && (!mask || mask[i0] != 0), m1 = i1 < n && (!mask || mask[i1] != 0);
-        const int c = __popcll(__ballot(m0)) + __popcll(__ballot(m1));
-        if (__lane_id() == 0) sc[t >> 6] = c;
-        __syncthreads();
-        if (t == 0) {
-            int tot = 0;
-            for (int w = 0; w < 8; ++w) tot += sc[w];
-            cnt[blockIdx.x] = tot;
-        }
-    }
-}
-
-// Block sums added in block order: mom[2j] = mean (pass 0) or mom[2j+1] = sd
-// (pass 1); mom[2k] = number of masked points.
-__global__ void moment_final_kernel(const double* __restrict__ part, const int* __restrict__ cnt, long long nb, int k,
-                                    int pass, double* __restrict__ mom)
-{
-    // the count of masked points (an integer sum: any order), by the block
-    __shared__ long long s_n[256];
-    const int j = threadIdx.x;
-    long long c = 0;
-    if (pass == 0)
-        for (long long b = j; b < nb; b += blockDim.x) c += cnt[b];
-    s_n[j] = c;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (j < o) s_n[j] += s_n[j + o];
-        __syncthreads();
-    }
-    if (pass == 0 && j == 0) mom[2 * k] = (double)s_n[0];
-    if (j >= k) return;
-    // block sums added strictly in block order; the loads of 16 blocks are
-    // issued together so the dependent adds do not wait on each load
-    double tot = 0.0;
-    const double* pj = part + j * nb;
-    long long b = 0;
-    for (; b + 16 <= nb; b += 16) {
-        double v[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = pj[b + u];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) tot = tot + v[u];
-    }
-    for (; b < nb; ++b) tot = tot + pj[b];
-    if (pass == 0) {
-        const long long N = s_n[0];
-        mom[2 * j] = N > 0 ? tot / (double)N : 0.0;
-    } else {
-        const double N = mom[2 * k];
-        mom[2 * j + 1] = N > 0 ? sqrt(tot / N) : 0.0;
-    }
-}
-
-// dst = z-scored src on the masked points, src elsewhere (and everywhere when
-// no point is masked); box: encoded bounding box of dst over masked points.
-__global__ __launch_bounds__(256) void normalize_kernel(const double* x0, const double* x1, const double* x2, double* y0,
-                                                       double* y1, double* y2, const int* __restrict__ mask,
-                                                       long long n, int k, const double* __restrict__ mom,
-                                                       unsigned long long* box)
-{
-    // grid-stride: a few thousand blocks, so the box takes a few thousand
-    // atomics per address rather than one per 256 points
-    __shared__ unsigned long long smin[3][4], smax[3][4];
-    const double* xs[3] = {x0, x1, x2};
-    double* ys[3] = {y0, y1, y2};
-    const bool any = mom[2 * k] > 0;
-    const int lane = __lane_id(), wave = threadIdx.x >> 6;
-    unsigned long long mn[3] = {~0ull, ~0ull, ~0ull}, mx[3] = {0ull, 0ull, 0ull};
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x) {
-        const bool in = !mask || mask[i] != 0;
-        for (int j = 0; j < k; ++j) {
-            const double v = xs[j][i];
-            const double mean = mom[2 * j], sd = mom[2 * j + 1];
-            const double z = (!any || !in) ? v : (sd > 0 ? (v - mean) / sd : 0.0);
-            ys[j][i] = z;
-            if (in) {
-                const unsigned long long e = ord_enc(z);
-                mn[j] = e < mn[j] ? e : mn[j];
-                mx[j] = e > mx[j] ? e : mx[j];
+        const double sw = wave_tree_sum(a0 + a1);
+        if (lane == 0) s_w[j][wave] = sw;
+        if (pass == 0) {
+            unsigned long long lo = ~0ull, hi = 0ull;
+            if (m0) lo = hi = ord_enc(v0);
+            if (m1) {
+                const unsigned long long e = ord_enc(v1);
+                lo = e < lo ? e : lo;
+                hi = e > hi ? e : hi;
+            }
+            lo = wave_min_enc(lo);
+            hi = wave_max_enc(hi);
+            if (lane == 0) {
+                s_mm[2 * j][wave] = lo;
+                s_mm[2 * j + 1][wave] = hi;
             }
         }
     }
-    if (!box) return;
-    for (int j = 0; j < k; ++j) {
-        const unsigned long long a = wave_min_u64(mn[j]), b = wave_max_u64(mx[j]);
-        if (lane == 0) {
-            smin[j][wave] = a;
-            smax[j][wave] = b;
-        }
+    if (pass == 0) {
+        const int c = __popcll(__ballot(m0)) + __popcll(__ballot(m1));
+        if (lane == 0) s_c[wave] = c;
     }
     __syncthreads();
-    if (threadIdx.x < k) {
-        const int j = threadIdx.x;
-        unsigned long long a = smin[j][0], b = smax[j][0];
-        for (int w = 1; w < 4; ++w) {
-            a = smin[j][w] < a ? smin[j][w] : a;
-            b = smax[j][w] > b ? smax[j][w] : b;
+    if (t == 0) {
+        const long long b = blockIdx.x;
+        for (int j = 0; j < k; ++j) {
+            const double* w = s_w[j];
+            part[j * nb + b] = ((w[0] + w[1]) + (w[2] + w[3])) + ((w[4] + w[5]) + (w[6] + w[7]));
+            if (pass == 0) {
+                unsigned long long lo = s_mm[2 * j][0], hi = s_mm[2 * j + 1][0];
+                for (int u = 1; u < 8; ++u) {
+                    lo = s_mm[2 * j][u] < lo ? s_mm[2 * j][u] : lo;
+                    hi = s_mm[2 * j + 1][u] > hi ? s_mm[2 * j + 1][u] : hi;
+                }
+                pmm[(2 * j) * nb + b] = lo;
+                pmm[(2 * j + 1) * nb + b] = hi;
+            }
         }
-        if (a != ~0ull) {
-            atomicMin(&box[j], a);
-            atomicMax(&box[3 + j], b);
+        if (pass == 0) {
+            int tot = 0;
+            for (int u = 0; u < 8; ++u) tot += s_c[u];
+            cnt[b] = tot;
         }
     }
 }
 
-__global__ void box_init_kernel(unsigned long long* box)
+// The z-score of value v with moments (mean, sd): the host restatement's
+// expression (sd 0: every value maps to 0).
+__device__ __forceinline__ double zval(double v, double mean, double sd) { return sd > 0 ? (v - mean) / sd : 0.0; }
+
+// Values per LDS tree (16 KB of doubles).
+constexpr int kTreeChunk = 2048;
+
+// Pairwise tree of w values s[0..w) (w a power of two <= kTreeChunk) by a
+// 256-thread block: level by level, pairs (2t, 2t+1) -> t.  Every thread of
+// the block calls it and gets the sum; s is overwritten.
+__device__ double lds_tree(double* s, int w)
 {
-    if (threadIdx.x < 6) box[threadIdx.x] = threadIdx.x < 3 ? ~0ull : 0ull;
+    for (int h = w >> 1; h >= 1; h >>= 1) {
+        double v[kTreeChunk / 512];
+#pragma unroll
+        for (int u = 0; u < kTreeChunk / 512; ++u) {
+            const int t = threadIdx.x + 256 * u;
+            v[u] = t < h ? s[2 * t] + s[2 * t + 1] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kTreeChunk / 512; ++u) {
+            const int t = threadIdx.x + 256 * u;
+            if (t < h) s[t] = v[u];
+        }
+        __syncthreads();
+    }
+    const double r = s[0];
+    __syncthreads();
+    return r;
 }
 
-size_t zscore_scratch_bytes(long long n, int k)
+// 1024-point runs of the padded tree for n points: P / 1024 with P = max(1024,
+// the power of two >= n).
+__host__ __device__ inline long long moment_runs(long long n)
 {
-    const long long nb = (n + 1023) / 1024;
-    return align256(8 * (size_t)k * nb) + align256(8 * (size_t)(2 * k + 1)) + align256(4 * (size_t)nb) + 256;
+    long long P = 1024;
+    while (P < n) P <<= 1;
+    return P / 1024;
 }
 
-void zscore_device(CallCtx& ctx, const double* const* src, double* const* dst, int k, const int* mask, long long n,
-                   double* moments, unsigned long long* box, void* scratch)
+// mom = [mean_0, sd_0, ..., mean_{k-1}, sd_{k-1}, N]; rmm = encoded raw
+// minimum and maximum per feature.  The block partials in part / cnt / pmm
+// cover runs of `span` points (1024: moment_block_kernel; 256: the basins
+// sweep's epilogue), nb of them; the tree goes on from there over the
+// 1024-point runs (four 256-point runs pairwise) up to the padded whole.
+// Pass 0: N, the means and rmm; pass 1: the standard deviations and, when
+// box is not NULL, the encoded bounding box of the z-scored masked points (z
+// is monotone in v, so the box is the z-scores of the raw extremes; empty:
+// min ~0, max 0).  One block of 256 threads.
+__global__ __launch_bounds__(256) void moment_final_kernel(const double* __restrict__ part, const int* __restrict__ cnt,
+                                                           const unsigned long long* __restrict__ pmm, long long nb,
+                                                           int span, long long n, int k, int pass,
+                                                           double* __restrict__ mom, unsigned long long* __restrict__ rmm,
+                                                           unsigned long long* __restrict__ box)
+{
+    __shared__ double s[kTreeChunk];
+    __shared__ double s_chunk[1024];
+    __shared__ long long s_n[256];
+    __shared__ unsigned long long s_mm[6][256];
+    const int t = threadIdx.x;
+    if (pass == 0) {
+        // the count (an integer sum) and the raw extremes, in any order
+        long long c = 0;
+        unsigned long long mm[6] = {~0ull, 0ull, ~0ull, 0ull, ~0ull, 0ull};
+        for (long long b = t; b < nb; b += 256) {
+            c += cnt[b];
+            for (int f = 0; f < k; ++f) {
+                const unsigned long long lo = pmm[(2 * f) * nb + b], hi = pmm[(2 * f + 1) * nb + b];
+                mm[2 * f] = lo < mm[2 * f] ? lo : mm[2 * f];
+                mm[2 * f + 1] = hi > mm[2 * f + 1] ? hi : mm[2 * f + 1];
+            }
+        }
+        s_n[t] = c;
+        for (int f = 0; f < 6; ++f) s_mm[f][t] = mm[f];
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (t < o) {
+                s_n[t] += s_n[t + o];
+                for (int f = 0; f < 6; f += 2) {
+                    s_mm[f][t] = s_mm[f][t + o] < s_mm[f][t] ? s_mm[f][t + o] : s_mm[f][t];
+                    s_mm[f + 1][t] = s_mm[f + 1][t + o] > s_mm[f + 1][t] ? s_mm[f + 1][t + o] : s_mm[f + 1][t];
+                }
+            }
+            __syncthreads();
+        }
+        if (t == 0) {
+            mom[2 * k] = (double)s_n[0];
+            for (int f = 0; f < 2 * k; ++f) rmm[f] = s_mm[f][0];
+        }
+        __syncthreads();
+    }
+    const long long runs = moment_runs(n);
+    const double N = pass == 0 ? (double)s_n[0] : mom[2 * k];
+    for (int j = 0; j < k; ++j) {
+        const double* pj = part + j * nb;
+        auto leaf = [&](long long r) -> double {  // the 1024-point run r
+            if (span == 1024) return r < nb ? pj[r] : 0.0;
+            const long long q = 4 * r;
+            const double a = q < nb ? pj[q] : 0.0, b = q + 1 < nb ? pj[q + 1] : 0.0;
+            const double c = q + 2 < nb ? pj[q + 2] : 0.0, d = q + 3 < nb ? pj[q + 3] : 0.0;
+            return (a + b) + (c + d);
+        };
+        double tot;
+        if (runs <= kTreeChunk) {
+            for (int u = t; u < runs; u += 256) s[u] = leaf(u);
+            __syncthreads();
+            tot = lds_tree(s, (int)runs);
+        } else {
+            const long long nch = runs / kTreeChunk;  // <= 1024 for n < 2^31
+            for (long long ch = 0; ch < nch; ++ch) {
+                for (int u = t; u < kTreeChunk; u += 256) s[u] = leaf(ch * kTreeChunk + u);
+                __syncthreads();
+                const double cs = lds_tree(s, kTreeChunk);
+                if (t == 0) s_chunk[ch] = cs;
+            }
+            __syncthreads();
+            for (int u = t; u < nch; u += 256) s[u] = s_chunk[u];
+            __syncthreads();
+            tot = lds_tree(s, (int)nch);
+        }
+        if (t == 0) {
+            if (pass == 0) {
+                mom[2 * j] = N > 0 ? tot / N : 0.0;
+            } else {
+                const double sd = N > 0 ? sqrt(tot / N) : 0.0;
+                mom[2 * j + 1] = sd;
+                if (box) {
+                    const unsigned long long lo = rmm[2 * j], hi = rmm[2 * j + 1];
+                    const bool any = lo != ~0ull;
+                    box[j] = any ? ord_enc(zval(ord_dec(lo), mom[2 * j], sd)) : ~0ull;
+                    box[3 + j] = any ? ord_enc(zval(ord_dec(hi), mom[2 * j], sd)) : 0ull;
+                }
+            }
+        }
+    }
+    if (pass == 1 && box && t < 3 && t >= k) {
+        box[t] = ~0ull;
+        box[3 + t] = 0ull;
+    }
+}
+
+namespace {
+
+struct ZLayout {
+    double* part;
+    int* cnt;
+    unsigned long long* pmm;
+    double* mom;
+    unsigned long long* rmm;
+    size_t bytes;
+};
+
+// Partials sized for 256-point blocks (the larger form), then the moments
+// and the raw extremes.
+ZLayout zlayout(void* scratch, long long n, int k)
+{
+    const long long nb = (n + 255) / 256;
+    char* p = static_cast<char*>(scratch);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char* q = p ? p + off : nullptr;
+        off += align256(bytes);
+        return q;
+    };
+    ZLayout z{};
+    z.part = (double*)take(8 * (size_t)k * nb);
+    z.cnt = (int*)take(4 * (size_t)nb);
+    z.pmm = (unsigned long long*)take(8 * (size_t)(2 * k) * nb);
+    z.mom = (double*)take(8 * (size_t)(2 * k + 1));
+    z.rmm = (unsigned long long*)take(8 * (size_t)(2 * k));
+    z.bytes = off;
+    return z;
+}
+
+}  // namespace
+
+size_t zscore_scratch_bytes(long long n, int k) { return zlayout(nullptr, n, k).bytes; }
+
+ZPartials zscore_sweep_partials(void* scratch, long long n, int k)
+{
+    const ZLayout z = zlayout(scratch, n, k);
+    return ZPartials{z.part, z.cnt, z.pmm, (n + 255) / 256};
+}
+
+const double* zscore_moments_device(CallCtx& ctx, const double* const* src, int k, const int* mask, long long n,
+                                    unsigned long long* box, void* scratch, bool sweep_partials)
 {
     CHAOS_REQUIRE(k >= 1 && k <= 3, "z-score of 1 to 3 features");
-    if (n <= 0) return;
+    CHAOS_REQUIRE(n >= 1 && n < (1LL << 31), "z-score point count out of range");
+    CHAOS_REQUIRE(scratch, "z-score scratch is NULL");
     hipStream_t st = ctx.stream();
-    const long long nb = (n + 1023) / 1024;
-    DevBuf<char> own;
-    char* base = static_cast<char*>(scratch);
-    if (!base) {
-        own.alloc(zscore_scratch_bytes(n, k));
-        base = own.get();
-    }
-    double* part = (double*)base;
-    double* mom = (double*)(base + align256(8 * (size_t)k * nb));
-    int* cnt = (int*)((char*)mom + align256(8 * (size_t)(2 * k + 1)));
+    const ZLayout z = zlayout(scratch, n, k);
+    const long long nb1024 = (n + 1023) / 1024, nb256 = (n + 255) / 256;
     const double* s[3] = {src[0], k > 1 ? src[1] : src[0], k > 2 ? src[2] : src[0]};
-    double* d[3] = {dst[0], k > 1 ? dst[1] : dst[0], k > 2 ? dst[2] : dst[0]};
+    int launches = 0;
     for (int pass = 0; pass < 2; ++pass) {
-        hipLaunchKernelGGL(moment_block_kernel, dim3((unsigned)nb), dim3(512), 0, st, s[0], s[1], s[2], mask, n, k,
-                           pass, mom, part, cnt);
-        hipLaunchKernelGGL(moment_final_kernel, dim3(1), dim3(256), 0, st, part, cnt, nb, k, pass, mom);
+        const bool from_sweep = pass == 0 && sweep_partials;
+        if (!from_sweep) {
+            hipLaunchKernelGGL(moment_block_kernel, dim3((unsigned)nb1024), dim3(512), 0, st, s[0], s[1], s[2], mask,
+                               n, k, pass, z.mom, z.part, z.cnt, z.pmm);
+            ++launches;
+        }
+        hipLaunchKernelGGL(moment_final_kernel, dim3(1), dim3(256), 0, st, z.part, z.cnt, z.pmm,
+                           from_sweep ? nb256 : nb1024, from_sweep ? 256 : 1024, n, k, pass, z.mom, z.rmm,
+                           pass == 1 ? box : nullptr);
+        ++launches;
     }
-    if (box) hipLaunchKernelGGL(box_init_kernel, dim3(1), dim3(64), 0, st, box);
-    hipLaunchKernelGGL(normalize_kernel, dim3(std::min<unsigned>(ceil_div(n, 256), 2048u)), dim3(256), 0, st, s[0],
-                       s[1], s[2], d[0], d[1], d[2],
-                       mask, n, k, mom, box);
     CHAOS_HIP(hipGetLastError());
-    ctx.add_launch(5 + (box ? 1 : 0));
-    if (moments) {
-        double h[7] = {};
-        CHAOS_HIP(hipMemcpyAsync(h, mom, (2 * k + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
-        CHAOS_HIP(hipStreamSynchronize(st));
-        for (int j = 0; j < 2 * k; ++j) moments[j] = h[j];
-    } else if (!scratch) {
-        CHAOS_HIP(hipStreamSynchronize(st));  // own scratch is released on return
-    }
+    ctx.add_launch(launches);
+    return z.mom;
 }
 
 }  // namespace chaos

@@ -3,7 +3,7 @@
 // allocate (code under hipGraph capture, tight serving loops).
 //
 // The array holds the workspace of CHAOS_DBSCAN_NOALLOC_MAX_POINTS points
-// with a cell table of 2^18 slots (dbscan_ws_bytes_c; about 47 MB).  Every
+// with a cell table of 2^18 slots (dbscan_ws_bytes_c; about 48 MB).  Every
 // GPU the library runs on holds its own copy until the process ends, and
 // every call on one GPU uses that GPU's copy: calls that overlap on one GPU
 // race.  The kernels initialise every part of it they read, so a call does

@@ -403,8 +403,7 @@ struct SweepProg {
 
     __device__ __forceinline__ void finish(const SweepArgs& A, long long q) const
     {
-        const int B = A.B;
-        const int flag = lane_bit(alive) ? (fixed ? -1 : 1) : 0;
+        const int flag = this->flag();
         if (A.flags) A.flags[q] = flag;
         if (KIND == KIND_BIF) {
             A.counts[q] = cnt;
@@ -432,21 +431,33 @@ struct SweepProg {
             if (A.f2) A.f2[q] = 0.0;
             if (A.counts) A.counts[q] = cnt;
         } else {
-            double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-            if (flag != 0) {
-                f0 = sx / B;
-                f1 = sz / B;
-                const int n = cnt > 1 ? cnt - 1 : 0;
-                if (n >= 2) {
-                    const double m = s0 / n;
-                    const double var = s1 / n - m * m;
-                    f2 = var > 0 ? sqrt(var) : 0.0;
-                }
-            }
-            A.f0[q] = f0;
-            A.f1[q] = f1;
-            A.f2[q] = f2;
+            double f[3];
+            config_features(A, flag, f);
+            A.f0[q] = f[0];
+            A.f1[q] = f[1];
+            A.f2[q] = f[2];
             if (A.counts) A.counts[q] = cnt;
+        }
+    }
+
+    // Lane flag (1 alive, -1 fixed point, 0 diverged).
+    __device__ __forceinline__ int flag() const { return lane_bit(alive) ? (fixed ? -1 : 1) : 0; }
+
+    // CONFIG features (mean X[0], mean X[NX-1], population std of the kept
+    // peaks), all 0 for a diverged lane.
+    __device__ __forceinline__ void config_features(const SweepArgs& A, int flag, double* f) const
+    {
+        const int B = A.B;
+        f[0] = f[1] = f[2] = 0.0;
+        if (flag != 0) {
+            f[0] = sx / B;
+            f[1] = sz / B;
+            const int n = cnt > 1 ? cnt - 1 : 0;
+            if (n >= 2) {
+                const double m = s0 / n;
+                const double var = s1 / n - m * m;
+                f[2] = var > 0 ? sqrt(var) : 0.0;
+            }
         }
     }
 

@@ -14,6 +14,7 @@
 
 #include "dispatch.hpp"
 #include "lanes.hpp"
+#include "moments.hpp"
 #include "sweep.hpp"
 
 namespace chaos {
@@ -32,6 +33,15 @@ void sweep_kernel(const SweepArgs A)
     L.template ticks<WV>(A, WV, cur, wave, 0, SweepProg<SYS, KIND>::total(A));
     if (KIND == KIND_BIF) close_chunk(A.pool, cur);
     if (in_range) L.finish(A, q);
+    if constexpr (KIND == KIND_BASINS_CFG) {
+        // the z-score's first pass, from the features still in registers
+        if (A.zsum) {
+            const int flag = in_range ? L.flag() : 0;
+            double f[3];
+            L.config_features(A, flag, f);
+            block256_moments(f, 3, flag != 0, A.zb, A.zsum, A.zcnt, A.zmm);
+        }
+    }
 }
 
 double divergence_fast_bound(int check, double maxValue)

@@ -46,6 +46,13 @@ struct SweepArgs {
     double* f1;
     double* f2;
     PeakPool pool;        // Bifurcation peak store
+    // Basins CONFIG, one-shot sweep: first moments of the masked features
+    // (flag != 0) per 256-point block for the z-score (zsum [3][zb], zcnt
+    // [zb], zmm [6][zb], zb = the launch's blocks), or zsum NULL
+    double* zsum;
+    int* zcnt;
+    unsigned long long* zmm;
+    long long zb;
 };
 
 // Fast-path bound for the divergence test of each check mode.

@@ -246,7 +246,7 @@ int chaos_dbscan(const chaos_options* opt, const double* x, const double* y,
 /* No counterpart in the reference.  chaos_dbscan for callers that cannot
  * allocate (code under hipGraph capture, tight serving loops): the same
  * kernels and results, with no hipMalloc, hipFree or other allocation -- the
- * clustering scratch is a static device array of the library (about 47 MB
+ * clustering scratch is a static device array of the library (about 48 MB
  * per GPU, held until the process ends).  x, y, z, include and labels are
  * device pointers whatever opt->device_buffers says; the call runs on
  * opt->stream (NULL: the null stream) and fills no opt->stats.  At most

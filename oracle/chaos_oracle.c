@@ -952,24 +952,38 @@ int orc_basins_cluster_ref(const double* f0, const double* f1,
 }
 
 /*
- * Deterministic fixed-order sum shared with the HIP reduction: blocks of 1024
- * consecutive entries (masked-out entries contribute +0), each reduced by the
- * pairwise tree s[t] += s[t+stride] for stride = 512, 256, ..., 1; block sums
- * then added sequentially in block order.
+ * Deterministic fixed-order sum shared with the HIP reduction: the entries
+ * (masked-out ones and the padding contribute +0) padded to P = max(1024,
+ * the power of two >= n) and summed by the complete pairwise tree over
+ * adjacent pairs: level 1 adds (v[2t], v[2t+1]), level 2 the sums of
+ * (v[4t..4t+1], v[4t+2..4t+3]), and so on up to the whole range.  Every
+ * aligned run of 2^k entries is a subtree, so the device may form the sums
+ * of 256- or 1024-entry runs in its blocks and combine them by the same tree.
+ * Pinned by: math.fsum accuracy and an independent recursive evaluation of
+ * the tree (tests/test_oracle_cluster.py).
  */
 double orc_det_sum(const double* x, const int* mask, long long n)
 {
+    long long P = 1024;
+    while (P < n) P <<= 1;
+    /* level 1 on the fly (1024-entry runs), then the upper levels */
+    const long long nr = P / 1024;
+    double* run = (double*)malloc(sizeof(double) * (size_t)nr);
+    if (!run) return NAN;
     double s[1024];
-    double tot = 0.0;
-    for (long long b = 0; b * 1024 < n; ++b) {
+    for (long long r = 0; r < nr; ++r) {
         for (int t = 0; t < 1024; ++t) {
-            long long i = b * 1024 + t;
+            const long long i = r * 1024 + t;
             s[t] = (i < n && (!mask || mask[i] != 0)) ? x[i] : 0.0;
         }
-        for (int stride = 512; stride > 0; stride >>= 1)
-            for (int t = 0; t < stride; ++t) s[t] = s[t] + s[t + stride];
-        tot = tot + s[0];
+        for (int w = 1024; w > 1; w >>= 1)
+            for (int t = 0; t < w / 2; ++t) s[t] = s[2 * t] + s[2 * t + 1];
+        run[r] = s[0];
     }
+    for (long long w = nr; w > 1; w >>= 1)
+        for (long long t = 0; t < w / 2; ++t) run[t] = run[2 * t] + run[2 * t + 1];
+    const double tot = run[0];
+    free(run);
     return tot;
 }
 

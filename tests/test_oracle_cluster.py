@@ -161,6 +161,40 @@ def test_det_sum_accuracy_and_mask():
         assert abs(O.det_sum(x, m) - math.fsum(x[m != 0])) <= 1e-12 * max(1.0, np.abs(x).sum())
 
 
+def _pairwise(v):
+    """The complete adjacent-pair tree, evaluated recursively: the sum of a
+    run is the sum of its two halves."""
+    if len(v) == 1:
+        return float(v[0])
+    h = len(v) // 2
+    return _pairwise(v[:h]) + _pairwise(v[h:])
+
+
+def test_det_sum_is_the_padded_pairwise_tree():
+    """orc_det_sum pads to max(1024, next power of two) with +0 and sums by
+    the complete pairwise tree; values of mixed magnitude make the order
+    visible (sequential and blocked orders give other bits)."""
+    rng = np.random.default_rng(11)
+    differs = 0
+    for n in (1, 7, 1000, 1024, 1025, 3000, 5000):
+        x = rng.normal(0, 1, n) * 10.0 ** rng.integers(-8, 9, n)
+        m = (rng.random(n) < 0.7).astype(np.int32)
+        P = 1024
+        while P < n:
+            P *= 2
+        for mask in (None, m):
+            v = np.zeros(P)
+            v[:n] = x if mask is None else np.where(mask != 0, x, 0.0)
+            want = _pairwise(list(v))
+            got = O.det_sum(x) if mask is None else O.det_sum(x, mask)
+            assert got == want
+            seq = 0.0
+            for t in v:
+                seq = seq + t
+            differs += seq != want
+    assert differs > 0
+
+
 def test_zscore():
     rng = np.random.default_rng(4)
     f = [rng.normal(3, 2, 3000) for _ in range(3)]
