@@ -494,24 +494,42 @@ void Engine::allocate(JobRec& r)
 
 void Engine::admit()
 {
-    while (!pending_.empty() && !free_slots_.empty()) {
+    const int m = (int)std::min(pending_.size(), free_slots_.size());
+    if (m == 0) return;
+    std::vector<long long> fp(m), tl(m);
+    for (int i = 0; i < m; ++i) {
+        const JobRec& r = *jobs_[pending_[i]];
+        fp[i] = r.footprint;
+        tl[i] = r.tiles;
+    }
+    int k;
+    {
+        // the worker thread only lowers hbm_used_, so the rule sees at most
+        // the footprints still charged
+        std::lock_guard<std::mutex> lk(mu_);
+        k = admit_fifo(fp.data(), tl.data(), m, (int)free_slots_.size(), &hbm_used_, hbm_budget_, &active_tiles_,
+                       prm_.max_tiles);
+    }
+    for (int i = 0; i < k; ++i) {
         JobRec& r = *jobs_[pending_.front()];
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            if (hbm_used_ + r.footprint > hbm_budget_) break;
-            hbm_used_ += r.footprint;
-        }
-        if (active_tiles_ + r.tiles > prm_.max_tiles) {
-            std::lock_guard<std::mutex> lk(mu_);
-            hbm_used_ -= r.footprint;
-            break;
-        }
         r.slot = free_slots_.back();
         free_slots_.pop_back();
-        active_tiles_ += r.tiles;
         r.status = 1;
         r.t_admit = now_ms();
-        allocate(r);
+        try {
+            allocate(r);
+        } catch (...) {
+            // this job and the ones after it stay pending, uncharged
+            free_slots_.push_back(r.slot);
+            r.slot = -1;
+            r.status = 0;
+            std::lock_guard<std::mutex> lk(mu_);
+            for (int j = i; j < k; ++j) {
+                hbm_used_ -= fp[j];
+                active_tiles_ -= tl[j];
+            }
+            throw;
+        }
         active_.push_back(r.id);
         pending_.pop_front();
     }

@@ -11,6 +11,7 @@
 #include <thread>
 #include <vector>
 
+#include "admission.hpp"
 #include "bifurcation.hpp"
 #include "lanes.hpp"
 #include "lle.hpp"
