@@ -1309,8 +1309,8 @@ int dbscan_device(CallCtx& ctx, const double* const* pts, const int* include, lo
         if (c.flags & DB_RANGE) fail(CHAOS_E_UNSUPPORTED, "feature range too wide for eps %.3g", prm.eps);
         if (std::getenv("CHAOS_DBSCAN_TRACE"))
             std::fprintf(stderr, "dbscan: %lld points, %d cells of %lld slots, %d sparse-cell points, "
-                         "%d core cells, %d clusters, flags %d\n", n, c.ncells, H, c.ncand, c.ncore_cells, c.K,
-                         c.flags);
+                         "%d core cells, %d clusters, %d split pieces of %lld, flags %d\n", n, c.ncells, H,
+                         c.ncand, c.ncore_cells, c.K, c.npieces, core_piece_cap(n), c.flags);
         if (!(c.flags & DB_FULL)) return c.K;
         if (H >= dbscan_max_cells(n)) fail(CHAOS_E_HIP, "cell table full at %lld cells", H);
         H = dbscan_max_cells(n);

@@ -119,3 +119,43 @@ def test_dbscan_large_crowds_matches_bruteforce(chaos, oracle):
     got, _ = chaos.dbscan(pts, 0.5, 20)
     want = oracle.dbscan_bruteforce(pts, 0.5, 20)
     assert np.array_equal(got, want)
+
+
+def test_dbscan_split_core_tests_match_bruteforce(chaos, oracle, capfd, monkeypatch):
+    """Core tests too long for one wavefront: 300 candidates in a shell 0.42-0.5
+    from a cloud of 20 000 points in one cell-sized box, minPts 300, so each
+    candidate's count is decided only by scanning much of the cloud (a lone
+    point at (-0.88, -0.88, -0.88) sets the grid origin so that the whole
+    cloud falls in one cell of side 0.5 / sqrt(3)).  The
+    candidates scan each cell's head themselves, then split the cloud into
+    4096-member pieces; the piece list (n/16 + 64 entries) fills, so the last
+    candidates fall back to scanning in their own wavefront.  Labels must
+    equal the brute force, and the candidates must be a mix of core and
+    non-core points."""
+    import re
+    rng = np.random.default_rng(5)
+    cloud = rng.uniform(0, 0.25, (20000, 3))
+    cands = []
+    while len(cands) < 300:
+        p = rng.uniform(-0.6, 0.85, 3)
+        g = np.maximum(0, np.maximum(-p, p - 0.25))
+        if 0.42 <= np.sqrt((g ** 2).sum()) <= 0.5:
+            cands.append(p)
+    cands = np.array(cands)
+    pts = np.concatenate([cloud, cands, np.full((1, 3), -0.88)])
+    pts = pts[rng.permutation(len(pts))]
+    monkeypatch.setenv("CHAOS_DBSCAN_TRACE", "1")
+    lo, side = pts.min(0), 0.5 / np.sqrt(3.0) * (1 - 1e-7)
+    assert len(np.unique(np.floor((cloud - lo) / side), axis=0)) == 1
+    print(f"dbscan split core tests n={len(pts)} min_pts=300", flush=True)
+    got, k = chaos.dbscan(pts, 0.5, 300)
+    line = [l for l in capfd.readouterr().err.splitlines() if l.startswith("dbscan:")][-1]
+    want = oracle.dbscan_bruteforce(pts, 0.5, 300)
+    # neighbour counts of the candidates, summed in coordinate order as the kernels do
+    d = pts[None, :, :] - cands[:, None, :]
+    cnt = (np.sqrt((d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]) <= 0.5).sum(1)
+    print(line, f"; core candidates {(cnt >= 300).sum()} of 300", flush=True)
+    assert np.array_equal(got, want)
+    pieces, cap = map(int, re.search(r"(\d+) split pieces of (\d+)", line).groups())
+    assert pieces > 0 and pieces + 5 > cap  # the list was used and filled
+    assert 0 < (cnt >= 300).sum() < 300
