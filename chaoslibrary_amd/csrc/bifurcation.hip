@@ -201,7 +201,8 @@ static void run_bif(CallCtx& ctx, const chaos_options& o, const BifInputs& in, B
         pool.alloc(ctx.stream(), nchunks);
         A.pool = pool.view();
         ctx.begin();
-        launch_sweep(ctx, o.system, in.writableVar, KIND_BIF, A);
+        // a map that clusters peak values only leaves the detector's time logic out
+        launch_sweep(ctx, o.system, in.writableVar, R.need_times ? KIND_BIF : KIND_BIF_VALS, A);
         ctx.add_kernel_ms(ctx.end());
         ctx.add_steps((double)n * ((double)A.S + (double)A.B * A.pre));
         if (bif_post(ctx, R, pool)) return;

@@ -212,8 +212,10 @@ struct SweepProg {
     static constexpr int ND = NX + 8;  // X, p, dn, first, last, s0, s1, sx, sz
     static constexpr int NI = 4;       // pstart, cnt, prev_t, bits
     double X[NX], a[NA];
-    // CONFIG basins use peak values only
-    WavePeakDetector<KIND != KIND_BASINS_CFG> det;
+    static constexpr bool BIF = stores_peaks(KIND);
+    // peak times: the intervals of the bifurcation lists and of the
+    // reference basins features; the other kinds use peak values only
+    WavePeakDetector<KIND == KIND_BIF || KIND == KIND_BASINS_REF> det;
     int cnt, prev_t;
     unsigned long long alive;  // wave mask of the lanes still integrating
     bool fixed;
@@ -231,7 +233,7 @@ struct SweepProg {
     // Parameter vector of point pq.
     __device__ __forceinline__ void params(const SweepArgs& A, long long pq)
     {
-        if (KIND == KIND_BIF && !A.ic_sweep) {
+        if (BIF && !A.ic_sweep) {
             // swept entries of the parameter vector (bifurcationCUDA.cu:282-284)
             double g0, g1;
             grid(A, pq, g0, g1);
@@ -245,7 +247,7 @@ struct SweepProg {
     __device__ __forceinline__ void init(const SweepArgs& A, long long pq, bool in_range)
     {
         params(A, pq);
-        if (KIND == KIND_BIF && !A.ic_sweep) {
+        if (BIF && !A.ic_sweep) {
 #pragma unroll
             for (int k = 0; k < NX; ++k) X[k] = A.ic[k];
         } else {
@@ -296,7 +298,7 @@ struct SweepProg {
         // em is wave-uniform: a wave with no peak at this sample skips the
         // per-peak work altogether
         if (em) {
-            if (KIND == KIND_BIF) {
+            if (BIF) {
                 append_peaks(A.pool, cur, wave, em, pv, pt, cnt);
                 if (e) ++cnt;
             } else if (KIND == KIND_BASINS_REF) {
@@ -342,7 +344,7 @@ struct SweepProg {
     {
         long long g = g0;
         // ---- transient ----
-        if (KIND == KIND_BIF && A.legacy_transient) {
+        if (BIF && A.legacy_transient) {
             // legacy IC sweeps (debri/cudaLibrary.cu:1063-1070): the transient
             // tests X[0] for NaN/inf only (maxValue 0) and a fixed point at its
             // end is still recorded
@@ -354,7 +356,7 @@ struct SweepProg {
                 }
                 g = e;
             }
-        } else if (KIND == KIND_BIF) {
+        } else if (BIF) {
             // checked every step; a transient that ends diverged or at a
             // fixed point is not recorded and reports 0 (bifurcationCUDA.cu:334-342)
             if (g < A.S - 1) {
@@ -405,7 +407,7 @@ struct SweepProg {
     {
         const int flag = this->flag();
         if (A.flags) A.flags[q] = flag;
-        if (KIND == KIND_BIF) {
+        if (BIF) {
             A.counts[q] = cnt;
             A.first[q] = first;
         } else if (KIND == KIND_BASINS_REF) {

@@ -31,7 +31,7 @@ void sweep_kernel(const SweepArgs A)
     WaveCursor cur{A.pool.nchunks, kChunk};
     // every lane of the wave stays in the loop (wave-level peak appends)
     L.template ticks<WV>(A, WV, cur, wave, 0, SweepProg<SYS, KIND>::total(A));
-    if (KIND == KIND_BIF) close_chunk(A.pool, cur);
+    if (stores_peaks(KIND)) close_chunk(A.pool, cur);
     if (in_range) L.finish(A, q);
     if constexpr (KIND == KIND_BASINS_CFG) {
         // the z-score's first pass, from the features still in registers
@@ -76,6 +76,9 @@ void launch_sweep(CallCtx& ctx, int system, int writableVar, int kind, const Swe
             if constexpr (WV < NXs) {
                 if (kind == KIND_BIF)
                     hipLaunchKernelGGL((sweep_kernel<S, WV, KIND_BIF>), dim3(grid), dim3(block), 0, ctx.stream(), A);
+                else if (kind == KIND_BIF_VALS)
+                    hipLaunchKernelGGL((sweep_kernel<S, WV, KIND_BIF_VALS>), dim3(grid), dim3(block), 0,
+                                       ctx.stream(), A);
                 else if (kind == KIND_BASINS_REF)
                     hipLaunchKernelGGL((sweep_kernel<S, WV, KIND_BASINS_REF>), dim3(grid), dim3(block), 0,
                                        ctx.stream(), A);

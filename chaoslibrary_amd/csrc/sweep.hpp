@@ -11,8 +11,12 @@ namespace chaos {
 enum SweepKind : int {
     KIND_BIF = 0,          // store every peak (value, time) -> Bifurcation 1D/2D
     KIND_BASINS_REF = 1,   // mean peak / mean interval (+ sentinels)
-    KIND_BASINS_CFG = 2    // mean X[0], mean X[NX-1], peak std
+    KIND_BASINS_CFG = 2,   // mean X[0], mean X[NX-1], peak std
+    KIND_BIF_VALS = 3      // store every peak's value (times left 0) -> the CONFIG peak-count map
 };
+
+// Kinds that store peaks in the pool.
+constexpr bool stores_peaks(int kind) { return kind == KIND_BIF || kind == KIND_BIF_VALS; }
 
 enum CheckMode : int {
     CHK_BIF_REF = 0,     // v!=v || v==v+1 ; (double)fabsf((float)v) > max ; sqrtf(fabsf((float)t)) < 1e-9
