@@ -328,7 +328,8 @@ def run_bif2d(args):
     res["config"] = {"workload": f"bif2d_lorenz_rk4_{n}x{n}", "grid": n, "transient_time": 500.0, "tMax": 500.0,
                      "h": 0.01, "dbscan_eps": 1e-2, "peaks": stats.peaks, "semantics": sem}
     summary = f"sweep_kernel_bif2d_lorenz_{n}" if n == 1024 and sem == 1 else None
-    res["roofline"] = roofline("lorenz_rk4", steps / args.steps, kms / 1e3, "sweep_kernel<6,0,0>", kms, summary)
+    res["roofline"] = roofline("lorenz_rk4", steps / args.steps, kms / 1e3,
+                               "sweep_kernel<6,0,3>" if sem == 1 else "sweep_kernel<6,0,0>", kms, summary)
     res["analysis_ms"] = ams
     # peak output and 1-D DBSCAN stages: measured HBM bytes over their live time
     ab = profile_figure(summary, "analysis_bytes_per_run") if summary else None
