@@ -639,7 +639,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1, help="untimed runs")
     ap.add_argument("--workload", default="lle", choices=["lle", "bif1d", "bif2d", "basins", "stream", "copy", "ls",
                                                            "regression", "tune"])
-    ap.add_argument("--tune-out", default=os.path.join("gpurun_out", "engine_tune.txt"),
+    ap.add_argument("--tune-out", default=os.path.join("workspace", "engine_tune.txt"),
                     help="tune: where the autotuner writes its cache (chosen parameters + candidates)")
     ap.add_argument("--grid", type=int, default=2048)
     ap.add_argument("--block", type=int, default=0)
