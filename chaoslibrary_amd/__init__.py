@@ -56,6 +56,25 @@ class Options(C.Structure):
 OUT_CSV, OUT_BINARY = 1, 2
 
 
+# HIP maps a process's streams onto GPU_MAX_HW_QUEUES hardware queues (4 by
+# default).  The job server integrates on up to four streams and finishes jobs
+# on two more; with four queues a finishing stream shares a queue with an
+# integration stream, and its kernels wait behind whole persistent rounds
+# (on a saturated stream the finishing falls behind and the median latency
+# doubles).  HIP reads the variable when it starts, so it is raised here,
+# before the library loads, unless the process asked for at least as many.
+HW_QUEUES = 16
+
+
+def _raise_hw_queues():
+    v = os.environ.get("GPU_MAX_HW_QUEUES", "")
+    if not v.isdigit() or int(v) < HW_QUEUES:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(HW_QUEUES)
+
+
+_raise_hw_queues()
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"chaoslibrary_amd: {LIB_PATH} is missing; build it with "

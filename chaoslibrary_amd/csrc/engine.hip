@@ -245,16 +245,19 @@ Engine::Engine(int device, const chaos_engine_params& prm) : prm_(prm), device_(
     grid_ = std::max(1, waves / (prm_.block / 64));
     const size_t free_b = dev_mem_available();
     hbm_budget_ = (long long)(prm_.hbm_fraction * (double)free_b);
-    CHAOS_HIP(hipStreamCreateWithFlags(&fstream_, hipStreamNonBlocking));  // default priority
+    for (hipStream_t& fs : fstreams_)
+        CHAOS_HIP(hipStreamCreateWithFlags(&fs, hipStreamNonBlocking));  // default priority
     t0_ = steady_ms();
-    worker_ = std::thread([this] { finisher(); });
+    for (int k = 0; k < kFinishers; ++k) workers_.emplace_back([this, k] { finisher(k); });
 }
 
-// Worker: finish jobs in completion order on fstream_.  A failure marks the
-// job failed with its message; the engine keeps serving the others.
-void Engine::finisher()
+// Worker k: takes finished jobs in completion order and finishes them on its
+// stream.  A failure marks the job failed with its message; the engine keeps
+// serving the others.
+void Engine::finisher(int k)
 {
     (void)hipSetDevice(device_);
+    hipStream_t fstream = fstreams_[k];
     std::unique_ptr<CallCtx> fctx;
     for (;;) {
         JobRec* r = nullptr;
@@ -272,14 +275,14 @@ void Engine::finisher()
             if (!fctx) {
                 chaos_options o = resolve_options(nullptr);
                 o.device = device_;
-                o.stream = fstream_;
+                o.stream = fstream;
                 fctx.reset(new CallCtx(o));
             }
             finalize(*r, *fctx);
         } catch (const std::exception& e) {
             st = -1;
             err = e.what();
-            (void)hipStreamSynchronize(fstream_);
+            (void)hipStreamSynchronize(fstream);
         }
         {
             std::lock_guard<std::mutex> lk(mu_);
@@ -313,8 +316,10 @@ Engine::~Engine()
         stop_ = true;
     }
     cv_.notify_all();
-    if (worker_.joinable()) worker_.join();
-    if (fstream_) (void)hipStreamDestroy(fstream_);
+    for (std::thread& w : workers_)
+        if (w.joinable()) w.join();
+    for (hipStream_t fs : fstreams_)
+        if (fs) (void)hipStreamDestroy(fs);
     jobs_.clear();
     for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
     graphs_.clear();

@@ -130,7 +130,7 @@ private:
     void enqueue_round(unsigned mask);
     void launch_family(int f, hipStream_t s);
     void fail_active(const std::string& msg);
-    void finisher();                 // worker thread body
+    void finisher(int k);            // body of worker thread k
     int take_completed();            // jobs finished since the last call
     void wait_finisher_progress();   // block until a finishing job completes
 
@@ -163,15 +163,22 @@ private:
     long long launches_ = 0, graph_replays_ = 0, tiles_ = 0;
     double busy_ms_ = 0, finalize_ms_ = 0, steps_done_ = 0, t0_ = 0;
     // Finishing stages (peak post-processing, clustering, result copies) run
-    // on a worker thread with its own stream, overlapping the next rounds.
+    // on kFinishers worker threads, each with its own stream, overlapping the
+    // next rounds; the jobs waiting for them keep their HBM charged, so
+    // finishing that falls behind stalls admission.  A finishing stream must
+    // not share a hardware queue with a busy integration stream, or its
+    // kernels wait behind whole persistent rounds: the process needs
+    // GPU_MAX_HW_QUEUES >= 8 (HIP's default is 4; the Python package raises
+    // it to 16 before HIP starts).
     // mu_ guards finq_, hbm_used_, finalize_ms_, steps_done_ and stop_.
+    static constexpr int kFinishers = 2;
     mutable std::mutex mu_;
     std::condition_variable cv_;
     std::deque<JobRec*> finq_;
     std::atomic<int> finishing_{0}, completed_{0};
     bool stop_ = false;
-    hipStream_t fstream_ = nullptr;
-    std::thread worker_;
+    hipStream_t fstreams_[kFinishers] = {};
+    std::vector<std::thread> workers_;
 };
 
 }  // namespace chaos
