@@ -223,7 +223,7 @@ ENGINE_TUNED = ("block", "waves_per_simd", "steps_per_launch", "points_per_tile"
 
 def engine_params(args):
     """Engine parameters: the library defaults, overridden by the autotuner's
-    choice recorded in profiles/engine_autotune_calibration_mix.md (lines
+    choice recorded in profiles/engine_autotune_served_stream.md (lines
     `tuned_<field>: value`; --untuned skips them), then --spl, then --engine
     "key=value,..." (block, waves_per_simd, steps_per_launch,
     points_per_tile, use_graph, max_jobs, max_tiles)."""
@@ -231,7 +231,7 @@ def engine_params(args):
     prm = ch.engine_params()
     if not getattr(args, "untuned", False):
         for k in ENGINE_TUNED:
-            v = profile_figure("engine_autotune_calibration_mix", "tuned_" + k)
+            v = profile_figure("engine_autotune_served_stream", "tuned_" + k)
             if v is not None:
                 setattr(prm, k, int(v))
     if args.spl:
@@ -440,9 +440,9 @@ def run_stream(args):
 
 def run_tune(args):
     """The engine autotuner on this GPU: each candidate serves the same short
-    seeded stream (200 jobs/s over 5 s, horizons x0.3) and is scored by the
-    mean job latency; every candidate's latency and rate, and the chosen
-    parameters (also written to --tune-out)."""
+    seeded stream (200 jobs/s over 1.5 s, full horizons) and is scored by its
+    power, RK4 steps/s over the mean job latency; every candidate's rate and
+    latency, and the chosen parameters (also written to --tune-out)."""
     import chaoslibrary_amd as ch
     path = args.tune_out
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
@@ -460,11 +460,12 @@ def run_tune(args):
     log(f"autotune: {len(cands)} candidates in {wall:.1f} s; chosen block {prm.block} waves {prm.waves_per_simd} "
         f"steps {prm.steps_per_launch} points/tile {prm.points_per_tile} graph {prm.use_graph} "
         f"(mean latency {mean_ms} ms, {rate:.3e} steps/s)")
-    return {"metric": "engine autotuner: mean job latency of a short served stream per candidate", "value": mean_ms,
-            "unit": "ms", "n_gpus": 1, "steps": len(cands), "warmup": 1, "ms_per_step": wall * 1e3 /
-            max(1, len(cands)), "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f64",
-            "data": "synthetic seeded stream (seed 7): 200 jobs/s over 5 s, bifurcation / LLE / basins kinds and "
-                    "sizes of the served stream, horizons x0.3",
+    power = rate / mean_ms if mean_ms else None
+    return {"metric": "engine autotuner: power (RK4 steps/s per ms of mean job latency) of a short served stream",
+            "value": power, "unit": "RK4 steps/s per ms", "n_gpus": 1, "steps": len(cands), "warmup": 1,
+            "ms_per_step": wall * 1e3 / max(1, len(cands)), "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic seeded stream (seed 7): 200 jobs/s over 1.5 s, bifurcation / LLE / basins kinds and "
+                    "sizes of the served stream, full horizons",
             "config": {"workload": "engine_autotune_served_stream"},
             "chosen": {"block": prm.block, "waves_per_simd": prm.waves_per_simd,
                        "steps_per_launch": prm.steps_per_launch, "points_per_tile": prm.points_per_tile,

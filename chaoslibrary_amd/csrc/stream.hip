@@ -362,18 +362,20 @@ extern "C" int chaos_stream_jobs(const chaos_stream_config* cfg, chaos_job* jobs
 // ---------------------------------------------------------------------------
 
 // Tuning workload: a seeded stream of the served kinds and sizes at the
-// configurations' physics (200 jobs/s over 5 s, horizons x0.3: about 1000
-// jobs and 1e13 RK4 steps, 2.6x what the GPU completes in those 5 s, so a
-// backlog builds as in the configured stream), served through the engine.
-// Scored by the mean job latency: with the same arrivals for every
-// candidate, that rewards throughput and the order of completion together.
-// A batch submitted at once, or a stream too short to build a backlog,
-// rewards long launches, which on a saturated stream complete jobs in bursts
-// and double the median latency.  The seed differs from the benchmark
-// stream's.
+// configurations' physics and full horizons (200 jobs/s over 1.5 s: about
+// 300 jobs and 1e13 RK4 steps, 8x what the GPU completes in those 1.5 s, so
+// a backlog builds as in the configured stream), served through the engine.
+// Full horizons matter: the best steps per launch grows with job length.
+// Scored by its power, RK4 steps per second over the makespan divided by
+// the mean job latency (Kleinrock's throughput / delay): a server wants
+// both, and on a saturated stream either alone is nearly flat across the
+// candidates.  A batch submitted at once, or a stream too short to build a
+// backlog, measures neither as served.  The seed differs from the
+// benchmark stream's.
 struct Calib {
-    double mean_ms;       // mean job latency (the objective, lower is better)
+    double mean_ms;       // mean job latency
     double steps_per_s;   // RK4 steps over the makespan
+    double power() const { return steps_per_s / mean_ms; }  // the objective, higher is better
 };
 
 static Calib calibrate(int device, const chaos_engine_params& prm)
@@ -381,8 +383,8 @@ static Calib calibrate(int device, const chaos_engine_params& prm)
     chaos_stream_config c;
     chaos_stream_config_default(&c);
     c.lambda = 200.0;
-    c.duration_s = 5.0;
-    c.horizon_scale = 0.3;
+    c.duration_s = 1.5;
+    c.horizon_scale = 1.0;
     c.seed = 7;
     c.mode = 0;
     c.progress_s = 0.0;
@@ -424,7 +426,7 @@ extern "C" int chaos_autotune(int device, const char* cache_path, chaos_engine_p
                 chaos_engine_params p = b;
                 set(p, v);
                 const Calib r = eval(p);
-                if (r.mean_ms < best_c.mean_ms) {
+                if (r.power() > best_c.power()) {
                     best_c = r;
                     b = p;
                 }
