@@ -456,7 +456,7 @@ typedef struct chaos_server_stats {
     int reserved;
 } chaos_server_stats;
 
-/* Defaults: block 256, 4 waves per SIMD, 10000 steps per launch, 64 points
+/* Defaults: block 256, 4 waves per SIMD, 40000 steps per launch, 64 points
  * per tile, graph replay, 256 job slots, 65536 tiles, 60 % of free HBM (the
  * autotuner's choice on an MI355X; chaos_autotune measures others). */
 int chaos_engine_params_default(chaos_engine_params* p);

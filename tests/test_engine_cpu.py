@@ -4,7 +4,7 @@ import pytest
 
 def test_engine_params_default(chaos):
     p = chaos.engine_params()
-    assert (p.block, p.waves_per_simd, p.steps_per_launch, p.use_graph) == (256, 4, 10000, 1)
+    assert (p.block, p.waves_per_simd, p.steps_per_launch, p.use_graph) == (256, 4, 40000, 1)
     assert p.points_per_tile == 64
     assert p.max_jobs >= 1 and p.max_tiles >= 1 and 0 < p.hbm_fraction <= 1
 
