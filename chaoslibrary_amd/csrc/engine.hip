@@ -116,16 +116,19 @@ __device__ __forceinline__ bool tile_ok(const JobDev& J, const Tile& T, int fam,
     return true;
 }
 
-// Persistent loop of one family: every wave claims `take` consecutive tiles
-// of its family's queue per atomic (points per tile = 64 x take) until the
-// queue is empty; each wave reaches the exit test after every claim, so the
-// grid drains once the last tile is taken.
+// Persistent loop of one family: every taking part wave (global index <
+// Q->waves, family_waves) claims `take` consecutive tiles of its family's
+// queue per atomic (points per tile = 64 x take) until the queue is empty;
+// each wave reaches the exit test after every claim, so the grid drains once
+// the last tile is taken.
 template <class Body>
 __device__ __forceinline__ void drain_family(const Tile* __restrict__ tiles, RoundCtl* __restrict__ ctl, int fam,
                                              Body&& body)
 {
     const int lane = threadIdx.x & 63;
     FamQueue* Q = &ctl->fam[fam];
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (wave >= Q->waves) return;
     const int off = Q->off, n = Q->n, take = ctl->take;
     for (;;) {
         unsigned t = 0;
@@ -733,8 +736,9 @@ int Engine::step()
     int fill[kFamilies];
     unsigned mask = 0;
     int nt = 0;
+    const long long grid_waves = (long long)grid_ * (prm_.block / 64);
     for (int f = 0; f < kFamilies; ++f) {
-        h_ctl_->fam[f] = FamQueue{nt, cnt[f], 0u, 0};
+        h_ctl_->fam[f] = FamQueue{nt, cnt[f], 0u, family_waves(cnt[f], gpt_, grid_waves)};
         fill[f] = nt;
         nt += cnt[f];
         if (cnt[f]) mask |= 1u << f;

@@ -58,12 +58,13 @@ __host__ __device__ inline int engine_family(int prog, int lorenz, int wv)
 }
 
 // Per-round queue of one family: its tiles are tiles[off, off + n), taken
-// in order through the atomic head.
+// in order through the atomic head by the family kernel's wavefronts of
+// global index < waves (the others return at once).
 struct FamQueue {
     int off;
     int n;
     unsigned head;
-    int pad;
+    int waves;  // family_waves (admission.hpp)
 };
 
 // Everything the host hands the device per round in one copy: the family

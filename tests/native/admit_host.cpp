@@ -7,3 +7,8 @@ extern "C" int admit_host(const long long* footprint, const long long* tiles, in
 {
     return chaos::admit_fifo(footprint, tiles, n, free_slots, hbm_used, hbm_budget, active_tiles, max_tiles);
 }
+
+extern "C" int family_waves_host(long long tiles, int take, long long grid_waves)
+{
+    return chaos::family_waves(tiles, take, grid_waves);
+}

@@ -8,7 +8,8 @@ the queue; the first job that does not fit stops admission (no overtaking).
 A simulated serving loop on top of it (jobs finish after a fixed number of
 rounds and release their charges) checks that every job is eventually
 admitted, in submission order, and that no round ever exceeds the slot, HBM
-or tile limits.
+or tile limits.  The round sizing of the same header (family_waves) is
+checked here too.
 """
 import ctypes as C
 import os
@@ -23,13 +24,21 @@ _LL = C.POINTER(C.c_longlong)
 
 
 @pytest.fixture(scope="module")
-def admit(tmp_path_factory):
+def native(tmp_path_factory):
     if shutil.which("g++") is None:
         pytest.skip("g++ not available")
     so = str(tmp_path_factory.mktemp("admit") / "admit_host.so")
     subprocess.run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-I", os.path.join(ROOT, "chaoslibrary_amd", "csrc"),
                     os.path.join(ROOT, "tests", "native", "admit_host.cpp"), "-o", so], check=True)
     lib = C.CDLL(so)
+    lib.family_waves_host.restype = C.c_int
+    lib.family_waves_host.argtypes = [C.c_longlong, C.c_int, C.c_longlong]
+    return lib
+
+
+@pytest.fixture(scope="module")
+def admit(native):
+    lib = native
     lib.admit_host.restype = C.c_int
     lib.admit_host.argtypes = [_LL, _LL, C.c_int, C.c_int, _LL, C.c_longlong, _LL, C.c_longlong]
 
@@ -112,3 +121,23 @@ def test_serving_loop_admits_every_job_in_order(admit):
                 finishing.append(j)
     assert order == list(range(n))
     assert used == 0 and act_tiles == 0
+
+
+def test_family_waves_round_sizing(native):
+    """family_waves (admission.hpp), the wavefronts of a family kernel that
+    take part in a round: no tiles -> 0; otherwise one per claim of `take`
+    tiles, at most the grid's wavefronts."""
+    fw = native.family_waves_host
+    assert fw(0, 1, 4096) == 0
+    assert fw(1, 1, 4096) == 1
+    assert fw(1407, 1, 4096) == 1407   # a 300x300 job: 1407 64-point groups
+    assert fw(1407, 2, 4096) == 704    # 128-point claims
+    assert fw(1407, 4, 4096) == 352
+    assert fw(4096, 1, 4096) == 4096
+    assert fw(5625, 1, 4096) == 4096   # a 600x600 job: the whole grid
+    for tiles in [1, 2, 63, 64, 65, 1000, 4095, 4096, 4097, 8191, 8192, 8193, 12289, 10 ** 6]:
+        for take in [1, 2, 4]:
+            for g in [1, 7, 64, 1024, 4096, 8192]:
+                w = fw(tiles, take, g)
+                claims = -(-tiles // take)
+                assert w == min(claims, g) and w >= 1
