@@ -13,6 +13,8 @@
 //        reference's per-point dbscan on the kept intervals.
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <cstdlib>
 #include <memory>
 #include <vector>
 
@@ -108,6 +110,19 @@ SweepArgs make_bif_args(const chaos_options& o, const BifInputs& in)
     return A;
 }
 
+// Device bytes of one chunk of the peak store (entries + its two meta words).
+constexpr size_t kChunkBytes = kChunk * sizeof(PeakEntry) + 2 * sizeof(int);
+
+// Cap on the device memory of one sweep's peak store: CHAOS_PEAK_STORE_BYTES
+// when the process sets it, else none.
+static size_t peak_store_cap()
+{
+    const char* s = std::getenv("CHAOS_PEAK_STORE_BYTES");
+    if (!s || !*s) return SIZE_MAX;
+    const long long v = std::atoll(s);
+    return v > 0 ? (size_t)v : SIZE_MAX;
+}
+
 long long bif_initial_chunks(const SweepArgs& A)
 {
     const long long n = A.n_points, waves = (n + 63) / 64;
@@ -116,20 +131,50 @@ long long bif_initial_chunks(const SweepArgs& A)
     // an overflow is detected on the device and the sweep rerun with the
     // exact per-wave need.
     const double max_pp = std::min((double)A.B / 2.0 + 1.0, std::max(256.0, (double)A.B / 32.0));
-    const long long budget_chunks = (long long)(0.25 * (double)free_b / (kChunk * sizeof(PeakEntry) + 8));
+    const size_t budget = std::min((size_t)(0.25 * (double)free_b), peak_store_cap());
+    const long long budget_chunks = (long long)(budget / kChunkBytes);
     const long long want = (long long)std::ceil((double)n * max_pp / (kChunk - 64)) + waves;
     return std::max<long long>(waves, std::min(want, budget_chunks));
 }
 
+// Exact store of one 64-point wave holding s peaks: chunks of kChunk - 64
+// usable entries plus one for the chunk switch.
+static long long wave_chunks(long long s) { return (s + (kChunk - 64) - 1) / (kChunk - 64) + 1; }
+
 long long bif_exact_chunks(const std::vector<int>& counts, long long n)
 {
     long long need = 0;
-    for (long long w = 0; w * 64 < n; ++w) {
+    for (long long w = 0; w < n; w += 64) {
         long long s = 0;
-        for (long long p = w * 64; p < std::min(n, w * 64 + 64); ++p) s += counts[p];
-        need += (s + (kChunk - 64) - 1) / (kChunk - 64) + 1;
+        for (long long p = w; p < std::min(n, w + 64); ++p) s += counts[p];
+        need += wave_chunks(s);
     }
     return need;
+}
+
+// Ranges of whole 64-point waves whose exact passes each fit `budget`
+// device bytes: the peak store, 32 bytes per stored peak (scattered values
+// and times, the 2-D analysis's scratch) and 128 per point (flags, counts,
+// offsets, class lists).  Returns the range starts and n: [s[k], s[k + 1]).
+static std::vector<long long> exact_ranges(const std::vector<int>& counts, long long n, size_t budget)
+{
+    std::vector<long long> starts{0};
+    size_t bytes = 0;
+    for (long long w = 0; w < n; w += 64) {
+        const long long e = std::min(n, w + 64);
+        long long s = 0;
+        for (long long p = w; p < e; ++p) s += counts[p];
+        const size_t wb = (size_t)wave_chunks(s) * kChunkBytes + 32 * (size_t)s + 128 * (size_t)(e - w);
+        CHAOS_REQUIRE(wb <= budget, "the peaks of 64 points need %zu bytes of device memory, more than the %zu "
+                      "available", wb, budget);
+        if (bytes + wb > budget) {
+            starts.push_back(w);
+            bytes = 0;
+        }
+        bytes += wb;
+    }
+    starts.push_back(n);
+    return starts;
 }
 
 bool bif_post(CallCtx& ctx, BifDevice& R, PeakPoolBuf& pool)
@@ -182,9 +227,12 @@ bool bif_post(CallCtx& ctx, BifDevice& R, PeakPoolBuf& pool)
     return true;
 }
 
-static void run_bif(CallCtx& ctx, const chaos_options& o, const BifInputs& in, BifDevice& R)
+// One sweep of the points [A.p0, A.p0 + A.n_points) into R with a peak store
+// of nchunks chunks; false when the store overflowed (R.h_counts then hold
+// every point's exact count).
+static bool sweep_pass(CallCtx& ctx, const chaos_options& o, const BifInputs& in, SweepArgs A, BifDevice& R,
+                       long long nchunks)
 {
-    SweepArgs A = make_bif_args(o, in);
     const long long n = A.n_points;
     R.n = n;
     R.hp = A.hp;
@@ -195,19 +243,54 @@ static void run_bif(CallCtx& ctx, const chaos_options& o, const BifInputs& in, B
     A.flags = R.flags.get();
     A.counts = R.counts.get();
     A.first = R.first.get();
-    long long nchunks = bif_initial_chunks(A);
-    for (int attempt = 0; attempt < 2; ++attempt) {
-        PeakPoolBuf pool;
-        pool.alloc(ctx.stream(), nchunks);
-        A.pool = pool.view();
-        ctx.begin();
-        // a map that clusters peak values only leaves the detector's time logic out
-        launch_sweep(ctx, o.system, in.writableVar, R.need_times ? KIND_BIF : KIND_BIF_VALS, A);
-        ctx.add_kernel_ms(ctx.end());
-        ctx.add_steps((double)n * ((double)A.S + (double)A.B * A.pre));
-        if (bif_post(ctx, R, pool)) return;
-        CHAOS_REQUIRE(attempt == 0, "peak store overflow after resize");
-        nchunks = bif_exact_chunks(R.h_counts, n);
+    PeakPoolBuf pool;
+    pool.alloc(ctx.stream(), nchunks);
+    A.pool = pool.view();
+    ctx.begin();
+    // a map that clusters peak values only leaves the detector's time logic out
+    launch_sweep(ctx, o.system, in.writableVar, R.need_times ? KIND_BIF : KIND_BIF_VALS, A);
+    ctx.add_kernel_ms(ctx.end());
+    ctx.add_steps((double)n * ((double)A.S + (double)A.B * A.pre));
+    return bif_post(ctx, R, pool);
+}
+
+// Sweeps the grid and hands each pass's results to use(R, p0), p0 the grid
+// index of the pass's first point.  One pass over the whole grid when the
+// first-guess peak store holds its peaks; otherwise a second pass with the
+// exact store: over the whole grid, or, with `ranges` and when that store
+// and its post stages need more than 60 % of the device memory left (or
+// than CHAOS_PEAK_STORE_BYTES), one pass per range of whole waves that fits
+// -- as the reference sizes its chunks of the grid by the free memory
+// (Bifurcation/src/bifurcationCUDA.cu:64-70).  R holds the last pass.
+template <class Use>
+static void run_bif(CallCtx& ctx, const chaos_options& o, const BifInputs& in, BifDevice& R, bool ranges, Use&& use)
+{
+    const SweepArgs A = make_bif_args(o, in);
+    const long long n = A.n_points;
+    if (sweep_pass(ctx, o, in, A, R, bif_initial_chunks(A))) {
+        use(R, 0LL);
+        return;
+    }
+    const std::vector<int> counts = R.h_counts;
+    R.flags.release();
+    R.counts.release();
+    R.first.release();
+    const size_t budget = std::min((size_t)(0.6 * (double)dev_mem_available()), peak_store_cap());
+    const std::vector<long long> starts = ranges ? exact_ranges(counts, n, budget) : std::vector<long long>{0, n};
+    for (size_t k = 0; k + 1 < starts.size(); ++k) {
+        const long long a = starts[k], b = starts[k + 1];
+        SweepArgs Ak = A;
+        Ak.p0 = a;
+        Ak.n_points = b - a;
+        const std::vector<int> sub(counts.begin() + a, counts.begin() + b);
+        BifDevice Rk;
+        BifDevice& dst = starts.size() == 2 ? R : Rk;
+        dst.need_times = R.need_times;
+        dst.need_vals = R.need_vals;
+        dst.need_host = R.need_host;
+        CHAOS_REQUIRE(sweep_pass(ctx, o, in, Ak, dst, bif_exact_chunks(sub, b - a)),
+                      "peak store overflow after resize");
+        use(dst, a);
     }
 }
 
@@ -973,7 +1056,8 @@ static void bif1d_impl(const chaos_options* opt, double tMax, int nPts, double h
                               preScaller, 0.0, 1, legacy_ic);
     CallCtx ctx(o);
     BifDevice R;
-    run_bif(ctx, o, in, R);
+    // the 1-D results are assembled from one pass over the whole grid
+    run_bif(ctx, o, in, R, false, [](BifDevice&, long long) {});
     bif1d_collect(ctx, R, counts, first, peaks, intervals, capacity, total);
     const long long k_out = *total;
     ctx.finish();
@@ -993,16 +1077,20 @@ static void bif2d_impl(const chaos_options* opt, double tMax, int nPts, double h
     R.need_times = o.semantics == CHAOS_SEM_REFERENCE;  // the CONFIG count reads peak values only
     R.need_vals = o.semantics != CHAOS_SEM_REFERENCE;   // the reference dbscan reads peak times only
     R.need_host = false;                                // the 2-D analysis lists its points on the device
-    run_bif(ctx, o, in, R);
+    const long long n = (long long)nPts * nPts;
     DevBuf<int> tmp;
     int* d_out = out;
-    if (!o.device_buffers) {
-        tmp.alloc((size_t)R.n);
-        d_out = tmp.get();
-    }
-    bif2d_analysis(ctx, o.semantics, R, eps, d_out);
+    // each pass (the whole grid, or a range of it when its peaks do not fit
+    // the device at once) writes its part of the map
+    run_bif(ctx, o, in, R, true, [&](BifDevice& Rk, long long p0) {
+        if (!o.device_buffers && !tmp.get()) {
+            tmp.alloc((size_t)n);
+            d_out = tmp.get();
+        }
+        bif2d_analysis(ctx, o.semantics, Rk, eps, d_out + p0);
+    });
     if (!o.device_buffers)
-        CHAOS_HIP(hipMemcpyAsync(out, d_out, R.n * sizeof(int), hipMemcpyDeviceToHost, ctx.stream()));
+        CHAOS_HIP(hipMemcpyAsync(out, d_out, n * sizeof(int), hipMemcpyDeviceToHost, ctx.stream()));
     ctx.sync();
     ctx.finish();
 }

@@ -27,7 +27,8 @@ void sweep_kernel(const SweepArgs A)
     const int wave = (int)(q >> 6);
     const bool in_range = q < A.n_points;
     SweepProg<SYS, KIND> L;
-    L.init(A, in_range ? q : 0, in_range);
+    // parameters from the grid index, outputs at the launch's own index q
+    L.init(A, A.p0 + (in_range ? q : 0), in_range);
     WaveCursor cur{A.pool.nchunks, kChunk};
     // every lane of the wave stays in the loop (wave-level peak appends)
     L.template ticks<WV>(A, WV, cur, wave, 0, SweepProg<SYS, KIND>::total(A));

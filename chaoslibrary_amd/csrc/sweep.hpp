@@ -31,7 +31,8 @@ struct SweepArgs {
     int idx0, idx1;       // swept index into values (Bifurcation) or X (Basins); idx1 < 0 for 1-D
     int nPts;             // grid side
     int dims;             // 1 or 2
-    long long n_points;   // nPts or nPts^2
+    long long n_points;   // points of this launch: nPts or nPts^2, or a range of them
+    long long p0;         // grid index of the launch's point 0 (a multiple of 64; outputs are indexed from it)
     int S;                // transient steps
     int B;                // recorded samples
     int pre;              // steps per sample (Basins; Bifurcation records every step)

@@ -86,3 +86,26 @@ def test_engine_overflow_reruns_job(chaos, oracle, kind):
                             0.0, T_T, LOR["values"], 1, 0.01)
         assert np.array_equal(got, want)
     srv.close()
+
+
+@pytest.mark.parametrize("sem", [0, 1])
+def test_bif2d_exact_pass_in_ranges_matches_oracle(chaos, oracle, monkeypatch, sem):
+    """When the exact peak store of the whole grid does not fit the device
+    (here: CHAOS_PEAK_STORE_BYTES caps it at 8 MB, about three 64-point waves
+    of these peaks), the second pass runs over ranges of whole waves, each
+    writing its part of the map.  The map equals the oracle's and the
+    one-range pass's; the ranges show as more launches."""
+    n = 24
+    ranges = [40.0, 80.0, 2.0, 3.0]
+    args = (T_MAX, n, H, LOR["ic"], ranges, [1, 2], 0, 0.0, T_T, LOR["values"], 1, 0.01)
+    print(f"bif2d exact pass in ranges sem={sem}", flush=True)
+    whole = chaos.Stats()
+    got_whole = chaos.bif2d(*args, system="lorenz_rk4", semantics=sem, stats=whole)
+    monkeypatch.setenv("CHAOS_PEAK_STORE_BYTES", str(8 << 20))
+    split = chaos.Stats()
+    got = chaos.bif2d(*args, system="lorenz_rk4", semantics=sem, stats=split)
+    want = oracle.bif2d(oracle.SYS["lorenz_rk4"], sem, *args)
+    assert _first_guess_overflows(whole.peaks, n * n)
+    assert split.peaks == whole.peaks and split.rk4_steps == whole.rk4_steps
+    assert split.launches > whole.launches
+    assert np.array_equal(got, want) and np.array_equal(got_whole, want)
