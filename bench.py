@@ -150,7 +150,7 @@ def timed(args, fn):
 def run_lle(args):
     import numpy as np
     import chaoslibrary_amd as ch
-    n = args.grid
+    n = args.grid or 2048
     w = dict(system="rossler_rk4", tMax=2000.0, NT=0.5, h=0.01, eps=1e-8, ic=[0.1, 0.1, 0.1],
              ranges=[0.05, 0.35, 2.0, 10.0], idx=[0, 2], transient=500.0, values=[0.2, 0.2, 5.7], seed=12345)
     stats = ch.Stats()
@@ -308,7 +308,7 @@ def run_bif1d(args):
 def run_bif2d(args):
     import chaoslibrary_amd as ch
     stats = ch.Stats()
-    n = args.grid if args.grid != 2048 else 1024
+    n = args.grid or 1024
     sem = 1 if args.semantics is None else args.semantics
     dptr = C.c_void_p()
     ch._check(ch.lib.chaos_device_malloc(C.byref(dptr), n * n * 4, 0))
@@ -340,7 +340,7 @@ def run_bif2d(args):
 def run_basins(args):
     import chaoslibrary_amd as ch
     stats = ch.Stats()
-    n = args.grid
+    n = args.grid or 2048
     sem = 1 if args.semantics is None else args.semantics
     dptr = C.c_void_p()
     ch._check(ch.lib.chaos_device_malloc(C.byref(dptr), n * n * 4, 0))
@@ -375,7 +375,7 @@ def run_ls(args):
     """Legacy LS2D: Lyapunov spectrum of Lorenz over (rho, beta), 4 trajectories per point."""
     import numpy as np
     import chaoslibrary_amd as ch
-    n = args.grid if args.grid != 2048 else 1024
+    n = args.grid or 1024
     stats = ch.Stats()
     dptr = C.c_void_p()
     ch._check(ch.lib.chaos_device_malloc(C.byref(dptr), n * n * 3 * 8, 0))
@@ -647,7 +647,8 @@ def main():
                                                            "regression", "tune"])
     ap.add_argument("--tune-out", default=os.path.join("workspace", "engine_tune.txt"),
                     help="tune: where the autotuner writes its cache (chosen parameters + candidates)")
-    ap.add_argument("--grid", type=int, default=2048)
+    ap.add_argument("--grid", type=int, default=0,
+                    help="grid side (default: 2048 for lle and basins, 1024 for bif2d and ls)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")

@@ -51,10 +51,21 @@ __global__ void scatter_kernel(const PeakPool pool, unsigned used, const int* __
 
 void PeakPoolBuf::alloc(hipStream_t st, long long chunks)
 {
-    nchunks = chunks;
     entries.alloc((size_t)chunks * kChunk);
     meta.alloc((size_t)(2 * chunks + 2));
+    cap = nchunks = chunks;
     // head and overflow start at zero
+    CHAOS_HIP(hipMemsetAsync(meta.get() + 2 * chunks, 0, 2 * sizeof(int), st));
+}
+
+void PeakPoolBuf::prepare(hipStream_t st, long long chunks)
+{
+    if (!meta.get() || chunks > cap) {
+        alloc(st, chunks);
+        return;
+    }
+    nchunks = chunks;
+    // head and overflow (after this store's owner and fill words) start at zero
     CHAOS_HIP(hipMemsetAsync(meta.get() + 2 * chunks, 0, 2 * sizeof(int), st));
 }
 
@@ -123,18 +134,31 @@ static size_t peak_store_cap()
     return v > 0 ? (size_t)v : SIZE_MAX;
 }
 
+// First guess of the peaks per point: one per 32 samples (at most one per
+// two is possible); an overflow is detected on the device and the sweep
+// rerun with the exact per-wave need.
+static double first_guess_per_point(const SweepArgs& A)
+{
+    return std::min((double)A.B / 2.0 + 1.0, std::max(256.0, (double)A.B / 32.0));
+}
+
+// Chunks the first-guess store may take: a sixth of the device memory left
+// (47 GB on a 288 GB device: a 1024^2 peak-count map in one pass, 2048^2 in
+// three), or CHAOS_PEAK_STORE_BYTES.  A store that size stays in the device
+// allocator's cache (a quarter of the device) beside the values scattered
+// out of it, so repeated calls reuse it: mapping a 70 GB store anew took
+// 1.2-1.8 s per call.
+static long long first_guess_budget_chunks()
+{
+    const size_t budget = std::min((size_t)((double)dev_mem_available() / 6.0), peak_store_cap());
+    return (long long)(budget / kChunkBytes);
+}
+
 long long bif_initial_chunks(const SweepArgs& A)
 {
     const long long n = A.n_points, waves = (n + 63) / 64;
-    const size_t free_b = dev_mem_available();
-    // First guess: one peak per 32 samples (at most one per two is possible);
-    // an overflow is detected on the device and the sweep rerun with the
-    // exact per-wave need.
-    const double max_pp = std::min((double)A.B / 2.0 + 1.0, std::max(256.0, (double)A.B / 32.0));
-    const size_t budget = std::min((size_t)(0.25 * (double)free_b), peak_store_cap());
-    const long long budget_chunks = (long long)(budget / kChunkBytes);
-    const long long want = (long long)std::ceil((double)n * max_pp / (kChunk - 64)) + waves;
-    return std::max<long long>(waves, std::min(want, budget_chunks));
+    const long long want = (long long)std::ceil((double)n * first_guess_per_point(A) / (kChunk - 64)) + waves;
+    return std::max<long long>(waves, std::min(want, first_guess_budget_chunks()));
 }
 
 // Exact store of one 64-point wave holding s peaks: chunks of kChunk - 64
@@ -228,10 +252,10 @@ bool bif_post(CallCtx& ctx, BifDevice& R, PeakPoolBuf& pool)
 }
 
 // One sweep of the points [A.p0, A.p0 + A.n_points) into R with a peak store
-// of nchunks chunks; false when the store overflowed (R.h_counts then hold
-// every point's exact count).
+// of nchunks chunks (in `pool`, which the call's passes share); false when
+// the store overflowed (R.h_counts then hold every point's exact count).
 static bool sweep_pass(CallCtx& ctx, const chaos_options& o, const BifInputs& in, SweepArgs A, BifDevice& R,
-                       long long nchunks)
+                       long long nchunks, PeakPoolBuf& pool)
 {
     const long long n = A.n_points;
     R.n = n;
@@ -243,8 +267,7 @@ static bool sweep_pass(CallCtx& ctx, const chaos_options& o, const BifInputs& in
     A.flags = R.flags.get();
     A.counts = R.counts.get();
     A.first = R.first.get();
-    PeakPoolBuf pool;
-    pool.alloc(ctx.stream(), nchunks);
+    pool.prepare(ctx.stream(), nchunks);
     A.pool = pool.view();
     ctx.begin();
     // a map that clusters peak values only leaves the detector's time logic out
@@ -255,42 +278,79 @@ static bool sweep_pass(CallCtx& ctx, const chaos_options& o, const BifInputs& in
 }
 
 // Sweeps the grid and hands each pass's results to use(R, p0), p0 the grid
-// index of the pass's first point.  One pass over the whole grid when the
-// first-guess peak store holds its peaks; otherwise a second pass with the
-// exact store: over the whole grid, or, with `ranges` and when that store
-// and its post stages need more than 60 % of the device memory left (or
-// than CHAOS_PEAK_STORE_BYTES), one pass per range of whole waves that fits
-// -- as the reference sizes its chunks of the grid by the free memory
-// (Bifurcation/src/bifurcationCUDA.cu:64-70).  R holds the last pass.
+// index of the pass's first point; R holds the last pass.
+//   * First passes: the whole grid; or, with `ranges` and when the
+//     first-guess store of the whole grid exceeds its budget (a quarter of
+//     the device memory left, or CHAOS_PEAK_STORE_BYTES), ranges of whole
+//     waves whose first guesses fit.
+//   * A first pass whose store overflowed runs again with the exact store:
+//     over its whole range, or, with `ranges` and when that store and its
+//     post stages need more than 60 % of the device memory left (or than
+//     CHAOS_PEAK_STORE_BYTES), over ranges of whole waves that each fit.
+// The grid is cut by the memory as the reference sizes its chunks of the
+// grid by the free memory (Bifurcation/src/bifurcationCUDA.cu:64-70).
 template <class Use>
 static void run_bif(CallCtx& ctx, const chaos_options& o, const BifInputs& in, BifDevice& R, bool ranges, Use&& use)
 {
     const SweepArgs A = make_bif_args(o, in);
-    const long long n = A.n_points;
-    if (sweep_pass(ctx, o, in, A, R, bif_initial_chunks(A))) {
-        use(R, 0LL);
-        return;
+    const long long n = A.n_points, waves = (n + 63) / 64;
+    std::vector<long long> first{0, n};
+    if (ranges) {
+        const double wave_guess = 64.0 * first_guess_per_point(A) / (kChunk - 64) + 1.0;
+        const long long per = std::max(1LL, (long long)((double)first_guess_budget_chunks() / wave_guess));
+        if (per < waves) {
+            first.assign(1, 0);
+            for (long long w = per; w < waves; w += per) first.push_back(64 * w);
+            first.push_back(n);
+        }
     }
-    const std::vector<int> counts = R.h_counts;
-    R.flags.release();
-    R.counts.release();
-    R.first.release();
-    const size_t budget = std::min((size_t)(0.6 * (double)dev_mem_available()), peak_store_cap());
-    const std::vector<long long> starts = ranges ? exact_ranges(counts, n, budget) : std::vector<long long>{0, n};
-    for (size_t k = 0; k + 1 < starts.size(); ++k) {
-        const long long a = starts[k], b = starts[k + 1];
+    const bool single = first.size() == 2;
+    PeakPoolBuf pool;  // one store for every pass, grown when a pass needs more
+    // R for a pass over the whole grid, else a record of the pass's own
+    auto record = [&](BifDevice& own, bool whole) -> BifDevice& {
+        if (whole) return R;
+        own.need_times = R.need_times;
+        own.need_vals = R.need_vals;
+        own.need_host = R.need_host;
+        return own;
+    };
+    auto args = [&](long long a, long long b) {
         SweepArgs Ak = A;
         Ak.p0 = a;
         Ak.n_points = b - a;
-        const std::vector<int> sub(counts.begin() + a, counts.begin() + b);
-        BifDevice Rk;
-        BifDevice& dst = starts.size() == 2 ? R : Rk;
-        dst.need_times = R.need_times;
-        dst.need_vals = R.need_vals;
-        dst.need_host = R.need_host;
-        CHAOS_REQUIRE(sweep_pass(ctx, o, in, Ak, dst, bif_exact_chunks(sub, b - a)),
-                      "peak store overflow after resize");
-        use(dst, a);
+        return Ak;
+    };
+    for (size_t k = 0; k + 1 < first.size(); ++k) {
+        const long long a = first[k], b = first[k + 1];
+        BifDevice own;
+        BifDevice& dst = record(own, single);
+        const SweepArgs Ak = args(a, b);
+        if (sweep_pass(ctx, o, in, Ak, dst, bif_initial_chunks(Ak), pool)) {
+            use(dst, a);
+            continue;
+        }
+        // exact passes over this range from its per-point counts
+        const std::vector<int> counts = dst.h_counts;
+        dst.flags.release();
+        dst.counts.release();
+        dst.first.release();
+        pool = PeakPoolBuf();  // the exact passes size the store anew
+        const long long m = b - a;
+        const size_t budget = std::min((size_t)(0.6 * (double)dev_mem_available()), peak_store_cap());
+        const std::vector<long long> cuts = ranges ? exact_ranges(counts, m, budget) : std::vector<long long>{0, m};
+        std::vector<long long> need(cuts.size() - 1);
+        for (size_t j = 0; j + 1 < cuts.size(); ++j)
+            need[j] = bif_exact_chunks(std::vector<int>(counts.begin() + cuts[j], counts.begin() + cuts[j + 1]),
+                                       cuts[j + 1] - cuts[j]);
+        pool.prepare(ctx.stream(), *std::max_element(need.begin(), need.end()));
+        for (size_t j = 0; j + 1 < cuts.size(); ++j) {
+            const long long c = cuts[j], d = cuts[j + 1];
+            BifDevice own2;
+            BifDevice& dst2 = record(own2, single && cuts.size() == 2);
+            CHAOS_REQUIRE(sweep_pass(ctx, o, in, args(a + c, a + d), dst2, need[j], pool),
+                          "peak store overflow after resize");
+            use(dst2, a + c);
+        }
     }
 }
 

@@ -41,7 +41,12 @@ struct PeakPoolBuf {
     DevBuf<PeakEntry> entries;
     DevBuf<int> meta;  // PeakPool::meta
     long long nchunks = 0;
+    long long cap = 0;  // chunks the buffers hold
     void alloc(hipStream_t st, long long chunks);
+    // A store of `chunks` for the next sweep: the buffers of an earlier one
+    // when they are large enough (a grid swept in ranges allocates its store
+    // once), else new ones.
+    void prepare(hipStream_t st, long long chunks);
     PeakPool view() const;
 };
 
