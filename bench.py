@@ -183,6 +183,9 @@ def run_lle(args):
     res["pct_fp64_peak"] = 100.0 * res["tflop_per_s"] / 78.6
     if not args.no_cpu:
         log("cpu baseline ...")
+        # threads spread over the cores (OpenMP reads this when the oracle
+        # library loads its runtime); a caller's own setting wins
+        os.environ.setdefault("OMP_PROC_BIND", "spread")
         O = oracle()
         ic = np.asarray(w["ic"])
         rg = np.asarray(w["ranges"])
@@ -211,7 +214,8 @@ def run_lle(args):
             "value": rows * n * steps_pt / dt, "unit": "RK4 steps/s", "cores": used, "kind": "port",
             "sample": f"{rows} rows (every {stride}th) x {n} points of the same map = {rows * n} points, "
                       f"{steps_pt} RK4 steps each; median of 3 runs "
-                      f"({', '.join(f'{d:.2f}' for d in dts)} s) on {used} OpenMP threads",
+                      f"({', '.join(f'{d:.2f}' for d in dts)} s) on {used} OpenMP threads, "
+                      f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}",
             **cpu_info()}
     if not args.no_stream:
         res["stream"] = stream_supplement(args)
