@@ -854,9 +854,9 @@ __global__ __launch_bounds__(256) void core_sparse_kernel(const GridDev G, int m
                 int w = base + off;
                 for (int j = 0; j < 2; ++j) {
                     if (rl[j] != 1) continue;
-                    const int c0 = G.hstart[sl[j]], e = c0 + (int)G.hcount[sl[j]];
-                    for (int k = c0 + kHeadMembers; k < e; k += kPieceMembers)
-                        G.pieces[w++] = make_int4(p, k, min(k + kPieceMembers, e), 0);
+                    const long long c0 = G.hstart[sl[j]], e = c0 + (long long)G.hcount[sl[j]];
+                    for (long long k = c0 + kHeadMembers; k < e; k += kPieceMembers)
+                        G.pieces[w++] = make_int4(p, (int)k, (int)(k + kPieceMembers < e ? k + kPieceMembers : e), 0);
                 }
                 if (lane == 0) G.labroot[p] = (int)cnt;
                 continue;
@@ -885,12 +885,12 @@ __global__ __launch_bounds__(256) void core_piece_kernel(const GridDev G, int mi
     for (long long i = wave; i < np; i += nw) {
         const int4 pc = G.pieces[i];
         const int p = pc.x;
-        for (int k = pc.y; k < pc.z; k += 64 * kScanU) {
+        for (long long k = pc.y; k < pc.z; k += 64 * kScanU) {
             if (__builtin_amdgcn_readfirstlane(uf_load(&G.labroot[p])) >= minPts) break;
             int q[kScanU];
 #pragma unroll
             for (int u = 0; u < kScanU; ++u) {
-                const int kk = k + 64 * u + lane;
+                const long long kk = k + 64 * u + lane;
                 q[u] = kk < pc.z ? G.cellpts[kk] : -1;
             }
             int hits = 0;
