@@ -20,6 +20,7 @@
 
 #include "csv.hpp"
 #include "dispatch.hpp"
+#include "peak_ranges.hpp"
 #include "scan.hpp"
 #include "bifurcation.hpp"
 #include "sweep.hpp"
@@ -161,43 +162,26 @@ long long bif_initial_chunks(const SweepArgs& A)
     return std::max<long long>(waves, std::min(want, first_guess_budget_chunks()));
 }
 
-// Exact store of one 64-point wave holding s peaks: chunks of kChunk - 64
-// usable entries plus one for the chunk switch.
-static long long wave_chunks(long long s) { return (s + (kChunk - 64) - 1) / (kChunk - 64) + 1; }
-
 long long bif_exact_chunks(const std::vector<int>& counts, long long n)
 {
     long long need = 0;
     for (long long w = 0; w < n; w += 64) {
         long long s = 0;
         for (long long p = w; p < std::min(n, w + 64); ++p) s += counts[p];
-        need += wave_chunks(s);
+        need += wave_chunks(s, kChunk);
     }
     return need;
 }
 
-// Ranges of whole 64-point waves whose exact passes each fit `budget`
-// device bytes: the peak store, 32 bytes per stored peak (scattered values
-// and times, the 2-D analysis's scratch) and 128 per point (flags, counts,
-// offsets, class lists).  Returns the range starts and n: [s[k], s[k + 1]).
-static std::vector<long long> exact_ranges(const std::vector<int>& counts, long long n, size_t budget)
+// Exact passes over ranges of whole waves that each fit `budget`
+// (exact_ranges, peak_ranges.hpp); fails when one wave alone does not.
+static std::vector<long long> exact_pass_ranges(const std::vector<int>& counts, long long n, size_t budget)
 {
-    std::vector<long long> starts{0};
-    size_t bytes = 0;
-    for (long long w = 0; w < n; w += 64) {
-        const long long e = std::min(n, w + 64);
-        long long s = 0;
-        for (long long p = w; p < e; ++p) s += counts[p];
-        const size_t wb = (size_t)wave_chunks(s) * kChunkBytes + 32 * (size_t)s + 128 * (size_t)(e - w);
-        CHAOS_REQUIRE(wb <= budget, "the peaks of 64 points need %zu bytes of device memory, more than the %zu "
-                      "available", wb, budget);
-        if (bytes + wb > budget) {
-            starts.push_back(w);
-            bytes = 0;
-        }
-        bytes += wb;
-    }
-    starts.push_back(n);
+    size_t wave_bytes = 0;
+    std::vector<long long> starts = exact_ranges(counts.data(), n, budget, kChunk, kChunkBytes, &wave_bytes);
+    if (starts.empty())
+        fail(CHAOS_E_NOMEM, "the peaks of 64 points need %zu bytes of device memory, more than the %zu available",
+             wave_bytes, budget);
     return starts;
 }
 
@@ -293,17 +277,10 @@ template <class Use>
 static void run_bif(CallCtx& ctx, const chaos_options& o, const BifInputs& in, BifDevice& R, bool ranges, Use&& use)
 {
     const SweepArgs A = make_bif_args(o, in);
-    const long long n = A.n_points, waves = (n + 63) / 64;
-    std::vector<long long> first{0, n};
-    if (ranges) {
-        const double wave_guess = 64.0 * first_guess_per_point(A) / (kChunk - 64) + 1.0;
-        const long long per = std::max(1LL, (long long)((double)first_guess_budget_chunks() / wave_guess));
-        if (per < waves) {
-            first.assign(1, 0);
-            for (long long w = per; w < waves; w += per) first.push_back(64 * w);
-            first.push_back(n);
-        }
-    }
+    const long long n = A.n_points;
+    const double wave_guess = 64.0 * first_guess_per_point(A) / (kChunk - 64) + 1.0;
+    const std::vector<long long> first = ranges ? first_pass_cuts(n, wave_guess, first_guess_budget_chunks())
+                                                : std::vector<long long>{0, n};
     const bool single = first.size() == 2;
     PeakPoolBuf pool;  // one store for every pass, grown when a pass needs more
     // R for a pass over the whole grid, else a record of the pass's own
@@ -337,7 +314,7 @@ static void run_bif(CallCtx& ctx, const chaos_options& o, const BifInputs& in, B
         pool = PeakPoolBuf();  // the exact passes size the store anew
         const long long m = b - a;
         const size_t budget = std::min((size_t)(0.6 * (double)dev_mem_available()), peak_store_cap());
-        const std::vector<long long> cuts = ranges ? exact_ranges(counts, m, budget) : std::vector<long long>{0, m};
+        const std::vector<long long> cuts = ranges ? exact_pass_ranges(counts, m, budget) : std::vector<long long>{0, m};
         std::vector<long long> need(cuts.size() - 1);
         for (size_t j = 0; j + 1 < cuts.size(); ++j)
             need[j] = bif_exact_chunks(std::vector<int>(counts.begin() + cuts[j], counts.begin() + cuts[j + 1]),
