@@ -528,9 +528,14 @@ int chaos_stream_run(int device, const chaos_stream_config* cfg,
                      const chaos_engine_params* p, chaos_stream_report* rep,
                      double* latencies_ms /* [jobs] or NULL */, int latencies_cap);
 
-/* Autotuner: times a fixed calibration mix over block size, waves per SIMD
- * and steps per launch; writes the fastest to *best and, if cache_path is
- * not NULL, to that file (read back by chaos_autotune_load). */
+/* Autotuner: a coordinate search over steps per launch, points per tile,
+ * waves per SIMD, block size and graph replay.  Each candidate serves the
+ * same short seeded stream (seed 7, 200 jobs/s for 1.5 s, the served kinds at
+ * full horizons) and is scored by its power, RK4 steps/s over the mean job
+ * latency; candidates within 1 % of the best power are ties, and the one of
+ * them with the highest rate wins.  Writes it to *best (*best_steps_per_s:
+ * its rate) and, if cache_path is not NULL, to that file with every
+ * candidate's figures (read back by chaos_autotune_load). */
 int chaos_autotune(int device, const char* cache_path, chaos_engine_params* best,
                    double* best_steps_per_s);
 int chaos_autotune_load(const char* cache_path, chaos_engine_params* p);
