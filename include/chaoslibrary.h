@@ -387,13 +387,16 @@ int chaos_TimeDomainCalculation(const chaos_options* opt, double tMax, int nPts,
 
 /* ---------------- Batched engine and job server ------------------------- */
 /* No counterpart in the reference (one synchronous call per sweep, one
- * system per build).  Jobs of any of the four kinds run together in one
- * persistent kernel that pulls (job, 64-point group, tick range) tiles from
- * a device queue; lane state stays in HBM between launches; the launch
- * sequence is replayed from a captured hipGraph.  Jobs are admitted FIFO as
- * job slots and the HBM budget allow (continuous batching).  The engine runs
- * the RK4 systems (CHAOS_SYS_ROSSLER_RK4, CHAOS_SYS_LORENZ_RK4); results are
- * bit-identical to the one-shot entry points for the same job. */
+ * system per build).  Jobs of any of the four kinds run together: every
+ * round launches one persistent kernel per job family (program x system x
+ * recorded variable), each on its own stream, whose wavefronts pull (job,
+ * 64-point group, tick range) tiles from the family's device queue; lane
+ * state stays in HBM between rounds; a round's launch sequence is replayed
+ * from a hipGraph captured per set of families.  Jobs are admitted FIFO as
+ * job slots, the HBM budget and the tile queue allow (continuous batching),
+ * and finished by two worker threads while later rounds integrate.  The
+ * engine runs the RK4 systems (CHAOS_SYS_ROSSLER_RK4, CHAOS_SYS_LORENZ_RK4);
+ * results are bit-identical to the one-shot entry points for the same job. */
 typedef enum chaos_job_kind {
     CHAOS_JOB_LLE2D    = 0,   /* = chaos_lle2d      */
     CHAOS_JOB_BIF1D    = 1,   /* = chaos_bif1d      */
@@ -419,7 +422,8 @@ typedef struct chaos_job {
 
 typedef struct chaos_engine_params {
     int block;              /* threads per persistent workgroup (64|128|256)  */
-    int waves_per_simd;     /* persistent waves per SIMD (1..8)               */
+    int waves_per_simd;     /* persistent waves per SIMD (1..8); a family with
+                               fewer claims runs one wave per claim          */
     int steps_per_launch;   /* RK4 steps a tile advances per launch           */
     int max_jobs;           /* active job slots                               */
     int max_tiles;          /* tile queue capacity per launch                 */
@@ -431,7 +435,7 @@ typedef struct chaos_engine_params {
 
 typedef struct chaos_job_info {
     int status;             /* 0 pending, 1 integrating, 3 finishing (post
-                               stages on the server's worker thread),
+                               stages on one of the server's worker threads),
                                2 done, -1 failed                             */
     int kind;
     double t_submit_ms, t_admit_ms, t_done_ms;   /* server clock              */
@@ -452,7 +456,7 @@ typedef struct chaos_server_stats {
     double rk4_steps;       /* steps completed                                */
     int active_jobs, pending_jobs;
     long long hbm_in_use, hbm_budget;
-    int finishing_jobs;     /* handed to the worker thread, not yet done      */
+    int finishing_jobs;     /* handed to the worker threads, not yet done     */
     int reserved;
 } chaos_server_stats;
 
@@ -464,8 +468,8 @@ int chaos_server_create(int device, const chaos_engine_params* p, void** server)
 int chaos_server_destroy(void* server);
 int chaos_server_submit(void* server, const chaos_job* job, long long* job_id);
 /* One scheduling round: admit, launch every active job's next tiles and hand
- * jobs whose integration ended to the server's worker thread, which runs
- * their post stages on its own stream while later rounds integrate.
+ * jobs whose integration ended to the server's worker threads, which run
+ * their post stages on their own streams while later rounds integrate.
  * *completed receives the number of jobs that became done (or failed) since
  * the previous call.  drain() returns once every job is done or failed. */
 int chaos_server_step(void* server, int* completed);
