@@ -187,7 +187,10 @@ int chaos_bifurcation1D(const chaos_options* opt, double tMax, int nPts,
 /* 2-D: out[j*nPts + i] (i over ranges[0..1], j over ranges[2..3]) is -1 for
  * a fixed point, 0 for a divergent or peak-less point, else the cluster
  * count (REFERENCE: the reference's per-point dbscan on intervals; CONFIG:
- * number of 1-D DBSCAN(eps, minPts=1) clusters of the peak values). */
+ * number of 1-D DBSCAN(eps, minPts=1) clusters of the peak values).  A grid
+ * whose peak store does not fit the device at once runs in ranges of points
+ * that do, with the same result; the environment variable
+ * CHAOS_PEAK_STORE_BYTES, read at each call, caps the store's bytes. */
 int chaos_bif2d(const chaos_options* opt, double tMax, int nPts, double h,
                 int amountOfInitialConditions, const double* initialConditions,
                 const double* ranges, const int* indicesOfMutVars,
