@@ -63,7 +63,16 @@ double divergence_fast_bound(int check, double maxValue)
 
 void launch_sweep(CallCtx& ctx, int system, int writableVar, int kind, const SweepArgs& A)
 {
-    const int block = 256;
+    // A grid of at most one wavefront per CU (a 1-D sweep of 1000 points: 16
+    // wavefronts) runs them one per CU in 64-thread blocks, so that a
+    // latency-bound wavefront does not share its CU's scalar unit and
+    // instruction issue with three others: 31.9 -> 28.3 ms for that sweep.
+    // (Grids of a few wavefronts per CU gained nothing and sometimes lost.)
+    // The basins epilogue's moments need the 256-thread blocks.
+    const bool spread = A.zsum == nullptr &&
+                        ceil_div((unsigned long long)A.n_points, 64) <=
+                            (unsigned long long)device_props(ctx.device()).multiProcessorCount;
+    const int block = spread ? 64 : 256;
     // a whole number of waves so every wave's lanes share one control flow
     const unsigned grid = ceil_div((unsigned long long)A.n_points, block);
     int nx = 0, na = 0;
