@@ -5,9 +5,10 @@
 // into families (program x system x recorded variable); every family has its
 // own persistent kernel, compiled alone so that it keeps the one-shot
 // kernels' register budget, and its own queue in the round's tile list.
-// Waves pull tiles from their family's queue (one atomic per tile), resume
-// the groups' lane state from HBM, advance it, and park it again (or write
-// the job's outputs when its last tick is done).  The per-lane programs are
+// The family's wavefronts (as many as it has claims, at most its persistent
+// grid) pull tiles from their family's queue (one atomic per claim of
+// `take` tiles), resume the groups' lane state from HBM, advance it, and
+// park it again (or write the job's outputs when its last tick is done).  The per-lane programs are
 // the ones the one-shot kernels run (lanes.hpp), so a job's results do not
 // depend on how it was batched.  The host side admits jobs FIFO while job
 // slots and the HBM budget allow, replays each round from a hipGraph
