@@ -121,6 +121,7 @@ public:
     chaos_server_stats stats() const;
     double now_ms() const;
     bool idle() const { return pending_.empty() && active_.empty() && finishing_.load() == 0; }
+    int take_completed();            // jobs finished (or failed) since the last call or step()
 
 private:
     void plan(JobRec& r);
@@ -132,7 +133,6 @@ private:
     void launch_family(int f, hipStream_t s);
     void fail_active(const std::string& msg);
     void finisher(int k);            // body of worker thread k
-    int take_completed();            // jobs finished since the last call
     void wait_finisher_progress();   // block until a finishing job completes
 
 

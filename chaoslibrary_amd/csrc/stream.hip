@@ -246,11 +246,20 @@ void stream_run(int device, const chaos_stream_config& c, const chaos_engine_par
                 next_report += c.progress_s * 1000.0;
             }
             while (next < N && jobs[next].arrival_ms <= now) ids[next] = eng.submit(jobs[next].job), ++next;
+            int k = 0;
             if (eng.idle()) {
-                if (next < N) sleep_until_ms(t_start + jobs[next].arrival_ms);
-                continue;
+                // Nothing to integrate.  A finishing thread may have
+                // completed jobs since the last step: count them first --
+                // skipping them here left the last job of a stream uncounted
+                // and this loop waiting for it for ever.
+                k = eng.take_completed();
+                if (k == 0) {
+                    if (next < N) sleep_until_ms(t_start + jobs[next].arrival_ms);
+                    continue;
+                }
+            } else {
+                k = eng.step();
             }
-            const int k = eng.step();
             if (k > 0) {
                 const double t = clock_ms() - t_start;
                 for (size_t i = 0; i < next; ++i) {
