@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "serve_loop.hpp"
 
 namespace chaos {
 
@@ -234,47 +235,17 @@ void stream_run(int device, const chaos_stream_config& c, const chaos_engine_par
     double t_start = 0.0;
     if (c.mode == 0) {
         Engine eng(device, prm);
-        std::vector<long long> ids(N, -1);
-        size_t next = 0, finished = 0;
-        t_start = clock_ms();
-        double next_report = c.progress_s > 0 ? c.progress_s * 1000.0 : INFINITY;
-        while (finished < N) {
-            const double now = clock_ms() - t_start;
-            if (now >= next_report) {
-                std::fprintf(stderr, "stream engine: %.0f s, %zu of %zu jobs arrived, %zu done\n", now / 1000.0,
-                             next, N, finished);
-                next_report += c.progress_s * 1000.0;
-            }
-            while (next < N && jobs[next].arrival_ms <= now) ids[next] = eng.submit(jobs[next].job), ++next;
-            int k = 0;
-            if (eng.idle()) {
-                // Nothing to integrate.  A finishing thread may have
-                // completed jobs since the last step: count them first --
-                // skipping them here left the last job of a stream uncounted
-                // and this loop waiting for it for ever.
-                k = eng.take_completed();
-                if (k == 0) {
-                    if (next < N) sleep_until_ms(t_start + jobs[next].arrival_ms);
-                    continue;
-                }
-            } else {
-                k = eng.step();
-            }
-            if (k > 0) {
-                const double t = clock_ms() - t_start;
-                for (size_t i = 0; i < next; ++i) {
-                    if (ids[i] < 0 || done_ms[i] != 0.0) continue;
-                    const JobRec& r = eng.job(ids[i]);
-                    if (r.status == -1) fail(CHAOS_E_HIP, "stream job %zu failed: %s", i, r.error.c_str());
-                    if (r.status == 2) {
-                        done_ms[i] = t;
-                        eng.release(ids[i]);
-                        ids[i] = -1;
-                        ++finished;
-                    }
-                }
-            }
+        std::vector<chaos_job> js(N);
+        std::vector<double> arr(N);
+        for (size_t i = 0; i < N; ++i) {
+            js[i] = jobs[i].job;
+            arr[i] = jobs[i].arrival_ms;
         }
+        t_start = clock_ms();
+        serve_stream(
+            eng, js, arr, done_ms, [&] { return clock_ms() - t_start; },
+            [&](double t) { sleep_until_ms(t_start + t); }, c.progress_s,
+            [](size_t i, const std::string& err) { fail(CHAOS_E_HIP, "stream job %zu failed: %s", i, err.c_str()); });
         const chaos_server_stats st = eng.stats();
         rep.busy_ms = st.busy_ms;
         rep.launches = st.launches;
